@@ -1,0 +1,86 @@
+"""ctypes binding of libaxr.so (the C ABI in include/axr.h).
+
+The library is built in-tree by ``axi_visualizer_amd._build.build_native``.
+There is no fallback: if the library is missing or cannot be loaded this
+module raises, so nothing silently runs a different path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libaxr.so")
+
+_f = ctypes.c_float
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_p = ctypes.c_void_p
+_ctx = ctypes.c_void_p
+
+# name -> (restype, argtypes); every function declared in include/axr.h.
+SIGNATURES = {
+    "axr_last_error": (ctypes.c_char_p, []),
+    "axr_create": (_i, [_i, _i, ctypes.POINTER(_ctx)]),
+    "axr_destroy": (_i, [_ctx]),
+    "axr_set_scale_cull": (_i, [_ctx, _f, _f, _f]),
+    "axr_add_texture_group": (_i, [_ctx, _p, _p, _p, _i, _p, _p, _p, _i64, _i, _i,
+                                   ctypes.POINTER(_i)]),
+    "axr_set_group_material": (_i, [_ctx, _i, _f, _i]),
+    "axr_set_primary_light": (_i, [_ctx, _p, _p]),
+    "axr_vert_light": (_i, [_ctx, _p, _i, _p, _p, _i, _p, _p, _i, _p, _p, _p, _i, _f]),
+    "axr_set_pos": (_i, [_ctx, _p]),
+    "axr_set_vm": (_i, [_ctx, _p]),
+    "axr_clear_zbuffer": (_i, [_ctx]),
+    "axr_set_clear_color": (_i, [_ctx, _p]),
+    "axr_draw_all": (_i, [_ctx, _p, _p, _i, _p, _i, _i]),
+    "axr_add_shadow_map": (_i, [_ctx, _i, _i, _f, _f]),
+    "axr_place_shadow_map": (_i, [_ctx, _i, _p, _p, _f]),
+    "axr_clear_shadow_map": (_i, [_ctx, _i]),
+    "axr_shadow_map": (_i, [_ctx, _i, _p, _i, _f]),
+    "axr_gamma": (_i, [_ctx, _f]),
+    "axr_get_frame": (_i, [_ctx, _p, _p, _p, _p]),
+    "axr_get_rgba8": (_i, [_ctx, _p]),
+    "axr_get_ids": (_i, [_ctx, _p]),
+    "axr_get_shadow_map": (_i, [_ctx, _i, _p]),
+    "axr_sync": (_i, [_ctx]),
+    "axr_get_stats": (_i, [_ctx, _p]),
+    "axr_set_timing": (_i, [_ctx, _i]),
+    "axr_get_timing": (_i, [_ctx, _p, _i, _p, _p, _i]),
+}
+
+_lib = None
+
+
+class AxrError(RuntimeError):
+    """A libaxr call returned an error code."""
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str) -> int:
+    if rc < 0:
+        msg = lib().axr_last_error()
+        raise AxrError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+    return rc
+
+
+def ptr(a) -> int:
+    """Address of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data

@@ -1,0 +1,807 @@
+// axr_api.cpp -- C ABI of libaxr.so (declared in include/axr.h).
+//
+// One context = one render target, its scene groups and shadow maps, and one
+// HIP stream.  Per-frame work is enqueued without host synchronisation:
+// triangle counts stay on the device (grid-stride kernels read them), group
+// flags travel as kernel arguments, and colour is materialised lazily -- the
+// depth pass leaves per-pixel winner keys, and the first call that needs
+// colour (gamma or a readback) runs one resolve+shade launch, with gamma
+// fused when gamma is what asked for it.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/axr.h"
+#include "axr_launch.h"
+
+using namespace axr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) return fail(AXR_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct HostGroup {
+  uint32_t base = 0, ntri = 0;
+  int tex_size = 0;
+  int is_mip = 0;
+  int64_t ntexels = 0;
+  float mip_bias = 0.0f;
+  int spec_pow = 16;
+  ushort4* d_tex = nullptr;
+};
+
+struct HostShadow {
+  bool exists = false;
+  bool placed = false;
+  int dim = 0;
+  float scale = 0.0f;
+  AxrCam cam{};
+  uint32_t* d_map = nullptr;
+};
+
+enum State {
+  ST_UNCLEARED,  // keys hold nothing usable yet
+  ST_CLEARED,    // keys clear, nothing drawn
+  ST_PENDING,    // keys hold winners; colour not materialised
+  ST_HDR,        // u16 planes hold HDR colour
+  ST_LDR8,       // rgba8 holds gamma'd colour
+  ST_LDR16       // u16 planes hold gamma'd colour
+};
+
+struct TimingRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+constexpr uint32_t kDefaultBinCap = 8u << 20;
+constexpr int kCounters = 8;
+
+}  // namespace
+
+struct axr_ctx {
+  int device = 0;
+  int W = 0, H = 0, WC = 0, HC = 0, NT = 0;
+  hipStream_t stream = nullptr;
+
+  std::vector<HostGroup> groups;
+  uint32_t T = 0;
+  size_t tri_cap = 0;
+  float4* d_vpos = nullptr;   // [3*tri_cap] (x,y,z,0)
+  float4* d_vattr = nullptr;  // [6*tri_cap] (nx,ny,nz,u),(v,Ix,Iy,Iz)
+  AxrGroupDev* d_groups = nullptr;
+
+  uint64_t* d_keys = nullptr;
+  float* d_depth = nullptr;
+  uint16_t *d_r = nullptr, *d_g = nullptr, *d_b = nullptr;
+  uint32_t* d_rgba = nullptr;
+  uint32_t* d_ids = nullptr;
+  AxrTriRec* d_small = nullptr;
+  AxrTriRec* d_large = nullptr;
+  AxrShRec* d_shrec = nullptr;
+  size_t rec_cap = 0;
+  uint32_t* d_counters = nullptr;
+  uint32_t* d_tile_cnt = nullptr;
+  uint32_t* d_tile_off = nullptr;
+  uint32_t* d_bins = nullptr;
+  uint32_t bin_cap = 0;
+  float* d_light = nullptr;  // vertLight light arrays
+  size_t light_cap = 0;
+
+  HostShadow sm[AXR_MAX_SHADOWS];
+  AxrCam view{};
+  float lint[3] = {1.0f, 1.0f, 1.0f};
+  float ldir[3] = {0.0f, -1.0f, 0.0f};
+  float amb = 0.2f;
+  uint16_t clear_rgb[3] = {1200, 4033, 21675};  // (60,110,255)^2/3, Ops.py:941
+
+  int state = ST_UNCLEARED;
+  AxrDrawConst last{};
+
+  bool timing = false;
+  std::vector<TimingRec> trecs;
+  std::vector<hipEvent_t> free_events;
+  std::map<std::string, std::pair<double, int64_t>> tacc;
+};
+
+namespace {
+
+// Scoped event pair around a launch sequence when timing is enabled.
+struct Timed {
+  axr_ctx* c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipEvent_t take() {
+    if (!c->free_events.empty()) {
+      hipEvent_t e = c->free_events.back();
+      c->free_events.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  Timed(axr_ctx* ctx, const char* n) : c(ctx), name(n) {
+    if (!c->timing) return;
+    a = take();
+    b = take();
+    (void)hipEventRecord(a, c->stream);
+  }
+  ~Timed() {
+    if (!c->timing) return;
+    (void)hipEventRecord(b, c->stream);
+    c->trecs.push_back({name, a, b});
+  }
+};
+
+void drain_timing(axr_ctx* c) {
+  if (c->trecs.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->trecs) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      auto& acc = c->tacc[r.name];
+      acc.first += ms;
+      acc.second += 1;
+    }
+    c->free_events.push_back(r.a);
+    c->free_events.push_back(r.b);
+  }
+  c->trecs.clear();
+}
+
+template <class T>
+int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+  return 0;
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+int check_ctx(axr_ctx* c) {
+  if (!c) return fail(AXR_EINVAL, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  return 0;
+}
+
+// Grows the concatenated vertex and record arrays to hold ntri more.
+int reserve_tris(axr_ctx* c, size_t extra) {
+  const size_t need = (size_t)c->T + extra;
+  if (need <= c->tri_cap) return 0;
+  size_t cap = c->tri_cap ? c->tri_cap : 1024;
+  while (cap < need) cap *= 2;
+  float4 *np = nullptr, *na = nullptr;
+  int rc;
+  if ((rc = dalloc(&np, 3 * cap))) return rc;
+  if ((rc = dalloc(&na, 6 * cap))) return rc;
+  if (c->T) {
+    HIPCHK(hipMemcpyAsync(np, c->d_vpos, (size_t)3 * c->T * sizeof(float4),
+                          hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(na, c->d_vattr, (size_t)6 * c->T * sizeof(float4),
+                          hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  dfree(c->d_vpos);
+  dfree(c->d_vattr);
+  c->d_vpos = np;
+  c->d_vattr = na;
+  c->tri_cap = cap;
+  dfree(c->d_small);
+  dfree(c->d_large);
+  dfree(c->d_shrec);
+  if ((rc = dalloc(&c->d_small, cap))) return rc;
+  if ((rc = dalloc(&c->d_large, cap))) return rc;
+  if ((rc = dalloc(&c->d_shrec, cap))) return rc;
+  c->rec_cap = cap;
+  return 0;
+}
+
+int upload_groups(axr_ctx* c) {
+  std::vector<AxrGroupDev> h(c->groups.size());
+  for (size_t i = 0; i < h.size(); i++) {
+    const HostGroup& g = c->groups[i];
+    h[i].base = g.base;
+    h[i].ntri = g.ntri;
+    h[i].flags = 0;
+    h[i].tex_size = g.tex_size;
+    h[i].mip_bias = g.mip_bias;
+    h[i].spec_pow = g.spec_pow;
+    h[i].tex = g.d_tex;
+  }
+  if (!h.empty())
+    HIPCHK(hipMemcpy(c->d_groups, h.data(), h.size() * sizeof(AxrGroupDev),
+                     hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Resolve + shade of the pending draw into the requested target.
+int materialize(axr_ctx* c, int mode, float exposure) {
+  AxrDrawConst dc = c->last;
+  if (c->state == ST_CLEARED) {
+    dc = AxrDrawConst{};
+    dc.W = c->W;
+    dc.H = c->H;
+    dc.WC = c->WC;
+    dc.HC = c->HC;
+    dc.T = c->T;
+    dc.ngroups = (uint32_t)c->groups.size();
+    for (int k = 0; k < 3; k++) dc.clear_rgb[k] = c->clear_rgb[k];
+  }
+  Timed tm(c, mode == SHADE_LDR8 ? "shade_gamma" : "shade");
+  HIPCHK(launch_shade(mode, c->d_keys, c->d_vpos, c->d_vattr, c->d_groups, dc, exposure,
+                      c->d_depth, c->d_r, c->d_g, c->d_b, c->d_rgba, c->stream));
+  c->state = mode == SHADE_LDR8 ? ST_LDR8 : ST_HDR;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* axr_last_error(void) { return g_err.c_str(); }
+
+int axr_create(int w, int h, axr_ctx** out) {
+  if (!out) return fail(AXR_EINVAL, "null out pointer");
+  *out = nullptr;
+  if (w <= 0 || h <= 0 || (w % 16) != 0)
+    return fail(AXR_EINVAL, "width must be a positive multiple of 16 and height positive (%dx%d)",
+                w, h);
+  if ((int64_t)w * h > (int64_t)1 << 28) return fail(AXR_EINVAL, "frame too large");
+  axr_ctx* c = new axr_ctx();
+  HIPCHK(hipGetDevice(&c->device));
+  c->W = w;
+  c->H = h;
+  c->WC = w / 16;
+  c->HC = h / 16;
+  c->NT = c->WC * c->HC;
+  const size_t np = (size_t)w * h;
+  int rc = 0;
+  auto bail = [&](int r) {
+    axr_destroy(c);
+    return r;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(AXR_EHIP, "hipStreamCreate failed"));
+  if ((rc = dalloc(&c->d_keys, np)) || (rc = dalloc(&c->d_depth, np)) ||
+      (rc = dalloc(&c->d_r, np)) || (rc = dalloc(&c->d_g, np)) || (rc = dalloc(&c->d_b, np)) ||
+      (rc = dalloc(&c->d_rgba, np)) || (rc = dalloc(&c->d_ids, np)) ||
+      (rc = dalloc(&c->d_counters, kCounters)) || (rc = dalloc(&c->d_tile_cnt, c->NT + 1)) ||
+      (rc = dalloc(&c->d_tile_off, c->NT + 1)) || (rc = dalloc(&c->d_bins, kDefaultBinCap)) ||
+      (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)))
+    return bail(rc);
+  c->bin_cap = kDefaultBinCap;
+  if (launch_fill_u32(c->d_counters, kCounters, 0, c->stream) != hipSuccess ||
+      launch_fill_u32(c->d_tile_cnt, c->NT + 1, 0, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return bail(fail(AXR_EHIP, "initial fill failed"));
+  // Default view: identity basis at the origin, 90 degree fovx.
+  const float vm[9] = {0, 0, 1, -1, 0, 0, 0, -1, 0};
+  std::memcpy(c->view.vm, vm, sizeof vm);
+  c->view.scale = (float)(w / 2);
+  c->view.cax = 3.0f;
+  c->view.cay = 3.0f;
+  *out = c;
+  return AXR_OK;
+}
+
+int axr_destroy(axr_ctx* c) {
+  if (!c) return AXR_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->trecs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->free_events) (void)hipEventDestroy(e);
+  for (auto& g : c->groups) dfree(g.d_tex);
+  for (auto& s : c->sm) dfree(s.d_map);
+  dfree(c->d_vpos);
+  dfree(c->d_vattr);
+  dfree(c->d_groups);
+  dfree(c->d_keys);
+  dfree(c->d_depth);
+  dfree(c->d_r);
+  dfree(c->d_g);
+  dfree(c->d_b);
+  dfree(c->d_rgba);
+  dfree(c->d_ids);
+  dfree(c->d_small);
+  dfree(c->d_large);
+  dfree(c->d_shrec);
+  dfree(c->d_counters);
+  dfree(c->d_tile_cnt);
+  dfree(c->d_tile_off);
+  dfree(c->d_bins);
+  dfree(c->d_light);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AXR_OK;
+}
+
+int axr_set_scale_cull(axr_ctx* c, float s, float cax, float cay) {
+  if (!c) return fail(AXR_EINVAL, "null context");
+  c->view.scale = s;
+  c->view.cax = cax;
+  c->view.cay = cay;
+  return AXR_OK;
+}
+
+int axr_add_texture_group(axr_ctx* c, const float* xyz, const float* uv, const float* vn,
+                          int ntri, const uint16_t* r, const uint16_t* g, const uint16_t* b,
+                          int64_t ntexels, int tex_size, int is_mip, int* tn_out) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (!xyz || !uv || !vn || !r || !g || !b || ntri < 0 || ntexels <= 0)
+    return fail(AXR_EINVAL, "addTextureGroup: bad arguments");
+  if ((int)c->groups.size() >= AXR_MAX_GROUPS)
+    return fail(AXR_ECAP, "at most %d groups per context", AXR_MAX_GROUPS);
+  if ((uint64_t)c->T + (uint64_t)ntri >= (1ull << 30))
+    return fail(AXR_ECAP, "at most 2^30 triangles per context");
+  // Texture shape: N*N texels, or the full smallest-first chain of N*N.
+  if (is_mip) {
+    // Trilinear reads levels mip and mip+1 <= texSize, so a chain needs >= 2 levels.
+    if (tex_size < 1 || tex_size > 14) return fail(AXR_EINVAL, "mip texSize out of range");
+    const int64_t want = (((int64_t)1 << (2 * (tex_size + 1))) - 1) / 3;
+    if (ntexels != want)
+      return fail(AXR_EINVAL, "mip chain of 2^%d needs %lld texels, got %lld", tex_size,
+                  (long long)want, (long long)ntexels);
+  } else {
+    if (tex_size <= 0 || (tex_size & (tex_size - 1)) ||
+        (int64_t)tex_size * tex_size != ntexels)
+      return fail(AXR_EINVAL, "texture must be N*N with N a power of two");
+  }
+  if ((rc = reserve_tris(c, (size_t)ntri))) return rc;
+  const size_t nv = (size_t)3 * ntri;
+  std::vector<float4> hp(nv), ha(2 * nv);
+  for (size_t v = 0; v < nv; v++) {
+    hp[v] = make_float4(xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2], 0.0f);
+    ha[2 * v] = make_float4(vn[3 * v], vn[3 * v + 1], vn[3 * v + 2], uv[2 * v]);
+    ha[2 * v + 1] = make_float4(uv[2 * v + 1], 1.0f, 1.0f, 1.0f);
+  }
+  if (nv) {
+    HIPCHK(hipMemcpy(c->d_vpos + (size_t)3 * c->T, hp.data(), nv * sizeof(float4),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_vattr + (size_t)6 * c->T, ha.data(), 2 * nv * sizeof(float4),
+                     hipMemcpyHostToDevice));
+  }
+  std::vector<ushort4> ht((size_t)ntexels);
+  for (int64_t i = 0; i < ntexels; i++) ht[i] = make_ushort4(r[i], g[i], b[i], 0);
+  HostGroup hg;
+  hg.base = c->T;
+  hg.ntri = (uint32_t)ntri;
+  hg.tex_size = tex_size;
+  hg.is_mip = is_mip ? 1 : 0;
+  hg.ntexels = ntexels;
+  if ((rc = dalloc(&hg.d_tex, (size_t)ntexels))) return rc;
+  HIPCHK(hipMemcpy(hg.d_tex, ht.data(), ht.size() * sizeof(ushort4), hipMemcpyHostToDevice));
+  c->groups.push_back(hg);
+  c->T += (uint32_t)ntri;
+  if ((rc = upload_groups(c))) return rc;
+  if (tn_out) *tn_out = (int)c->groups.size() - 1;
+  return AXR_OK;
+}
+
+int axr_set_group_material(axr_ctx* c, int tn, float mip_bias, int spec_pow) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (tn < 0 || tn >= (int)c->groups.size()) return fail(AXR_EINVAL, "no group %d", tn);
+  if (spec_pow < 0) return fail(AXR_EINVAL, "spec_pow must be >= 0");
+  c->groups[tn].mip_bias = mip_bias;
+  c->groups[tn].spec_pow = spec_pow;
+  return upload_groups(c);
+}
+
+int axr_set_primary_light(axr_ctx* c, const float I[3], const float D[3]) {
+  if (!c || !I || !D) return fail(AXR_EINVAL, "setPrimaryLight: bad arguments");
+  for (int k = 0; k < 3; k++) {
+    c->lint[k] = I[k];
+    c->ldir[k] = D[k];
+  }
+  return AXR_OK;
+}
+
+int axr_vert_light(axr_ctx* c, const uint8_t* mask, int ngroups, const float* dirI,
+                   const float* dirD, int ndir, const float* pointI, const float* pointP,
+                   int npoint, const float* spotI, const float* spotD, const float* spotP,
+                   int nspot, float amb) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (ngroups != (int)c->groups.size() || !mask)
+    return fail(AXR_EINVAL, "vertLight: mask must cover all %d groups", (int)c->groups.size());
+  if (ndir < 0 || npoint < 0 || nspot < 0) return fail(AXR_EINVAL, "negative light count");
+  if ((ndir && (!dirI || !dirD)) || (npoint && (!pointI || !pointP)) ||
+      (nspot && (!spotI || !spotD || !spotP)))
+    return fail(AXR_EINVAL, "vertLight: missing light arrays");
+  std::vector<float> h;
+  auto put = [&](const float* p, int n) {
+    size_t off = h.size();
+    for (int i = 0; i < 3 * n; i++) h.push_back(p[i]);
+    return off;
+  };
+  const size_t oDI = put(dirI, ndir), oDD = put(dirD, ndir);
+  const size_t oPI = put(pointI, npoint), oPP = put(pointP, npoint);
+  const size_t oSI = put(spotI, nspot), oSD = put(spotD, nspot), oSP = put(spotP, nspot);
+  if (h.size() > c->light_cap) {
+    dfree(c->d_light);
+    if ((rc = dalloc(&c->d_light, h.size()))) return rc;
+    c->light_cap = h.size();
+  }
+  if (!h.empty())
+    HIPCHK(hipMemcpy(c->d_light, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  AxrLights L;
+  L.ndir = ndir;
+  L.npoint = npoint;
+  L.nspot = nspot;
+  L.amb = amb;
+  L.dirI = c->d_light + oDI;
+  L.dirD = c->d_light + oDD;
+  L.pointI = c->d_light + oPI;
+  L.pointP = c->d_light + oPP;
+  L.spotI = c->d_light + oSI;
+  L.spotD = c->d_light + oSD;
+  L.spotP = c->d_light + oSP;
+  c->amb = amb;
+  for (int g = 0; g < ngroups; g++) {
+    if (!mask[g]) continue;
+    const HostGroup& hg = c->groups[g];
+    HIPCHK(launch_vertlight(c->d_vpos, c->d_vattr, 3 * hg.base, 3 * hg.ntri, L, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AXR_OK;
+}
+
+int axr_set_pos(axr_ctx* c, const float pos[3]) {
+  if (!c || !pos) return fail(AXR_EINVAL, "setPos: bad arguments");
+  for (int k = 0; k < 3; k++) c->view.pos[k] = pos[k];
+  return AXR_OK;
+}
+
+int axr_set_vm(axr_ctx* c, const float vm[9]) {
+  if (!c || !vm) return fail(AXR_EINVAL, "setVM: bad arguments");
+  for (int k = 0; k < 9; k++) c->view.vm[k] = vm[k];
+  return AXR_OK;
+}
+
+int axr_set_clear_color(axr_ctx* c, const uint16_t rgb[3]) {
+  if (!c || !rgb) return fail(AXR_EINVAL, "setClearColor: bad arguments");
+  for (int k = 0; k < 3; k++) c->clear_rgb[k] = rgb[k];
+  return AXR_OK;
+}
+
+int axr_clear_zbuffer(axr_ctx* c) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  Timed tm(c, "clear");
+  HIPCHK(launch_fill_u64(c->d_keys, (size_t)c->W * c->H, key_clear(), c->stream));
+  c->state = ST_CLEARED;
+  return AXR_OK;
+}
+
+int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, int ngroups,
+                 const int* shadow_ids, int nshadow, int pcf_taps) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (ngroups != (int)c->groups.size())
+    return fail(AXR_EINVAL, "drawAll: %d shader entries for %d groups", ngroups,
+                (int)c->groups.size());
+  if (!shader_flags) return fail(AXR_EINVAL, "drawAll: null shaders");
+  if (c->state != ST_CLEARED)
+    return fail(AXR_ESTATE, "drawAll needs a clearZBuffer first (one drawAll per clear)");
+  if (nshadow < 0 || nshadow > AXR_SHADOW_BIND)
+    return fail(AXR_EINVAL, "drawAll: at most %d shadow maps", AXR_SHADOW_BIND);
+  if (pcf_taps != 5 && pcf_taps != 9) return fail(AXR_EINVAL, "pcf_taps must be 5 or 9");
+  AxrDrawConst dc{};
+  dc.view = c->view;
+  for (int k = 0; k < 3; k++) {
+    dc.lint[k] = c->lint[k];
+    dc.ldir[k] = c->ldir[k];
+    dc.clear_rgb[k] = c->clear_rgb[k];
+  }
+  dc.amb = c->amb;
+  dc.W = c->W;
+  dc.H = c->H;
+  dc.WC = c->WC;
+  dc.HC = c->HC;
+  dc.T = c->T;
+  dc.ngroups = (uint32_t)ngroups;
+  dc.pcf_taps = pcf_taps;
+  bool any_shadow = false;
+  for (int g = 0; g < ngroups; g++) {
+    const uint32_t f = shader_flags[g] & (AXR_SH_MIP | AXR_SH_SHADOW | AXR_SH_PHONG | AXR_SH_CULL);
+    if ((f & AXR_SH_MIP) && !c->groups[g].is_mip)
+      return fail(AXR_EINVAL, "group %d: mip shader on a texture without mips", g);
+    if ((f & AXR_SH_SHADOW) && !(mask && mask[g])) any_shadow = true;
+    dc.gflags[g] = f | ((mask && mask[g]) ? 0u : AXR_GRP_DRAW);
+  }
+  dc.nshadow = 0;
+  if (any_shadow) {
+    for (int k = 0; k < nshadow; k++) {
+      const int i = shadow_ids ? shadow_ids[k] : k;
+      if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists || !c->sm[i].placed)
+        return fail(AXR_ESTATE, "drawAll: shadow map %d missing or not placed", i);
+      AxrShadowDev& sd = dc.sh[dc.nshadow++];
+      sd.map = c->sm[i].d_map;
+      sd.dim = c->sm[i].dim;
+      sd.wS = c->sm[i].dim / 2;
+      sd.cam = c->sm[i].cam;
+    }
+  }
+  if (c->T > 0) {
+    HIPCHK(launch_fill_u32(c->d_counters, kCounters, 0, c->stream));
+    HIPCHK(launch_fill_u32(c->d_tile_cnt, c->NT + 1, 0, c->stream));
+    {
+      Timed tm(c, "setup");
+      HIPCHK(launch_setup(c->d_vpos, c->d_vattr, c->d_groups, dc, c->d_small, c->d_large,
+                          c->d_counters, c->stream));
+    }
+    {
+      Timed tm(c, "raster_small");
+      HIPCHK(launch_raster_small(c->d_small, c->d_counters, c->d_keys, c->W, c->H, c->T,
+                                 c->stream));
+    }
+    {
+      Timed tm(c, "bin_count");
+      HIPCHK(launch_bin_count(c->d_large, c->d_counters, c->d_tile_cnt, c->WC, c->HC, c->T,
+                              c->stream));
+    }
+    {
+      Timed tm(c, "bin_scan");
+      HIPCHK(launch_bin_scan(c->d_tile_cnt, c->d_tile_off, c->NT, c->d_counters, c->bin_cap,
+                             c->stream));
+    }
+    {
+      Timed tm(c, "bin_fill");
+      HIPCHK(launch_bin_fill(c->d_large, c->d_counters, c->d_tile_cnt, c->d_tile_off, c->d_bins,
+                             c->WC, c->HC, c->bin_cap, c->T, c->stream));
+    }
+    {
+      Timed tm(c, "raster_tiles");
+      HIPCHK(launch_raster_tiles(c->d_large, c->d_tile_off, c->d_bins, c->d_keys, c->W, c->WC,
+                                 c->HC, c->T, c->bin_cap, c->stream));
+    }
+  }
+  c->last = dc;
+  c->state = ST_PENDING;
+  return AXR_OK;
+}
+
+int axr_add_shadow_map(axr_ctx* c, int i, int size, float scale, float amb) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (i < 0 || i >= AXR_MAX_SHADOWS) return fail(AXR_EINVAL, "shadow map index %d", i);
+  if (size < 8 || size > 16384 || (size & 1)) return fail(AXR_EINVAL, "shadow map size %d", size);
+  HostShadow& s = c->sm[i];
+  dfree(s.d_map);
+  if ((rc = dalloc(&s.d_map, (size_t)size * size))) return rc;
+  s.exists = true;
+  s.placed = false;
+  s.dim = size;
+  s.scale = scale;
+  s.cam.scale = scale;
+  if (amb >= 0) c->amb = amb;
+  HIPCHK(launch_fill_u32(s.d_map, (size_t)size * size, ordf(AXR_DEPTH_CLEAR), c->stream));
+  return AXR_OK;
+}
+
+int axr_place_shadow_map(axr_ctx* c, int i, const float pos[3], const float vm[9], float amb) {
+  if (!c || !pos || !vm) return fail(AXR_EINVAL, "placeShadowMap: bad arguments");
+  if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists)
+    return fail(AXR_EINVAL, "no shadow map %d", i);
+  HostShadow& s = c->sm[i];
+  for (int k = 0; k < 3; k++) s.cam.pos[k] = pos[k];
+  for (int k = 0; k < 9; k++) s.cam.vm[k] = vm[k];
+  s.cam.scale = s.scale;
+  s.placed = true;
+  if (amb >= 0) c->amb = amb;
+  return AXR_OK;
+}
+
+int axr_clear_shadow_map(axr_ctx* c, int i) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists)
+    return fail(AXR_EINVAL, "no shadow map %d", i);
+  Timed tm(c, "shadow_clear");
+  HIPCHK(launch_fill_u32(c->sm[i].d_map, (size_t)c->sm[i].dim * c->sm[i].dim,
+                         ordf(AXR_DEPTH_CLEAR), c->stream));
+  return AXR_OK;
+}
+
+int axr_shadow_map(axr_ctx* c, int i, const uint8_t* cast, int ngroups, float bias) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists || !c->sm[i].placed)
+    return fail(AXR_EINVAL, "shadow map %d missing or not placed", i);
+  if (ngroups != (int)c->groups.size() || !cast)
+    return fail(AXR_EINVAL, "shadowMap: whichCast must cover all groups");
+  AxrCastMask m{};
+  for (int g = 0; g < ngroups; g++)
+    if (cast[g]) m.bits[g >> 5] |= 1u << (g & 31);
+  if (c->T == 0) return AXR_OK;
+  HIPCHK(launch_fill_u32(c->d_counters + 2, 1, 0, c->stream));
+  {
+    Timed tm(c, "shadow_setup");
+    HIPCHK(launch_shadow_setup(c->d_vpos, c->d_groups, (uint32_t)c->groups.size(), c->T, m,
+                               c->sm[i].cam, c->sm[i].dim, bias, c->d_shrec, c->d_counters + 2,
+                               c->stream));
+  }
+  {
+    Timed tm(c, "shadow_raster");
+    HIPCHK(launch_shadow_raster(c->d_shrec, c->d_counters + 2, c->sm[i].d_map, c->sm[i].dim, c->T,
+                                c->stream));
+  }
+  return AXR_OK;
+}
+
+int axr_gamma(axr_ctx* c, float ex) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  switch (c->state) {
+    case ST_CLEARED:
+    case ST_PENDING:
+      return materialize(c, SHADE_LDR8, ex);
+    case ST_HDR:
+    case ST_LDR16: {
+      Timed tm(c, "gamma");
+      HIPCHK(launch_gamma_planes(c->d_r, c->d_g, c->d_b, (size_t)c->W * c->H, ex, c->stream));
+      c->state = ST_LDR16;
+      return AXR_OK;
+    }
+    case ST_LDR8: {
+      Timed tm(c, "gamma");
+      const size_t n = (size_t)c->W * c->H;
+      HIPCHK(launch_expand_rgba8(c->d_rgba, c->d_r, c->d_g, c->d_b, n, c->stream));
+      HIPCHK(launch_gamma_planes(c->d_r, c->d_g, c->d_b, n, ex, c->stream));
+      c->state = ST_LDR16;
+      return AXR_OK;
+    }
+    default:
+      return fail(AXR_ESTATE, "gamma before any clearZBuffer");
+  }
+}
+
+int axr_get_frame(axr_ctx* c, uint16_t* r, uint16_t* g, uint16_t* b, float* depth) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  const size_t n = (size_t)c->W * c->H;
+  if (c->state == ST_UNCLEARED) return fail(AXR_ESTATE, "getFrame before any clearZBuffer");
+  if (c->state == ST_CLEARED || c->state == ST_PENDING) {
+    if ((rc = materialize(c, SHADE_HDR, 1.0f))) return rc;
+  } else if (c->state == ST_LDR8) {
+    HIPCHK(launch_expand_rgba8(c->d_rgba, c->d_r, c->d_g, c->d_b, n, c->stream));
+    c->state = ST_LDR16;
+  }
+  if (r) HIPCHK(hipMemcpyAsync(r, c->d_r, n * 2, hipMemcpyDeviceToHost, c->stream));
+  if (g) HIPCHK(hipMemcpyAsync(g, c->d_g, n * 2, hipMemcpyDeviceToHost, c->stream));
+  if (b) HIPCHK(hipMemcpyAsync(b, c->d_b, n * 2, hipMemcpyDeviceToHost, c->stream));
+  if (depth) HIPCHK(hipMemcpyAsync(depth, c->d_depth, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AXR_OK;
+}
+
+int axr_get_rgba8(axr_ctx* c, uint32_t* rgba) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (!rgba) return fail(AXR_EINVAL, "null output");
+  const size_t n = (size_t)c->W * c->H;
+  if (c->state == ST_LDR8) {
+    HIPCHK(hipMemcpyAsync(rgba, c->d_rgba, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return AXR_OK;
+  }
+  if (c->state == ST_LDR16) {
+    std::vector<uint16_t> r(n), g(n), b(n);
+    HIPCHK(hipMemcpyAsync(r.data(), c->d_r, n * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(g.data(), c->d_g, n * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(b.data(), c->d_b, n * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++)
+      rgba[i] = (uint32_t)(r[i] & 0xFF) | ((uint32_t)(g[i] & 0xFF) << 8) |
+                ((uint32_t)(b[i] & 0xFF) << 16) | 0xFF000000u;
+    return AXR_OK;
+  }
+  return fail(AXR_ESTATE, "getRGBA8 needs gamma() first");
+}
+
+int axr_get_ids(axr_ctx* c, uint32_t* ids) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (!ids) return fail(AXR_EINVAL, "null output");
+  if (c->state == ST_UNCLEARED) return fail(AXR_ESTATE, "getIds before any clearZBuffer");
+  const size_t n = (size_t)c->W * c->H;
+  HIPCHK(launch_ids(c->d_keys, c->T, c->d_ids, n, c->stream));
+  HIPCHK(hipMemcpyAsync(ids, c->d_ids, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AXR_OK;
+}
+
+int axr_get_shadow_map(axr_ctx* c, int i, float* out) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists || !out)
+    return fail(AXR_EINVAL, "no shadow map %d", i);
+  const size_t n = (size_t)c->sm[i].dim * c->sm[i].dim;
+  float* d = nullptr;
+  if ((rc = dalloc(&d, n))) return rc;
+  hipError_t e = launch_decode_shadow(c->sm[i].d_map, d, n, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dfree(d);
+  if (e != hipSuccess) return fail(AXR_EHIP, "getSHM: %s", hipGetErrorString(e));
+  return AXR_OK;
+}
+
+int axr_sync(axr_ctx* c) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AXR_OK;
+}
+
+int axr_get_stats(axr_ctx* c, uint32_t out[5]) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (!out) return fail(AXR_EINVAL, "null output");
+  uint32_t h[kCounters];
+  HIPCHK(hipMemcpyAsync(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 5; k++) out[k] = h[k];
+  return AXR_OK;
+}
+
+int axr_set_timing(axr_ctx* c, int enable) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  drain_timing(c);
+  c->tacc.clear();
+  c->timing = enable != 0;
+  return AXR_OK;
+}
+
+int axr_get_timing(axr_ctx* c, char* names, int name_len, double* total_ms, int64_t* count,
+                   int n) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  drain_timing(c);
+  int k = 0;
+  for (auto& kv : c->tacc) {
+    if (k >= n) break;
+    if (names && name_len > 0) {
+      std::strncpy(names + (size_t)k * name_len, kv.first.c_str(), name_len - 1);
+      names[(size_t)k * name_len + name_len - 1] = 0;
+    }
+    if (total_ms) total_ms[k] = kv.second.first;
+    if (count) count[k] = kv.second.second;
+    k++;
+  }
+  return k;
+}
+
+}  // extern "C"

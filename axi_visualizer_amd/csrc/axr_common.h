@@ -1,0 +1,133 @@
+// axr_common.h -- shared types and device arithmetic of the gfx950 raster path.
+//
+// Every float operation here is one IEEE binary32 operation (the library is
+// compiled with -ffp-contract=off and HIP's default correctly rounded
+// division and sqrt), so results are bit-identical to the serial restatement
+// of the reference's OpenCL kernels.  Decrees where OpenCL C is loose:
+// dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z, min(x,y) = y<x?y:x,
+// max(x,y) = x<y?y:x, float->int truncates and saturates (NaN -> 0),
+// float->ushort saturates, normalize(v) = v * (1/sqrt(v.v)),
+// pown by binary exponentiation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define AXR_SH_MIP 1u
+#define AXR_SH_SHADOW 2u
+#define AXR_SH_PHONG 4u
+#define AXR_SH_CULL 8u
+#define AXR_GRP_DRAW 0x100u  // group takes part in the current drawAll
+
+#define AXR_MAX_GROUPS 128
+
+#define AXR_MAX_SHADOWS 4
+#define AXR_SHADOW_BIND 2  // cascades sampled by one drawAll
+#define AXR_NONE 0xFFFFFFFFu
+
+// Camera: rows of vm are (forward, horizontal, vertical), as VIEWMAT
+// (Compute.py:313-314); scale/cax/cay as setScaleCull (Ops.py:231-233).
+struct AxrCam {
+  float pos[3];
+  float vm[9];
+  float scale, cax, cay;
+};
+
+// Static per-group data (device table, uploaded when groups change).
+struct AxrGroupDev {
+  uint32_t base;   // first global triangle index
+  uint32_t ntri;
+  uint32_t flags;  // filled per draw from AxrDrawConst::gflags
+  int tex_size;    // N, or log2(N) for a mip chain (texSize, Ops.py:371-374)
+  float mip_bias;
+  int spec_pow;
+  const ushort4* tex;  // texels (R,G,B,0) u16; mip chain smallest first
+};
+
+struct AxrShadowDev {
+  const uint32_t* map;  // orderable depth keys, dim*dim
+  int dim;
+  int wS;  // dim/2
+  AxrCam cam;
+};
+
+// Per-draw constants, passed by value to the raster and shade kernels.
+struct AxrDrawConst {
+  AxrCam view;
+  float lint[3];
+  float ldir[3];
+  float amb;
+  int W, H, WC, HC;
+  uint32_t T;        // total triangles in the context (order space)
+  uint32_t ngroups;
+  int nshadow;
+  int pcf_taps;
+  AxrShadowDev sh[AXR_SHADOW_BIND];
+  uint16_t clear_rgb[4];
+  uint32_t gflags[AXR_MAX_GROUPS];  // AXR_SH_* | AXR_GRP_DRAW per group
+};
+
+// Groups casting into one shadow pass (Ops.py:1094-1095 whichCast).
+struct AxrCastMask {
+  uint32_t bits[AXR_MAX_GROUPS / 32];
+};
+
+// Compact record of one visible triangle from setup.
+struct AxrTriRec {
+  float px[3], py[3], z[3];
+  uint32_t gi;
+};
+
+// Compact record of one shadow caster (int coords).
+struct AxrShRec {
+  int x[3], y[3];
+  float z[3];
+};
+
+// -------------------------------------------------------------- helpers
+#define AXR_HD __host__ __device__ __forceinline__
+
+AXR_HD float omin(float x, float y) { return y < x ? y : x; }
+AXR_HD float omax(float x, float y) { return x < y ? y : x; }
+AXR_HD int imin(int x, int y) { return y < x ? y : x; }
+AXR_HD int imax(int x, int y) { return x < y ? y : x; }
+AXR_HD float oclamp(float x, float lo, float hi) { return omin(omax(x, lo), hi); }
+AXR_HD int f2i(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return 2147483647;
+  if (x < -2147483648.0f) return (-2147483647 - 1);
+  return (int)x;
+}
+AXR_HD uint16_t f2u16(float x) {
+  if (!(x > 0.0f)) return 0;
+  if (x >= 65535.0f) return 65535;
+  return (uint16_t)x;
+}
+AXR_HD int abs_mask(int v, int m) {
+  unsigned u = v < 0 ? 0u - (unsigned)v : (unsigned)v;
+  return (int)(u & (unsigned)m);
+}
+
+// Orderable 32-bit key of a float: unsigned order == float order (non-NaN).
+AXR_HD uint32_t ordf(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+AXR_HD float unordf(uint32_t k) {
+  uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+// 64-bit depth-test key: (orderable depth << 32) | ~seq.  The minimum key
+// is the minimum depth and, among equal depths, the largest seq, i.e. the
+// last fragment in pass order -- exactly the serial `F >= tz` semantics.
+// seq = 2*order + half + 1, order = (large ? T : 0) + global triangle index.
+#define AXR_DEPTH_CLEAR 65535.0f
+AXR_HD uint64_t key_clear() { return ((uint64_t)ordf(AXR_DEPTH_CLEAR) << 32) | 0xFFFFFFFFull; }
+AXR_HD uint64_t make_key(float tz, uint32_t seq) {
+  float d = (tz == 0.0f) ? 0.0f : tz;
+  return ((uint64_t)ordf(d) << 32) | (uint64_t)(~seq);
+}
+AXR_HD uint32_t frag_seq(uint32_t order, int half) { return 2u * order + (uint32_t)half + 1u; }

@@ -1,0 +1,716 @@
+// axr_tri.h -- per-triangle device math: setup, scanline walks, replay, shade.
+//
+// One definition of each walk serves two purposes: the depth pass
+// enumerates fragments (ax, cy, tz, half) and issues 64-bit key minima; the
+// shade pass replays the same walk for a single pixel to rebuild the
+// winning fragment's interpolants.  Both execute the identical instruction
+// sequence on identical inputs, so they agree bit for bit.
+#pragma once
+#include "axr_common.h"
+
+namespace axr {
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 scl(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 normalize(V3 v) { return scl(v, 1.0f / sqrtf(dot(v, v))); }
+__device__ __forceinline__ float pown(float x, int n) {
+  float r = 1.0f, b = x;
+  unsigned e = (unsigned)(n < 0 ? 0 : n);
+  while (e) {
+    if (e & 1u) r = r * b;
+    b = b * b;
+    e >>= 1;
+  }
+  return r;
+}
+__device__ __forceinline__ V3 cam_row(const AxrCam& c, int r) {
+  return {c.vm[3 * r], c.vm[3 * r + 1], c.vm[3 * r + 2]};
+}
+
+// ------------------------------------------------------------- attributes
+// Perspective-premultiplied vertex attributes (AVSL.py:414-422): UV, I, N, PXYZ.
+struct Attr {
+  float u, v, ix, iy, iz, nx, ny, nz, px, py, pz;
+};
+#define AXR_ATTR_FIELDS(M) M(u) M(v) M(ix) M(iy) M(iz) M(nx) M(ny) M(nz) M(px) M(py) M(pz)
+
+__device__ __forceinline__ Attr a_scale(const Attr& a, float s) {
+  Attr r;
+#define M(f) r.f = a.f * s;
+  AXR_ATTR_FIELDS(M)
+#undef M
+  return r;
+}
+// (a - b) * s
+__device__ __forceinline__ Attr a_slope(const Attr& a, const Attr& b, float s) {
+  Attr r;
+#define M(f) r.f = (a.f - b.f) * s;
+  AXR_ATTR_FIELDS(M)
+#undef M
+  return r;
+}
+// -(a - b) * s
+__device__ __forceinline__ Attr a_nslope(const Attr& a, const Attr& b, float s) {
+  Attr r;
+#define M(f) r.f = (-(a.f - b.f)) * s;
+  AXR_ATTR_FIELDS(M)
+#undef M
+  return r;
+}
+// a1 + s * (a3 - a1)
+__device__ __forceinline__ Attr a_lerp4(const Attr& a1, const Attr& a3, float s) {
+  Attr r;
+#define M(f) r.f = a1.f + s * (a3.f - a1.f);
+  AXR_ATTR_FIELDS(M)
+#undef M
+  return r;
+}
+// a + d * s
+__device__ __forceinline__ Attr a_mad(const Attr& a, float d, const Attr& s) {
+  Attr r;
+#define M(f) r.f = a.f + d * s.f;
+  AXR_ATTR_FIELDS(M)
+#undef M
+  return r;
+}
+__device__ __forceinline__ void a_inc(Attr& a, const Attr& s) {
+#define M(f) a.f += s.f;
+  AXR_ATTR_FIELDS(M)
+#undef M
+}
+__device__ __forceinline__ void a_dec(Attr& a, const Attr& s) {
+#define M(f) a.f -= s.f;
+  AXR_ATTR_FIELDS(M)
+#undef M
+}
+template <class T>
+__device__ __forceinline__ void swp(T& a, T& b) {
+  T t = a;
+  a = b;
+  b = t;
+}
+
+// Vertex k of global triangle gi, premultiplied by its reciprocal depth z.
+// vpos: (x,y,z,0); vattr[2v] = (nx,ny,nz,u), vattr[2v+1] = (v,Ix,Iy,Iz).
+__device__ __forceinline__ Attr load_attr(const float4* __restrict__ vpos,
+                                          const float4* __restrict__ vattr, uint32_t vtx,
+                                          float z) {
+  float4 p = vpos[vtx];
+  float4 a0 = vattr[2 * (size_t)vtx];
+  float4 a1 = vattr[2 * (size_t)vtx + 1];
+  Attr r;
+  r.u = a0.w * z;
+  r.v = a1.x * z;
+  r.ix = a1.y * z;
+  r.iy = a1.z * z;
+  r.iz = a1.w * z;
+  r.nx = a0.x * z;
+  r.ny = a0.y * z;
+  r.nz = a0.z * z;
+  r.px = p.x * z;
+  r.py = p.y * z;
+  r.pz = p.z * z;
+  return r;
+}
+
+// ------------------------------------------------------------ a5: setup
+// Pipe/trisetup.c:28-80 (+ trisetup_cull.c:49-54).  0 reject, 1 small, 2 large.
+__device__ __forceinline__ int tri_setup(const float4* __restrict__ vpos,
+                                         const float4* __restrict__ vattr, uint32_t gi,
+                                         const AxrCam& cam, int W, int H, bool cull, float px[3],
+                                         float py[3], float zz[3]) {
+  const V3 vp = v3(cam.pos[0], cam.pos[1], cam.pos[2]);
+  const V3 SVd = cam_row(cam, 0), SVx = cam_row(cam, 1), SVy = cam_row(cam, 2);
+  V3 x[3];
+  float dd[3], dx[3], dy[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float4 p = vpos[3 * (size_t)gi + k];
+    x[k] = sub(v3(p.x, p.y, p.z), vp);
+    dd[k] = dot(x[k], SVd);
+    dx[k] = dot(x[k], SVx);
+    dy[k] = dot(x[k], SVy);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    dd[k] = 1.0f / dd[k];
+    dx[k] = dx[k] * dd[k];
+    dy[k] = dy[k] * dd[k];
+  }
+  bool yes = dd[0] > 0 && dd[1] > 0 && dd[2] > 0 && fabsf(dx[0]) < cam.cax &&
+             fabsf(dx[1]) < cam.cax && fabsf(dx[2]) < cam.cax && fabsf(dy[0]) < cam.cay &&
+             fabsf(dy[1]) < cam.cay && fabsf(dy[2]) < cam.cay;
+  if (cull) {
+    bool back = true;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      float4 n = vattr[2 * (3 * (size_t)gi + k)];
+      if (!(dot(v3(n.x, n.y, n.z), x[k]) < -0.0f)) back = false;
+    }
+    if (back) yes = false;
+  }
+  const float hw = (float)(W / 2), hh = (float)(H / 2);
+  int dix[3], diy[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    px[k] = dx[k] * -cam.scale + hw;
+    py[k] = dy[k] * cam.scale + hh;
+    zz[k] = dd[k];
+    dix[k] = f2i(px[k]);
+    diy[k] = f2i(py[k]);
+  }
+  long long maxX = imax(dix[0], imax(dix[1], dix[2]));
+  long long minX = imin(dix[0], imin(dix[1], dix[2]));
+  long long maxY = imax(diy[0], imax(diy[1], diy[2]));
+  long long minY = imin(diy[0], imin(diy[1], diy[2]));
+  long long dimx = maxX - minX, dimy = maxY - minY;
+  if (dimx == 0 || dimx > 2LL * W) yes = false;
+  if (dimy == 0 || dimy > 2LL * H) yes = false;
+  if (maxX < 0 || maxY < 0 || minX > W || minY > H) yes = false;
+  if (!yes) return 0;
+  return (dimx < 96 && dimy < 96) ? 1 : 2;  // SMALL_THRESH * TILE_SIZE
+}
+
+// ------------------------------------------------- sorted triangle shapes
+// Sort by y with strict '>' swaps in the order (1,2), (2,3), (1,2).
+// vk[] records which input vertex landed in each sorted slot.
+template <class T>
+struct Geom {
+  T x1, y1, x2, y2, x3, y3;
+  float z1, z2, z3;
+  int k1, k2, k3;
+  float yd, x4, z4;  // 4th vertex on the long edge at y2
+};
+
+template <class T>
+__device__ __forceinline__ void geom_sort(Geom<T>& g) {
+  if (g.y1 > g.y2) { swp(g.y1, g.y2); swp(g.x1, g.x2); swp(g.z1, g.z2); swp(g.k1, g.k2); }
+  if (g.y2 > g.y3) { swp(g.y2, g.y3); swp(g.x2, g.x3); swp(g.z2, g.z3); swp(g.k2, g.k3); }
+  if (g.y1 > g.y2) { swp(g.y1, g.y2); swp(g.x1, g.x2); swp(g.z1, g.z2); swp(g.k1, g.k2); }
+}
+
+// Small path (Templates.py:230-246): vertices snapped to int.
+__device__ __forceinline__ Geom<int> small_geom(const float px[3], const float py[3],
+                                                const float z[3]) {
+  Geom<int> g;
+  g.x1 = f2i(px[0]); g.y1 = f2i(py[0]); g.z1 = z[0]; g.k1 = 0;
+  g.x2 = f2i(px[1]); g.y2 = f2i(py[1]); g.z2 = z[1]; g.k2 = 1;
+  g.x3 = f2i(px[2]); g.y3 = f2i(py[2]); g.z3 = z[2]; g.k3 = 2;
+  geom_sort(g);
+  g.yd = (float)(g.y2 - g.y1) / (float)(g.y3 - g.y1);
+  g.x4 = (float)g.x1 + g.yd * (float)(g.x3 - g.x1);
+  g.z4 = g.z1 + g.yd * (g.z3 - g.z1);
+  return g;
+}
+
+// Tiled path (Templates.py:54-68): float vertices.
+__device__ __forceinline__ Geom<float> tile_geom(const float px[3], const float py[3],
+                                                 const float z[3]) {
+  Geom<float> g;
+  g.x1 = px[0]; g.y1 = py[0]; g.z1 = z[0]; g.k1 = 0;
+  g.x2 = px[1]; g.y2 = py[1]; g.z2 = z[1]; g.k2 = 1;
+  g.x3 = px[2]; g.y3 = py[2]; g.z3 = z[2]; g.k3 = 2;
+  geom_sort(g);
+  g.yd = (g.y2 - g.y1) / (g.y3 - g.y1);
+  g.x4 = g.x1 + g.yd * (g.x3 - g.x1);
+  g.z4 = g.z1 + g.yd * (g.z3 - g.z1);
+  return g;
+}
+
+// Edge slopes of one half, swapped so slope1 >= slope2 (Templates.py:95-111, 141-156).
+struct Edges {
+  float s1, s2, sz1, sz2, inv;
+  bool swapped;
+};
+__device__ __forceinline__ Edges small_edges(const Geom<int>& g, int half) {
+  Edges e;
+  if (half == 0) {
+    e.inv = 1 / (float)(g.y2 - g.y1);
+    e.s1 = (float)(g.x2 - g.x1) * e.inv;
+    e.s2 = (g.x4 - (float)g.x1) * e.inv;
+    e.sz1 = (g.z2 - g.z1) * e.inv;
+    e.sz2 = (g.z4 - g.z1) * e.inv;
+  } else {
+    e.inv = 1 / (float)(g.y3 - g.y2);
+    e.s1 = (float)(g.x3 - g.x2) * e.inv;
+    e.s2 = ((float)g.x3 - g.x4) * e.inv;
+    e.sz1 = (g.z3 - g.z2) * e.inv;
+    e.sz2 = (g.z3 - g.z4) * e.inv;
+  }
+  e.swapped = e.s1 < e.s2;
+  if (e.swapped) {
+    swp(e.s1, e.s2);
+    swp(e.sz1, e.sz2);
+  }
+  return e;
+}
+__device__ __forceinline__ Edges tile_edges(const Geom<float>& g, int half) {
+  Edges e;
+  if (half == 0) {
+    e.inv = 1 / (g.y2 - g.y1);
+    e.s1 = (g.x2 - g.x1) * e.inv;
+    e.s2 = (g.x4 - g.x1) * e.inv;
+    e.sz1 = (g.z2 - g.z1) * e.inv;
+    e.sz2 = (g.z4 - g.z1) * e.inv;
+  } else {
+    e.inv = 1 / (g.y3 - g.y2);
+    e.s1 = (g.x3 - g.x2) * e.inv;
+    e.s2 = (g.x3 - g.x4) * e.inv;
+    e.sz1 = (g.z3 - g.z2) * e.inv;
+    e.sz2 = (g.z3 - g.z4) * e.inv;
+  }
+  e.swapped = e.s1 < e.s2;
+  if (e.swapped) {
+    swp(e.s1, e.s2);
+    swp(e.sz1, e.sz2);
+  }
+  return e;
+}
+
+__device__ __forceinline__ float span_t(int ax, float cx1, float cx2) {
+  float t = ((float)ax - cx2) / (cx1 - cx2);
+  return omax(0.0f, omin(1.0f, t));
+}
+__device__ __forceinline__ float span_tz(float t, float cz1, float cz2) {
+  return 1 / ((1 - t) * cz2 + t * cz1);
+}
+
+// ------------------------------------------------- a7: small-path walk
+// Templates.py:250-321.  frag(ax, cy, tz, half) for on-screen pixels.
+template <class F>
+__device__ __forceinline__ void small_walk(const Geom<int>& g, int W, int H, F&& frag) {
+  {
+    Edges e = small_edges(g, 0);
+    float cx1 = (float)g.x1, cx2 = cx1, cz1 = g.z1, cz2 = g.z1;
+    for (int cy = g.y1; cy <= g.y2; cy++) {
+      if (cy >= H) break;
+      if (cy >= 0) {
+        for (int ax = imax(f2i(cx2), 0); ax < W && (float)ax <= cx1; ax++) {
+          float t = span_t(ax, cx1, cx2);
+          frag(ax, cy, span_tz(t, cz1, cz2), 0);
+        }
+      }
+      cx1 += e.s1;
+      cx2 += e.s2;
+      cz1 += e.sz1;
+      cz2 += e.sz2;
+    }
+  }
+  {
+    Edges e = small_edges(g, 1);
+    float cx1 = (float)g.x3, cx2 = cx1, cz1 = g.z3, cz2 = g.z3;
+    for (int cy = g.y3; cy >= g.y2; cy--) {
+      if (cy < 0) break;
+      if (cy < H) {
+        for (int ax = imax(f2i(cx1), 0); ax < W && (float)ax <= cx2; ax++) {
+          float t = span_t(ax, cx1, cx2);
+          frag(ax, cy, span_tz(t, cz1, cz2), 1);
+        }
+      }
+      cx1 -= e.s1;
+      cx2 -= e.s2;
+      cz1 -= e.sz1;
+      cz2 -= e.sz2;
+    }
+  }
+}
+
+// ------------------------------------------------- a9: tiled-path walk
+struct TileBox {
+  float xMin, xMax, yMin, yMax;
+};
+// Templates.py:23-29.
+__device__ __forceinline__ TileBox tile_box(int bx, int W) {
+  int tileX = bx % (int)((float)W / 16.0f);
+  int tileY = f2i((float)bx / ((float)W / 16.0f));
+  TileBox b;
+  b.xMin = (float)tileX * 16.0f;
+  b.xMax = ((float)tileX + 1.0f) * 16.0f - 1.0f;
+  b.yMin = (float)tileY * 16.0f;
+  b.yMax = ((float)tileY + 1.0f) * 16.0f - 1.0f;
+  return b;
+}
+
+// Templates.py:113-182 inside one tile.  frag(ax, cy, tz, half).
+template <class F>
+__device__ __forceinline__ void tile_walk(const Geom<float>& g, const TileBox& b, F&& frag) {
+  {
+    Edges e = tile_edges(g, 0);
+    int cy = f2i(oclamp(g.y1, b.yMin, b.yMax));
+    float dy = (float)cy - g.y1;
+    float cx1 = g.x1 + dy * e.s1, cx2 = g.x1 + dy * e.s2;
+    float cz1 = g.z1 + dy * e.sz1, cz2 = g.z1 + dy * e.sz2;
+    if (g.y2 > b.yMin) {
+      const float yEnd = oclamp(g.y2, b.yMin, b.yMax);
+      for (; (float)cy <= yEnd; cy++) {
+        if (cx2 < b.xMax && cx1 > b.xMin) {
+          const float xe = oclamp(cx1, b.xMin, b.xMax);
+          for (int ax = f2i(oclamp(cx2, b.xMin, b.xMax)); (float)ax <= xe; ax++) {
+            float t = span_t(ax, cx1, cx2);
+            frag(ax, cy, span_tz(t, cz1, cz2), 0);
+          }
+        }
+        cx1 += e.s1;
+        cx2 += e.s2;
+        cz1 += e.sz1;
+        cz2 += e.sz2;
+      }
+    }
+  }
+  {
+    Edges e = tile_edges(g, 1);
+    int cy = f2i(oclamp(g.y3, b.yMin, b.yMax));
+    float dy = (float)cy - g.y3;
+    float cx1 = g.x3 + dy * e.s1, cx2 = g.x3 + dy * e.s2;
+    float cz1 = g.z3 + dy * e.sz1, cz2 = g.z3 + dy * e.sz2;
+    if (g.y2 < b.yMax) {
+      const float yEnd = oclamp(g.y2, b.yMin, b.yMax);
+      for (; (float)cy >= yEnd; cy--) {
+        if (cx1 < b.xMax && cx2 > b.xMin) {
+          const float xe = oclamp(cx2, b.xMin, b.xMax);
+          for (int ax = f2i(oclamp(cx1, b.xMin, b.xMax)); (float)ax <= xe; ax++) {
+            float t = span_t(ax, cx1, cx2);
+            frag(ax, cy, span_tz(t, cz1, cz2), 1);
+          }
+        }
+        cx1 -= e.s1;
+        cx2 -= e.s2;
+        cz1 -= e.sz1;
+        cz2 -= e.sz2;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------- a8: coarse binning
+// Pipe/coarse.c:26-236 over one large triangle.  emit(tile) per entry,
+// duplicates included.
+template <class F>
+__device__ __forceinline__ void coarse_walk(const float px[3], const float py[3], int WC, int HC,
+                                            F&& emit) {
+  float x1 = px[0] / 16.0f - 0.5f, y1 = py[0] / 16.0f - 0.5f;
+  float x2 = px[1] / 16.0f - 0.5f, y2 = py[1] / 16.0f - 0.5f;
+  float x3 = px[2] / 16.0f - 0.5f, y3 = py[2] / 16.0f - 0.5f;
+  if (y1 > y2) { swp(y1, y2); swp(x1, x2); }
+  if (y2 > y3) { swp(y2, y3); swp(x2, x3); }
+  if (y1 > y2) { swp(y1, y2); swp(x1, x2); }
+  const float x4 = (x1 + ((y2 - y1) / (y3 - y1)) * (x3 - x1));
+  const float y4 = y2;
+  const float wmax = (float)WC - 1;
+  {
+    float slope1 = (x2 - x1) / (y2 - y1);
+    float slope2 = (x4 - x1) / (y4 - y1);
+    float cx1 = x1, cx2 = x1;
+    if (slope1 < slope2) swp(slope1, slope2);
+    const float sbias = 1.0f + floorf(y1) - y1;
+    if (y1 != y2) {
+      const float yEnd = ceilf(y2);
+      for (int cy = f2i(floorf(y1)); (float)cy <= yEnd; cy++) {
+        float mx1 = cx1 + sbias * slope1;
+        if (slope1 < 0) mx1 -= slope1;
+        float mx2 = cx2 + sbias * slope2;
+        if (slope2 > 0) mx2 -= slope2;
+        mx1 = omin(mx1, wmax);
+        mx2 = omax(mx2, 0.0f);
+        if (cy >= 0 && cy < HC) {
+          const float xe = ceilf(mx1);
+          for (int ax = f2i(floorf(mx2)); (float)ax <= xe; ax++) emit(WC * cy + ax);
+        }
+        cx1 += slope1;
+        cx2 += slope2;
+      }
+    }
+  }
+  {
+    float slope1 = (x3 - x2) / (y3 - y2);
+    float slope2 = (x3 - x4) / (y3 - y4);
+    float cx1 = x3, cx2 = x3;
+    if (slope1 < slope2) swp(slope1, slope2);
+    const float sbias = y3 - floorf(y3);
+    if (y2 != y3) {
+      const float yEnd = floorf(y2);
+      for (int cy = f2i(ceilf(y3)); (float)cy >= yEnd; cy--) {
+        float mx1 = cx1 - sbias * slope1;
+        if (slope1 < 0) mx1 += slope1;
+        float mx2 = cx2 - sbias * slope2;
+        if (slope2 > 0) mx2 += slope2;
+        mx2 = omin(mx2, wmax);
+        mx1 = omax(mx1, 0.0f);
+        if (cy >= 0 && cy < HC) {
+          const float xe = ceilf(mx2);
+          for (int ax = f2i(floorf(mx1)); (float)ax <= xe; ax++) emit(WC * cy + ax);
+        }
+        cx1 -= slope1;
+        cx2 -= slope2;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------ a11: shadow depth walk
+// Shaders/drawmin.c:86-176 (int coordinates, strict top loop).
+template <class F>
+__device__ __forceinline__ void shadow_walk(const AxrShRec& r, int wF, int hF, F&& frag) {
+  int x1 = r.x[0], y1 = r.y[0], x2 = r.x[1], y2 = r.y[1], x3 = r.x[2], y3 = r.y[2];
+  float z1 = r.z[0], z2 = r.z[1], z3 = r.z[2];
+  if (y1 > y2) { swp(y1, y2); swp(x1, x2); swp(z1, z2); }
+  if (y2 > y3) { swp(y2, y3); swp(x2, x3); swp(z2, z3); }
+  if (y1 > y2) { swp(y1, y2); swp(x1, x2); swp(z1, z2); }
+  if (!(y1 < hF && y3 >= 0)) return;
+  const int x4 = f2i((float)x1 + ((float)(y2 - y1) / (float)(y3 - y1)) * (float)(x3 - x1));
+  const int y4 = y2;
+  const float z4 = z1 + (float)(y2 - y1) / (float)(y3 - y1) * (z3 - z1);
+  {
+    float slope1 = (float)(x2 - x1) / (float)(y2 - y1);
+    float slope2 = (float)(x4 - x1) / (float)(y4 - y1);
+    float slopez1 = (z2 - z1) / (float)(y2 - y1);
+    float slopez2 = (z4 - z1) / (float)(y4 - y1);
+    float cx1 = (float)x1, cx2 = cx1, cz1 = z1, cz2 = z1;
+    if (slope1 < slope2) {
+      swp(slope1, slope2);
+      swp(slopez1, slopez2);
+    }
+    for (int cy = y1; cy <= y2; cy++) {
+      if (cy >= hF) break;
+      if (cy >= 0) {
+        const int xe = imin(f2i(cx1), wF - 1);
+        for (int ax = imax(f2i(cx2), 0); ax <= xe; ax++) {
+          float t = span_t(ax, cx1, cx2);
+          frag(ax, cy, span_tz(t, cz1, cz2));
+        }
+      }
+      cx1 += slope1;
+      cx2 += slope2;
+      cz1 += slopez1;
+      cz2 += slopez2;
+    }
+  }
+  {
+    float slope1 = (float)(x3 - x2) / (float)(y3 - y2);
+    float slope2 = (float)(x3 - x4) / (float)(y3 - y4);
+    float slopez1 = (z3 - z2) / (float)(y3 - y2);
+    float slopez2 = (z3 - z4) / (float)(y3 - y4);
+    float cx1 = (float)x3, cx2 = cx1, cz1 = z3, cz2 = z3;
+    if (slope1 < slope2) {
+      swp(slope1, slope2);
+      swp(slopez1, slopez2);
+    }
+    for (int cy = y3; cy > y2; cy--) {
+      if (cy < 0) break;
+      if (cy < hF) {
+        const int xe = imin(f2i(cx2), wF - 1);
+        for (int ax = imax(f2i(cx1), 0); ax <= xe; ax++) {
+          float t = span_t(ax, cx1, cx2);
+          frag(ax, cy, span_tz(t, cz1, cz2));
+        }
+      }
+      cx1 -= slope1;
+      cx2 -= slope2;
+      cz1 -= slopez1;
+      cz2 -= slopez2;
+    }
+  }
+}
+
+// Pipe/trisetupOrtho.c:20-59.
+__device__ __forceinline__ bool shadow_setup(const float4* __restrict__ vpos, uint32_t gi,
+                                             const AxrCam& cam, int dim, float bias,
+                                             AxrShRec& r) {
+  const V3 vp = v3(cam.pos[0], cam.pos[1], cam.pos[2]);
+  const V3 SVd = cam_row(cam, 0), SVx = cam_row(cam, 1), SVy = cam_row(cam, 2);
+  const float half = (float)(dim / 2);
+  bool yes = true;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float4 p = vpos[3 * (size_t)gi + k];
+    V3 x = sub(v3(p.x, p.y, p.z), vp);
+    float dd = dot(x, SVd), dx = dot(x, SVx), dy = dot(x, SVy);
+    dd += bias;
+    dd = 1 / dd;
+    dx = dx * cam.scale + half;
+    dy = dy * -cam.scale + half;
+    r.x[k] = f2i(dx);
+    r.y[k] = f2i(dy);
+    r.z[k] = dd;
+    if (!(dd > 0)) yes = false;
+    if (!(r.x[k] >= -120 && r.x[k] < dim + 120 && r.y[k] >= -120 && r.y[k] < dim + 120))
+      yes = false;
+  }
+  return yes;
+}
+
+// ------------------------------------------------------------- shading
+struct MipInfo {
+  int mip;
+  float fmip;
+  int lenMip, lenMip2, startMip, startMip2;
+};
+
+// av_draw_normmap_mip.c:26-59 on sorted coordinates; mutates a1..a3 through
+// the reference's /= z, *= z round trip (after the 4th vertex is formed).
+template <class T>
+__device__ __forceinline__ MipInfo mip_setup(Attr& a1, Attr& a2, Attr& a3, const Geom<T>& g,
+                                             int numMip, float bias) {
+  a1.u /= g.z1; a1.v /= g.z1; a2.u /= g.z2; a2.v /= g.z2; a3.u /= g.z3; a3.v /= g.z3;
+  a1.px /= g.z1; a1.py /= g.z1; a1.pz /= g.z1;
+  a2.px /= g.z2; a2.py /= g.z2; a2.pz /= g.z2;
+  a3.px /= g.z3; a3.py /= g.z3; a3.pz /= g.z3;
+  const float dU1x = a2.u - a1.u, dU1y = a2.v - a1.v;
+  const float dU2x = a3.u - a1.u, dU2y = a3.v - a1.v;
+  const float d1 = (float)((g.y2 - g.y1) * (g.y2 - g.y1) + (g.x2 - g.x1) * (g.x2 - g.x1));
+  const float d2 = (float)((g.y3 - g.y1) * (g.y3 - g.y1) + (g.x3 - g.x1) * (g.x3 - g.x1));
+  const float d3 = (float)((g.y3 - g.y2) * (g.y3 - g.y2) + (g.x3 - g.x2) * (g.x3 - g.x2));
+  const float s1x = fabsf(dU1x / d1), s1y = fabsf(dU1y / d1);
+  const float s2x = fabsf(dU2x / d2), s2y = fabsf(dU2y / d2);
+  const float s3x = fabsf((a3.u - a2.u) / d3), s3y = fabsf((a3.v - a2.v) / d3);
+  float dt = omax(s1x, omax(s1y, omax(s2x, s2y)));
+  dt = omax(dt, omax(s3x, s3y));
+  float fmip = omax(0.0f, omin((float)(numMip - 1), fabsf(log2f(dt)) / 2 + bias));
+  MipInfo m;
+  m.mip = f2i(fmip);
+  m.fmip = fmip - (float)m.mip;
+  m.lenMip = 1 << m.mip;
+  m.lenMip2 = m.lenMip << 1;
+  m.startMip = (int)(((1u << (2 * m.mip)) - 1u) / 3u);  // sum of 4^i, i < mip
+  m.startMip2 = m.startMip + m.lenMip * m.lenMip;
+  a1.u *= g.z1; a1.v *= g.z1; a2.u *= g.z2; a2.v *= g.z2; a3.u *= g.z3; a3.v *= g.z3;
+  a1.px *= g.z1; a1.py *= g.z1; a1.pz *= g.z1;
+  a2.px *= g.z2; a2.py *= g.z2; a2.pz *= g.z2;
+  a3.px *= g.z3; a3.py *= g.z3; a3.pz *= g.z3;
+  return m;
+}
+
+// Bilinear at one level (av_draw_base.c:29-42, av_draw_normmap_mip.c:73-93).
+__device__ __forceinline__ void bilerp(const ushort4* __restrict__ tex, float tu, float tv, int L,
+                                       int start, float o[3]) {
+  int tex1 = f2i(tu);
+  tu -= (float)tex1;
+  tex1 = abs_mask(tex1, L - 1);
+  int tex2 = f2i(tv);
+  tv -= (float)tex2;
+  tex2 = abs_mask(tex2, L - 1);
+  const int xn = imin(tex1 + 1, L - 1), yn = imin(tex2 + 1, L - 1);
+  const ushort4 t00 = tex[start + tex1 + L * tex2];
+  const ushort4 t10 = tex[start + xn + L * tex2];
+  const ushort4 t01 = tex[start + tex1 + L * yn];
+  const ushort4 t11 = tex[start + xn + L * yn];
+  const float texr1 = tu, texr2 = tv;
+  const float texi1 = 1 - texr1, texi2 = 1 - texr2;
+  const float w00 = texi1 * texi2, w10 = texr1 * texi2, w01 = texi1 * texr2, w11 = texr1 * texr2;
+  o[0] = w00 * (float)t00.x + w10 * (float)t10.x + w01 * (float)t01.x + w11 * (float)t11.x;
+  o[1] = w00 * (float)t00.y + w10 * (float)t10.y + w01 * (float)t01.y + w11 * (float)t11.y;
+  o[2] = w00 * (float)t00.z + w10 * (float)t10.z + w01 * (float)t01.z + w11 * (float)t11.z;
+}
+
+// PCF (drawreflcubemapfresneltexsh.c:162-182 cross, or a 3x3 grid), summed
+// over the bound cascades and capped (av_alpha_max.c:53-84).
+__device__ __forceinline__ float pcf_light(const AxrDrawConst& dc, V3 wpos) {
+  const int K = dc.pcf_taps == 9 ? 9 : 5;
+  const int d = 2;  // shDist
+  int shadow = 0;
+  for (int c = 0; c < dc.nshadow; c++) {
+    const AxrShadowDev& sm = dc.sh[c];
+    const V3 pos = sub(wpos, v3(sm.cam.pos[0], sm.cam.pos[1], sm.cam.pos[2]));
+    const float depth = dot(pos, cam_row(sm.cam, 0));
+    const int wS = sm.wS;
+    const int sx = f2i(dot(pos, cam_row(sm.cam, 1)) * sm.cam.scale) + wS;
+    const int sy = f2i(dot(pos, cam_row(sm.cam, 2)) * -sm.cam.scale) + wS;
+    if (sx >= d && sx < 2 * wS - d && sy >= d && sy < 2 * wS - d) {
+      const uint32_t* SD = sm.map;
+      const long long st = 2LL * wS;
+      if (K == 5) {
+        if (unordf(SD[st * sy + sx]) < depth) shadow += 1;
+        if (unordf(SD[st * (sy + d) + sx]) < depth) shadow += 1;
+        if (unordf(SD[st * (sy - d) + sx]) < depth) shadow += 1;
+        if (unordf(SD[st * sy + (sx + d)]) < depth) shadow += 1;
+        if (unordf(SD[st * sy + (sx - d)]) < depth) shadow += 1;
+      } else {
+#pragma unroll
+        for (int oy = -d; oy <= d; oy += d)
+#pragma unroll
+          for (int ox = -d; ox <= d; ox += d)
+            if (unordf(SD[st * (sy + oy) + (sx + ox)]) < depth) shadow += 1;
+      }
+    }
+  }
+  shadow = imin(K, shadow);
+  float light = (float)shadow * dc.amb + (float)(K - shadow);
+  if (K == 5)
+    light *= 0.2f;
+  else
+    light = light / 9.0f;
+  return light;
+}
+
+// One fragment's HDR colour (u16 per channel).
+__device__ __forceinline__ void shade(const AxrDrawConst& dc, const AxrGroupDev& grp,
+                                      const MipInfo& mi, float t, float tz, const Attr& c1,
+                                      const Attr& c2, uint16_t rgb[3]) {
+  const float u = 1 - t;
+  float L[3];
+  const float fu = (u * c2.u + t * c1.u) * tz;
+  const float fv = (u * c2.v + t * c1.v) * tz;
+  if (grp.flags & AXR_SH_MIP) {
+    float o1[3], o2[3];
+    bilerp(grp.tex, fu * (float)mi.lenMip, fv * (float)mi.lenMip, mi.lenMip, mi.startMip, o1);
+    bilerp(grp.tex, fu * (float)mi.lenMip2, fv * (float)mi.lenMip2, mi.lenMip2, mi.startMip2, o2);
+#pragma unroll
+    for (int c = 0; c < 3; c++) L[c] = (1 - mi.fmip) * o1[c] + mi.fmip * o2[c];
+  } else {
+    bilerp(grp.tex, fu * (float)grp.tex_size, fv * (float)grp.tex_size, grp.tex_size, 0, L);
+  }
+  const V3 wpos = v3((u * c2.px + t * c1.px) * tz, (u * c2.py + t * c1.py) * tz,
+                     (u * c2.pz + t * c1.pz) * tz);
+  float light = 1;
+  if ((grp.flags & AXR_SH_SHADOW) && dc.nshadow > 0) light = pcf_light(dc, wpos);
+  const float col[3] = {(u * c2.ix + t * c1.ix) * tz, (u * c2.iy + t * c1.iy) * tz,
+                        (u * c2.iz + t * c1.iz) * tz};
+  const V3 nrm = normalize(v3((u * c2.nx + t * c1.nx) * tz, (u * c2.ny + t * c1.ny) * tz,
+                              (u * c2.nz + t * c1.nz) * tz));
+  const V3 LDir = v3(dc.ldir[0], dc.ldir[1], dc.ldir[2]);
+  const float nl = omax(0.0f, dot(nrm, LDir));
+  const float dcol[3] = {dc.lint[0] * nl, dc.lint[1] * nl, dc.lint[2] * nl};
+  if (grp.flags & AXR_SH_PHONG) {
+    // av_draw_normmap.c:276-284, 303-310 (without the environment terms)
+    const V3 a = normalize(sub(wpos, v3(dc.view.pos[0], dc.view.pos[1], dc.view.pos[2])));
+    const V3 h = normalize(add(a, LDir));
+    const float theta = omax(0.0f, dot(h, nrm));
+    const float sp = pown(theta, grp.spec_pow) * 256 * 4.0f * light;
+    const float k = (float)(grp.spec_pow + 2) / (8 * 3.1416f);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const float spec = (dc.lint[c] * sp) * k;
+      rgb[c] = f2u16(spec + L[c] * (light * dcol[c] + col[c]));
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      rgb[c] = f2u16(L[c] * (light * (dcol[c] + col[c]) + (1 - light) * col[c]));
+  }
+}
+
+// Shaders/Post/gamma.c:13-29 on one pixel.
+__device__ __forceinline__ void gamma_px(float ex, uint16_t rgb[3]) {
+  const float k = 8 * ex;
+  float j0 = sqrtf(k * (float)rgb[0]) / 1.1f;
+  float j1 = sqrtf(k * (float)rgb[1]) / 1.1f;
+  float j2 = sqrtf(k * (float)rgb[2]) / 1.1f;
+  const float m = omax(j0, omax(j1, j2)) / 255.0f;
+  if (m > 1.0f) {
+    const float s = sqrtf(m);
+    j0 = j0 / s;
+    j1 = j1 / s;
+    j2 = j2 / s;
+  }
+  rgb[0] = (uint16_t)omin(255.0f, j0);
+  rgb[1] = (uint16_t)omin(255.0f, j1);
+  rgb[2] = (uint16_t)omin(255.0f, j2);
+}
+
+}  // namespace axr

@@ -1,0 +1,302 @@
+// shade.hip -- resolve + shade pass of the gfx950 raster path, and the small
+// per-pixel / per-vertex kernels around it (gamma, readback helpers, vertL).
+//
+// The depth pass leaves one 64-bit key per pixel naming the winning
+// fragment (triangle, pass, scanline half).  Here each pixel is shaded
+// exactly once: the winner's setup and scanline walk are replayed up to the
+// pixel's row -- the same accumulation sequence the walk performed, so the
+// interpolants are bit-identical to the reference's at that fragment -- and
+// the colour is written with fully coalesced stores (RGBA8 with gamma fused,
+// or HDR u16 planes).  Overdraw is never shaded.
+#include "axr_launch.h"
+#include "axr_tri.h"
+
+using namespace axr;
+
+namespace {
+
+__device__ __forceinline__ int find_group(const AxrGroupDev* __restrict__ groups, int ng,
+                                          uint32_t gi) {
+  int lo = 0, hi = ng - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (groups[mid].base <= gi)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Rebuilds the winning fragment at (x, y) and shades it.
+__device__ __noinline__ void resolve_pixel(uint64_t key, int x, int y,
+                                           const float4* __restrict__ vpos,
+                                           const float4* __restrict__ vattr,
+                                           const AxrGroupDev* __restrict__ groups,
+                                           const AxrDrawConst& dc, uint16_t rgb[3]) {
+  const uint32_t seq = ~(uint32_t)key;
+  const uint32_t order = (seq - 1u) >> 1;
+  const int half = (int)((seq - 1u) & 1u);
+  const bool large = order >= dc.T;
+  const uint32_t gi = large ? order - dc.T : order;
+  const int gidx = find_group(groups, (int)dc.ngroups, gi);
+  AxrGroupDev grp = groups[gidx];
+  grp.flags = dc.gflags[gidx];
+
+  float px[3], py[3], z[3];
+  tri_setup(vpos, vattr, gi, dc.view, dc.W, dc.H, (grp.flags & AXR_SH_CULL) != 0, px, py, z);
+  Attr A[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) A[k] = load_attr(vpos, vattr, 3 * gi + k, z[k]);
+
+  float cx1, cx2, cz1, cz2;
+  Attr c1, c2;
+  MipInfo mi = {0, 0.0f, 0, 0, 0, 0};
+  if (!large) {
+    const Geom<int> g = small_geom(px, py, z);
+    Attr a1 = A[g.k1], a2 = A[g.k2], a3 = A[g.k3];
+    const Attr a4 = a_lerp4(a1, a3, g.yd);
+    if (grp.flags & AXR_SH_MIP) mi = mip_setup(a1, a2, a3, g, grp.tex_size, grp.mip_bias);
+    const Edges e = small_edges(g, half);
+    if (half == 0) {
+      Attr s1 = a_slope(a2, a1, e.inv), s2 = a_slope(a4, a1, e.inv);
+      if (e.swapped) swp(s1, s2);
+      cx1 = cx2 = (float)g.x1;
+      cz1 = cz2 = g.z1;
+      c1 = c2 = a1;
+      for (int r = g.y1; r < y; r++) {
+        cx1 += e.s1; cx2 += e.s2; cz1 += e.sz1; cz2 += e.sz2;
+        a_inc(c1, s1);
+        a_inc(c2, s2);
+      }
+    } else {
+      Attr s1 = a_nslope(a2, a3, e.inv), s2 = a_nslope(a4, a3, e.inv);
+      if (e.swapped) swp(s1, s2);
+      cx1 = cx2 = (float)g.x3;
+      cz1 = cz2 = g.z3;
+      c1 = c2 = a3;
+      for (int r = g.y3; r > y; r--) {
+        cx1 -= e.s1; cx2 -= e.s2; cz1 -= e.sz1; cz2 -= e.sz2;
+        a_dec(c1, s1);
+        a_dec(c2, s2);
+      }
+    }
+  } else {
+    const Geom<float> g = tile_geom(px, py, z);
+    Attr a1 = A[g.k1], a2 = A[g.k2], a3 = A[g.k3];
+    const Attr a4 = a_lerp4(a1, a3, g.yd);
+    if (grp.flags & AXR_SH_MIP) mi = mip_setup(a1, a2, a3, g, grp.tex_size, grp.mip_bias);
+    const TileBox box = tile_box((y >> 4) * dc.WC + (x >> 4), dc.W);
+    const Edges e = tile_edges(g, half);
+    if (half == 0) {
+      Attr s1 = a_slope(a2, a1, e.inv), s2 = a_slope(a4, a1, e.inv);
+      if (e.swapped) swp(s1, s2);
+      int cy = f2i(oclamp(g.y1, box.yMin, box.yMax));
+      const float dy = (float)cy - g.y1;
+      cx1 = g.x1 + dy * e.s1; cx2 = g.x1 + dy * e.s2;
+      cz1 = g.z1 + dy * e.sz1; cz2 = g.z1 + dy * e.sz2;
+      c1 = a_mad(a1, dy, s1);
+      c2 = a_mad(a1, dy, s2);
+      for (; cy < y; cy++) {
+        cx1 += e.s1; cx2 += e.s2; cz1 += e.sz1; cz2 += e.sz2;
+        a_inc(c1, s1);
+        a_inc(c2, s2);
+      }
+    } else {
+      Attr s1 = a_nslope(a2, a3, e.inv), s2 = a_nslope(a4, a3, e.inv);
+      if (e.swapped) swp(s1, s2);
+      int cy = f2i(oclamp(g.y3, box.yMin, box.yMax));
+      const float dy = (float)cy - g.y3;
+      cx1 = g.x3 + dy * e.s1; cx2 = g.x3 + dy * e.s2;
+      cz1 = g.z3 + dy * e.sz1; cz2 = g.z3 + dy * e.sz2;
+      c1 = a_mad(a3, dy, s1);
+      c2 = a_mad(a3, dy, s2);
+      for (; cy > y; cy--) {
+        cx1 -= e.s1; cx2 -= e.s2; cz1 -= e.sz1; cz2 -= e.sz2;
+        a_dec(c1, s1);
+        a_dec(c2, s2);
+      }
+    }
+  }
+  const float t = span_t(x, cx1, cx2);
+  const float tz = span_tz(t, cz1, cz2);
+  shade(dc, grp, mi, t, tz, c1, c2, rgb);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_shade(const unsigned long long* __restrict__ keys,
+                                               const float4* __restrict__ vpos,
+                                               const float4* __restrict__ vattr,
+                                               const AxrGroupDev* __restrict__ groups,
+                                               AxrDrawConst dc, float exposure,
+                                               float* __restrict__ depth,
+                                               uint16_t* __restrict__ ro, uint16_t* __restrict__ go,
+                                               uint16_t* __restrict__ bo,
+                                               uint32_t* __restrict__ rgba) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= dc.W || y >= dc.H) return;
+  const size_t p = (size_t)y * dc.W + x;
+  const uint64_t key = keys[p];
+  uint16_t rgb[3] = {dc.clear_rgb[0], dc.clear_rgb[1], dc.clear_rgb[2]};
+  float d = AXR_DEPTH_CLEAR;
+  if ((uint32_t)key != 0xFFFFFFFFu) {
+    d = unordf((uint32_t)(key >> 32));
+    resolve_pixel(key, x, y, vpos, vattr, groups, dc, rgb);
+  }
+  depth[p] = d;
+  if (MODE == SHADE_LDR8) {
+    gamma_px(exposure, rgb);
+    rgba[p] = (uint32_t)rgb[0] | ((uint32_t)rgb[1] << 8) | ((uint32_t)rgb[2] << 16) | 0xFF000000u;
+  } else {
+    ro[p] = rgb[0];
+    go[p] = rgb[1];
+    bo[p] = rgb[2];
+  }
+}
+
+__global__ void k_gamma_planes(uint16_t* __restrict__ r, uint16_t* __restrict__ g,
+                               uint16_t* __restrict__ b, size_t n, float ex) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    uint16_t v[3] = {r[i], g[i], b[i]};
+    gamma_px(ex, v);
+    r[i] = v[0];
+    g[i] = v[1];
+    b[i] = v[2];
+  }
+}
+
+__global__ void k_expand_rgba8(const uint32_t* __restrict__ rgba, uint16_t* __restrict__ r,
+                               uint16_t* __restrict__ g, uint16_t* __restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t v = rgba[i];
+    r[i] = (uint16_t)(v & 0xFF);
+    g[i] = (uint16_t)((v >> 8) & 0xFF);
+    b[i] = (uint16_t)((v >> 16) & 0xFF);
+  }
+}
+
+__global__ void k_ids(const unsigned long long* __restrict__ keys, uint32_t T,
+                      uint32_t* __restrict__ ids, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t low = (uint32_t)keys[i];
+    uint32_t id = AXR_NONE;
+    if (low != 0xFFFFFFFFu) {
+      const uint32_t order = ((~low) - 1u) >> 1;
+      id = order >= T ? order - T : order;
+    }
+    ids[i] = id;
+  }
+}
+
+__global__ void k_decode_shadow(const uint32_t* __restrict__ map, float* __restrict__ out,
+                                size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = unordf(map[i]);
+}
+
+// Pipe/vert.c:52-86: per-vertex light from directional lights 1.., point
+// and spot lights, floored at the ambient level.
+__global__ void k_vertlight(const float4* __restrict__ vpos, float4* __restrict__ vattr,
+                            uint32_t v0, uint32_t nv, AxrLights L) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const uint32_t v = v0 + i;
+  const float4 p4 = vpos[v];
+  const float4 a0 = vattr[2 * (size_t)v];
+  const V3 x = v3(p4.x, p4.y, p4.z), n = v3(a0.x, a0.y, a0.z);
+  V3 light = v3(0.0f, 0.0f, 0.0f);
+  for (int k = 1; k < L.ndir; k++) {
+    const V3 d = v3(L.dirD[3 * k], L.dirD[3 * k + 1], L.dirD[3 * k + 2]);
+    const V3 c = v3(L.dirI[3 * k], L.dirI[3 * k + 1], L.dirI[3 * k + 2]);
+    const float s = omax(0.0f, dot(n, d) + 0.5f) * 0.66f;
+    light = add(light, scl(c, s));
+  }
+  for (int k = 0; k < L.npoint; k++) {
+    const V3 pl = sub(x, v3(L.pointP[3 * k], L.pointP[3 * k + 1], L.pointP[3 * k + 2]));
+    if (dot(n, pl) > 0.0f) {
+      const float len = sqrtf(dot(pl, pl));
+      const float s = dot(n, normalize(pl)) / (1.0f + len * len);
+      light = add(light, scl(v3(L.pointI[3 * k], L.pointI[3 * k + 1], L.pointI[3 * k + 2]), s));
+    }
+  }
+  for (int k = 0; k < L.nspot; k++) {
+    const V3 pl = sub(x, v3(L.spotP[3 * k], L.spotP[3 * k + 1], L.spotP[3 * k + 2]));
+    const V3 sd = v3(L.spotD[3 * k], L.spotD[3 * k + 1], L.spotD[3 * k + 2]);
+    if (dot(sd, pl) > 0.0f && dot(n, pl) > 0.0f) {
+      const float len = sqrtf(dot(pl, pl));
+      const V3 npl = normalize(pl);
+      const float s = dot(n, npl) * dot(npl, sd) / (1.5f + len * len);
+      light = add(light, scl(v3(L.spotI[3 * k], L.spotI[3 * k + 1], L.spotI[3 * k + 2]), s));
+    }
+  }
+  float4 a1 = vattr[2 * (size_t)v + 1];
+  a1.y = omax(light.x, L.amb);
+  a1.z = omax(light.y, L.amb);
+  a1.w = omax(light.z, L.amb);
+  vattr[2 * (size_t)v + 1] = a1;
+}
+
+inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
+  size_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+namespace axr {
+
+hipError_t launch_shade(int mode, const uint64_t* keys, const float4* vpos, const float4* vattr,
+                        const AxrGroupDev* groups, const AxrDrawConst& dc, float exposure,
+                        float* depth, uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
+                        hipStream_t s) {
+  dim3 grid((dc.W + 15) / 16, (dc.H + 15) / 16);
+  const unsigned long long* k = (const unsigned long long*)keys;
+  if (mode == SHADE_LDR8)
+    hipLaunchKernelGGL(k_shade<SHADE_LDR8>, grid, dim3(256), 0, s, k, vpos, vattr, groups, dc,
+                       exposure, depth, r, g, b, rgba);
+  else
+    hipLaunchKernelGGL(k_shade<SHADE_HDR>, grid, dim3(256), 0, s, k, vpos, vattr, groups, dc,
+                       exposure, depth, r, g, b, rgba);
+  return hipGetLastError();
+}
+
+hipError_t launch_gamma_planes(uint16_t* r, uint16_t* g, uint16_t* b, size_t n, float ex,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_gamma_planes, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, r, g, b, n,
+                     ex);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, uint16_t* b,
+                               size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_expand_rgba8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rgba, r, g,
+                     b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_ids(const uint64_t* keys, uint32_t T, uint32_t* ids, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_ids, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s,
+                     (const unsigned long long*)keys, T, ids, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_decode_shadow, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, map, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_vertlight(const float4* vpos, float4* vattr, uint32_t v0, uint32_t nv,
+                            const AxrLights& L, hipStream_t s) {
+  if (nv == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vertlight, dim3((nv + 255) / 256), dim3(256), 0, s, vpos, vattr, v0, nv, L);
+  return hipGetLastError();
+}
+
+}  // namespace axr

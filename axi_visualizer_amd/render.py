@@ -1,0 +1,53 @@
+"""Drives a Scene through CLDraw the way the reference engine does.
+
+``Renderer.frame()`` issues the per-frame call sequence of
+``ThreeDBackend.render`` (Compute.py:308-339) with the shadow pass of
+``CLDraw.shadowMap`` (Ops.py:1089-1141) in front and ``gamma`` as the
+post-process: clearShadowMap + shadowMap per cascade, setPos/setVM,
+clearZBuffer, drawAll, gamma.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .ops import CLDraw
+from .scenes import Scene
+
+
+class Renderer:
+    def __init__(self, scene: Scene):
+        self.scene = sc = scene
+        self.draw = d = CLDraw(1, len(sc.groups), sc.W, sc.H, 1)
+        d.setScaleCull(sc.view.scale, sc.view.cax, sc.view.cay)
+        for g in sc.groups:
+            planes = [g.tex[:, c] if g.mip else g.tex[:, c].reshape(g.tex_side, g.tex_side)
+                      for c in range(3)]
+            tn = d.addTextureGroup(g.xyz, g.uv, g.vn, *planes,
+                                   mip=g.tex_side if g.mip else False)
+            d.setGroupMaterial(tn, g.mip_bias, g.spec_pow)
+        d.ambLight = np.float32(sc.amb)
+        d.setPrimaryLight(sc.light_int[None], sc.light_dir[None])
+        # one directional light: vertL leaves I at the ambient floor
+        d.vertLight([True] * len(sc.groups), sc.light_int[None], sc.light_dir[None])
+        for i, c in enumerate(sc.cascades):
+            d.addShadowMap(i, c.dim, c.cam.scale, sc.amb)
+            d.placeShadowMap(i, c.cam.pos, c.cam.vm, sc.amb)
+        self.shaders = [g.flags for g in sc.groups]
+        self.mask = [g.skip for g in sc.groups]
+        self.cast = [g.cast for g in sc.groups]
+
+    def frame(self, gamma=True):
+        sc, d = self.scene, self.draw
+        for i, c in enumerate(sc.cascades):
+            d.clearShadowMap(i)
+            d.shadowMap(i, self.cast, self.shaders, c.bias)
+        d.setPos(sc.view.pos)
+        d.setVM(sc.view.vm)
+        d.clearZBuffer()
+        d.drawAll(self.shaders, mask=self.mask, shadowIds=list(range(len(sc.cascades))),
+                  pcf=sc.pcf_taps)
+        if gamma:
+            d.gamma(sc.exposure)
+
+    def close(self):
+        self.draw.close()

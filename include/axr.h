@@ -1,0 +1,136 @@
+/*
+ * axr.h -- C ABI of libaxr.so, the MI355X (gfx950) rendering hot path of
+ * AXI Visualizer.
+ *
+ * The reference drives its OpenCL kernels from Python through PyOpenCL
+ * (PyOpenCL27C/Ops.py, class CLDraw); it has no native FFI.  This ABI is the
+ * boundary that replaces those PyOpenCL launches: each entry point below
+ * names the CLDraw method whose device work it performs (reference
+ * file:line).  The Python shim axi_visualizer_amd.CLDraw keeps the
+ * reference's method names and argument order and calls these through
+ * ctypes.
+ *
+ * Conventions: plain pointers and sizes; host arrays are borrowed for the
+ * call only (copied before return); readbacks fill caller-allocated arrays
+ * and synchronise the context's stream.  Every call returns 0 on success or
+ * a negative AXR_E* code; axr_last_error() gives the message of the last
+ * failing call on the calling thread.  Calls on one context must be
+ * serialised by the caller; separate contexts may be used from separate
+ * threads.  Device memory lives for the context's lifetime.
+ */
+#ifndef AXR_H
+#define AXR_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXR_OK 0
+#define AXR_EINVAL (-1)  /* bad argument */
+#define AXR_EHIP (-2)    /* HIP runtime error */
+#define AXR_ESTATE (-3)  /* call out of order (e.g. drawAll without clear) */
+#define AXR_ECAP (-4)    /* a device capacity was exceeded */
+
+/* Shader flags per material group (the reference selects kernels by
+ * substrings of the group's shader string, Ops.py:768-805, 843-899). */
+#define AXR_SH_MIP 1u    /* "mip": trilinear, av_draw_normmap_mip.c:26-123 */
+#define AXR_SH_SHADOW 2u /* "shadow": PCF receive, av_alpha_max.c:53-84 */
+#define AXR_SH_PHONG 4u  /* Blinn-Phong term, av_draw_normmap.c:276-310 */
+#define AXR_SH_CULL 8u   /* "cull": back-face rejection, trisetup_cull.c:49-54 */
+
+typedef struct axr_ctx axr_ctx;
+
+const char* axr_last_error(void);
+
+/* CLDraw.__init__(size_sky, max_uv, w, h, max_particles), Ops.py:128-229.
+ * w must be a multiple of 16 (the 16-px tile grid, Ops.py:131). */
+int axr_create(int w, int h, axr_ctx** out);
+int axr_destroy(axr_ctx* ctx);
+
+/* CLDraw.setScaleCull(s, cx, cy), Ops.py:231-233. */
+int axr_set_scale_cull(axr_ctx* ctx, float s, float cax, float cay);
+
+/* CLDraw.addTextureGroup(xyz, uv, vn, r, g, b, mip), Ops.py:331-378.
+ * xyz, vn: [3*ntri][3] f32; uv: [3*ntri][2] f32; r, g, b: ntexels u16
+ * planes (N*N, or the smallest-first mip chain of Utils.createMips);
+ * tex_size is the reference's texSize: N, or log2(N) when is_mip.
+ * Returns the group index (tn) in *tn_out. */
+int axr_add_texture_group(axr_ctx* ctx, const float* xyz, const float* uv, const float* vn,
+                          int ntri, const uint16_t* r, const uint16_t* g, const uint16_t* b,
+                          int64_t ntexels, int tex_size, int is_mip, int* tn_out);
+
+/* Per-group material constants the reference passes as shader arguments
+ * (mipBias, av_draw_normmap_mip.c:21; specPow, av_draw_normmap.c:282). */
+int axr_set_group_material(axr_ctx* ctx, int tn, float mip_bias, int spec_pow);
+
+/* CLDraw.setPrimaryLight(dirI, dirD), Ops.py:1143-1151. */
+int axr_set_primary_light(axr_ctx* ctx, const float intensity[3], const float dir[3]);
+
+/* CLDraw.vertLight(mask, dirI, dirD, pointI, pointP, spotI, spotD, spotP),
+ * Ops.py:552-612 -> Pipe/vert.c:vertL.  mask[tn] != 0 selects the groups
+ * lit; arrays are [n][3] f32; amb is the ambient floor (self.ambLight). */
+int axr_vert_light(axr_ctx* ctx, const uint8_t* mask, int ngroups, const float* dirI,
+                   const float* dirD, int ndir, const float* pointI, const float* pointP,
+                   int npoint, const float* spotI, const float* spotD, const float* spotP,
+                   int nspot, float amb);
+
+/* CLDraw.setPos(vc), setVM(vM), Ops.py:475-479.  vm rows: forward,
+ * horizontal, vertical (Compute.py:313-314). */
+int axr_set_pos(axr_ctx* ctx, const float pos[3]);
+int axr_set_vm(axr_ctx* ctx, const float vm[9]);
+
+/* CLDraw.clearZBuffer(), Ops0.py:650-653 / Ops.py:939-946: depth to
+ * 65535, colour to the clear colour (default (60,110,255)^2/3). */
+int axr_clear_zbuffer(axr_ctx* ctx);
+int axr_set_clear_color(axr_ctx* ctx, const uint16_t rgb[3]);
+
+/* CLDraw.drawAll(shaders, mask, shadowIds), Ops.py:716-937 (Ops0.py:471-648).
+ * shader_flags[tn]: AXR_SH_* bits; mask[tn] != 0 skips the group (the
+ * reference's mask sense).  shadow_ids: shadow maps sampled by "shadow"
+ * groups (at most 2).  pcf_taps: 5 (reference cross) or 9 (3x3 grid).
+ * One drawAll per clearZBuffer. */
+int axr_draw_all(axr_ctx* ctx, const uint32_t* shader_flags, const uint8_t* mask, int ngroups,
+                 const int* shadow_ids, int nshadow, int pcf_taps);
+
+/* Shadow maps, Ops.py:1022-1141: addShadowMap(i, size, scale, ambLight),
+ * placeShadowMap(i, pos, facing) with vm = viewMat(*facing),
+ * clearShadowMap(i), shadowMap(i, whichCast, shaders, bias) (ortho). */
+int axr_add_shadow_map(axr_ctx* ctx, int i, int size, float scale, float amb);
+int axr_place_shadow_map(axr_ctx* ctx, int i, const float pos[3], const float vm[9], float amb);
+int axr_clear_shadow_map(axr_ctx* ctx, int i);
+int axr_shadow_map(axr_ctx* ctx, int i, const uint8_t* cast, int ngroups, float bias);
+
+/* CLDraw.gamma(ex), Ops.py:947-957 -> Shaders/Post/gamma.c. */
+int axr_gamma(axr_ctx* ctx, float ex);
+
+/* CLDraw.getFrame(), Ops.py:1224-1230: R, G, B u16 planes and f32 depth,
+ * each [h][w].  Any pointer may be NULL. */
+int axr_get_frame(axr_ctx* ctx, uint16_t* r, uint16_t* g, uint16_t* b, float* depth);
+/* The frame as packed RGBA8 (after gamma), [h][w] u32. */
+int axr_get_rgba8(axr_ctx* ctx, uint32_t* rgba);
+/* Depth-winning global triangle index per pixel (0xFFFFFFFF: none);
+ * global index = sum of earlier groups' ntri + index within the group. */
+int axr_get_ids(axr_ctx* ctx, uint32_t* ids);
+/* CLDraw.getSHM(i), Ops.py:1216-1222: shadow map depths, [size][size] f32. */
+int axr_get_shadow_map(axr_ctx* ctx, int i, float* out);
+
+/* Waits for the context's stream. */
+int axr_sync(axr_ctx* ctx);
+
+/* Counters of the last drawAll: [0] small tris, [1] large tris,
+ * [2] shadow casters of the last shadowMap, [3] bin entries, [4] bin
+ * overflow (0 if none).  Synchronises. */
+int axr_get_stats(axr_ctx* ctx, uint32_t out[5]);
+
+/* Per-kernel timing with HIP events on the context's stream.
+ * axr_set_timing(ctx, 1) starts recording; axr_get_timing copies up to n
+ * (name, total_ms, launches) rows and returns the row count. */
+int axr_set_timing(axr_ctx* ctx, int enable);
+int axr_get_timing(axr_ctx* ctx, char* names, int name_len, double* total_ms, int64_t* count,
+                   int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

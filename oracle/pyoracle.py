@@ -1,0 +1,245 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, where it is the checker (or the timed CPU
+baseline), never the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ORC_MIP, ORC_SHADOW, ORC_PHONG, ORC_CULL = 1, 2, 4, 8
+
+
+class OrcGroup(ctypes.Structure):
+    _fields_ = [("ntri", ctypes.c_int), ("xyz", ctypes.c_void_p), ("vn", ctypes.c_void_p),
+                ("uv", ctypes.c_void_p), ("li", ctypes.c_void_p), ("tr", ctypes.c_void_p),
+                ("tg", ctypes.c_void_p), ("tb", ctypes.c_void_p), ("tex_size", ctypes.c_int),
+                ("flags", ctypes.c_uint32), ("mip_bias", ctypes.c_float),
+                ("spec_pow", ctypes.c_int), ("skip", ctypes.c_int), ("cast", ctypes.c_int)]
+
+
+class OrcCam(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("vm", ctypes.c_float * 9),
+                ("scale", ctypes.c_float), ("cax", ctypes.c_float), ("cay", ctypes.c_float)]
+
+
+class OrcShadowCfg(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("cam", OrcCam), ("bias", ctypes.c_float)]
+
+
+class OrcFrame(ctypes.Structure):
+    _fields_ = [("W", ctypes.c_int), ("H", ctypes.c_int), ("view", OrcCam),
+                ("lint", ctypes.c_float * 3), ("ldir", ctypes.c_float * 3),
+                ("amb", ctypes.c_float), ("nshadow", ctypes.c_int),
+                ("shadow", OrcShadowCfg * 2), ("pcf_taps", ctypes.c_int),
+                ("exposure", ctypes.c_float), ("clear_rgb", ctypes.c_uint16 * 3),
+                ("nthreads", ctypes.c_int)]
+
+
+class OrcOut(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_void_p), ("id", ctypes.c_void_p), ("r", ctypes.c_void_p),
+                ("g", ctypes.c_void_p), ("b", ctypes.c_void_p),
+                ("shadow", ctypes.c_void_p * 2), ("stats", ctypes.c_int64 * 8)]
+
+
+_lib = None
+
+
+def build() -> str:
+    """Builds liboracle.so with its Makefile (g++, -ffp-contract=off)."""
+    res = subprocess.run(["make", "-C", HERE, "liboracle.so"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        p, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+        L.orc_render.argtypes = [p, i, p, p]
+        L.orc_trisetup.argtypes = [p, p, i, p, i, i, i, p, p, p]
+        L.orc_shadow_setup.argtypes = [p, i, p, i, f, p, p, p]
+        L.orc_coarse_tri.argtypes = [p, i, i, p, i]
+        L.orc_fragments.argtypes = [i, p, p, i, i, i, p, p, p, p, i]
+        L.orc_gamma.argtypes = [p, p, p, ctypes.c_int64, f]
+        L.orc_create_mips.argtypes = [p, i, p]
+        L.orc_vertlight.argtypes = [p, p, i, p, p, i, p, p, i, p, p, p, i, f, p]
+        for fn in ("orc_render", "orc_trisetup", "orc_shadow_setup", "orc_coarse_tri",
+                   "orc_fragments", "orc_gamma", "orc_create_mips", "orc_vertlight"):
+            getattr(L, fn).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype))
+
+
+def make_cam(cam) -> OrcCam:
+    c = OrcCam()
+    c.pos[:] = [float(x) for x in np.asarray(cam.pos, np.float32)]
+    c.vm[:] = [float(x) for x in np.asarray(cam.vm, np.float32).reshape(9)]
+    c.scale = float(np.float32(cam.scale))
+    c.cax = float(np.float32(cam.cax))
+    c.cay = float(np.float32(cam.cay))
+    return c
+
+
+def vertlight(xyz, vn, dirI, dirD, amb, pointI=None, pointP=None, spotI=None, spotD=None,
+              spotP=None):
+    """Pipe/vert.c vertL over (nv, 3) vertices."""
+    xyz = _c(xyz, np.float32).reshape(-1, 3)
+    vn = _c(vn, np.float32).reshape(-1, 3)
+    out = np.empty_like(xyz)
+    dI = _c(dirI, np.float32).reshape(-1, 3)
+    dD = _c(dirD, np.float32).reshape(-1, 3)
+    z = np.zeros((0, 3), np.float32)
+    pI = z if pointI is None else _c(pointI, np.float32).reshape(-1, 3)
+    pP = z if pointP is None else _c(pointP, np.float32).reshape(-1, 3)
+    sI = z if spotI is None else _c(spotI, np.float32).reshape(-1, 3)
+    sD = z if spotD is None else _c(spotD, np.float32).reshape(-1, 3)
+    sP = z if spotP is None else _c(spotP, np.float32).reshape(-1, 3)
+    lib().orc_vertlight(xyz.ctypes.data, vn.ctypes.data, xyz.shape[0], dI.ctypes.data,
+                        dD.ctypes.data, dI.shape[0], pI.ctypes.data, pP.ctypes.data,
+                        pI.shape[0], sI.ctypes.data, sD.ctypes.data, sP.ctypes.data,
+                        sI.shape[0], float(amb), out.ctypes.data)
+    return out
+
+
+def render(scene, nthreads=0, with_shadows_out=False):
+    """Renders a scenes.Scene: shadow cascades, clear, drawAll, gamma.
+    Returns dict(depth, id, r, g, b, stats[, shadow])."""
+    keep = []
+    groups = (OrcGroup * len(scene.groups))()
+    for k, g in enumerate(scene.groups):
+        xyz = _c(g.xyz, np.float32)
+        vn = _c(g.vn, np.float32)
+        uv = _c(g.uv, np.float32)
+        li = vertlight(xyz, vn, scene.light_int[None], scene.light_dir[None], scene.amb)
+        tr = _c(g.tex[:, 0], np.uint16)
+        tg = _c(g.tex[:, 1], np.uint16)
+        tb = _c(g.tex[:, 2], np.uint16)
+        keep += [xyz, vn, uv, li, tr, tg, tb]
+        G = groups[k]
+        G.ntri = g.ntri
+        G.xyz, G.vn, G.uv, G.li = xyz.ctypes.data, vn.ctypes.data, uv.ctypes.data, li.ctypes.data
+        G.tr, G.tg, G.tb = tr.ctypes.data, tg.ctypes.data, tb.ctypes.data
+        G.tex_size = g.tex_size
+        G.flags = int(g.flags)
+        G.mip_bias = float(g.mip_bias)
+        G.spec_pow = int(g.spec_pow)
+        G.skip = 1 if g.skip else 0
+        G.cast = 1 if g.cast else 0
+    fr = OrcFrame()
+    fr.W, fr.H = scene.W, scene.H
+    fr.view = make_cam(scene.view)
+    fr.lint[:] = [float(x) for x in np.asarray(scene.light_int, np.float32)]
+    fr.ldir[:] = [float(x) for x in np.asarray(scene.light_dir, np.float32)]
+    fr.amb = float(np.float32(scene.amb))
+    fr.nshadow = len(scene.cascades)
+    for i, c in enumerate(scene.cascades):
+        fr.shadow[i].dim = c.dim
+        fr.shadow[i].cam = make_cam(c.cam)
+        fr.shadow[i].bias = float(np.float32(c.bias))
+    fr.pcf_taps = scene.pcf_taps
+    fr.exposure = float(scene.exposure)
+    fr.clear_rgb[:] = list(scene.clear_rgb)
+    fr.nthreads = nthreads
+    n = scene.W * scene.H
+    out = {"depth": np.empty((scene.H, scene.W), np.float32),
+           "id": np.empty((scene.H, scene.W), np.uint32),
+           "r": np.empty((scene.H, scene.W), np.uint16),
+           "g": np.empty((scene.H, scene.W), np.uint16),
+           "b": np.empty((scene.H, scene.W), np.uint16)}
+    o = OrcOut()
+    o.depth, o.id = out["depth"].ctypes.data, out["id"].ctypes.data
+    o.r, o.g, o.b = out["r"].ctypes.data, out["g"].ctypes.data, out["b"].ctypes.data
+    if with_shadows_out:
+        out["shadow"] = [np.empty((c.dim, c.dim), np.float32) for c in scene.cascades]
+        for i, s in enumerate(out["shadow"]):
+            o.shadow[i] = s.ctypes.data
+    rc = lib().orc_render(ctypes.addressof(groups), len(scene.groups), ctypes.byref(fr),
+                          ctypes.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"orc_render failed: {rc}")
+    out["stats"] = list(o.stats)
+    del keep, n
+    return out
+
+
+def trisetup(xyz, vn, cam, W, H, cull=False):
+    xyz = _c(xyz, np.float32).reshape(-1, 3)
+    vn = _c(vn, np.float32).reshape(-1, 3)
+    T = xyz.shape[0] // 3
+    P = np.empty((T, 3, 2), np.float32)
+    Z = np.empty((T, 3), np.float32)
+    cls = np.empty(T, np.int8)
+    c = make_cam(cam)
+    lib().orc_trisetup(xyz.ctypes.data, vn.ctypes.data, T, ctypes.byref(c), W, H,
+                       1 if cull else 0, P.ctypes.data, Z.ctypes.data, cls.ctypes.data)
+    return P, Z, cls
+
+
+def coarse(P6, WC, HC, cap=1 << 16):
+    P6 = _c(P6, np.float32).reshape(6)
+    out = np.empty(cap, np.int32)
+    n = lib().orc_coarse_tri(P6.ctypes.data, WC, HC, out.ctypes.data, cap)
+    if n < 0:
+        raise RuntimeError("coarse capacity")
+    return out[:n]
+
+
+def fragments(mode, P6, Z3, W, H, tile=0, cap=1 << 20):
+    """Fragments of one triangle: mode 0 small path, 1 tiled path (one
+    tile), 2 shadow path.  Returns (x, y, tz, half) arrays in order."""
+    P6 = _c(P6, np.float32).reshape(6)
+    Z3 = _c(Z3, np.float32).reshape(3)
+    xs = np.empty(cap, np.int32)
+    ys = np.empty(cap, np.int32)
+    tz = np.empty(cap, np.float32)
+    hf = np.empty(cap, np.int8)
+    n = lib().orc_fragments(mode, P6.ctypes.data, Z3.ctypes.data, W, H, tile, xs.ctypes.data,
+                            ys.ctypes.data, tz.ctypes.data, hf.ctypes.data, cap)
+    if n < 0:
+        raise RuntimeError("fragment capacity")
+    return xs[:n], ys[:n], tz[:n], hf[:n]
+
+
+def gamma(r, g, b, ex):
+    r, g, b = (_c(x, np.uint16).copy() for x in (r, g, b))
+    lib().orc_gamma(r.ctypes.data, g.ctypes.data, b.ctypes.data, r.size, float(ex))
+    return r, g, b
+
+
+def create_mips(plane):
+    plane = _c(plane, np.uint16)
+    N = plane.shape[0]
+    out = np.empty(((4 * N * N - 1) // 3,), np.uint16)
+    n = lib().orc_create_mips(plane.ctypes.data, N, out.ctypes.data)
+    if n < 0:
+        raise RuntimeError("create_mips: N must be a power of two")
+    return out[:n]
+
+
+def shadow_setup(xyz, cam, dim, bias):
+    xyz = _c(xyz, np.float32).reshape(-1, 3)
+    T = xyz.shape[0] // 3
+    XY = np.empty((T, 3, 2), np.int32)
+    Z = np.empty((T, 3), np.float32)
+    ok = np.empty(T, np.int8)
+    c = make_cam(cam)
+    lib().orc_shadow_setup(xyz.ctypes.data, T, ctypes.byref(c), dim, float(bias),
+                           XY.ctypes.data, Z.ctypes.data, ok.ctypes.data)
+    return XY, Z, ok
