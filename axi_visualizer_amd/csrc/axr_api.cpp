@@ -3,7 +3,8 @@
 // One context = one render target, its scene groups and shadow maps, and one
 // HIP stream.  Per-frame work is enqueued without host synchronisation:
 // triangle counts stay on the device (grid-stride kernels read them), group
-// flags travel as kernel arguments, and colour is materialised lazily -- the
+// flags live in a device table rewritten only when they change, and colour
+// is materialised lazily -- the
 // depth pass leaves per-pixel winner keys, and the first call that needs
 // colour (gamma or a readback) runs one resolve+shade launch, with gamma
 // fused when gamma is what asked for it.
@@ -49,6 +50,7 @@ struct HostGroup {
   int64_t ntexels = 0;
   float mip_bias = 0.0f;
   int spec_pow = 16;
+  uint32_t flags = 0;  // as last uploaded to the device table
   ushort4* d_tex = nullptr;
 };
 
@@ -76,7 +78,7 @@ struct TimingRec {
 };
 
 constexpr uint32_t kDefaultBinCap = 8u << 20;
-constexpr int kCounters = 8;
+constexpr int kCounters = AXR_CTR_WORDS;
 
 }  // namespace
 
@@ -97,9 +99,7 @@ struct axr_ctx {
   uint16_t *d_r = nullptr, *d_g = nullptr, *d_b = nullptr;
   uint32_t* d_rgba = nullptr;
   uint32_t* d_ids = nullptr;
-  AxrTriRec* d_small = nullptr;
   AxrTriRec* d_large = nullptr;
-  AxrShRec* d_shrec = nullptr;
   size_t rec_cap = 0;
   uint32_t* d_counters = nullptr;
   uint32_t* d_tile_cnt = nullptr;
@@ -213,12 +213,8 @@ int reserve_tris(axr_ctx* c, size_t extra) {
   c->d_vpos = np;
   c->d_vattr = na;
   c->tri_cap = cap;
-  dfree(c->d_small);
   dfree(c->d_large);
-  dfree(c->d_shrec);
-  if ((rc = dalloc(&c->d_small, cap))) return rc;
   if ((rc = dalloc(&c->d_large, cap))) return rc;
-  if ((rc = dalloc(&c->d_shrec, cap))) return rc;
   c->rec_cap = cap;
   return 0;
 }
@@ -229,12 +225,14 @@ int upload_groups(axr_ctx* c) {
     const HostGroup& g = c->groups[i];
     h[i].base = g.base;
     h[i].ntri = g.ntri;
-    h[i].flags = 0;
+    h[i].flags = g.flags;
     h[i].tex_size = g.tex_size;
     h[i].mip_bias = g.mip_bias;
     h[i].spec_pow = g.spec_pow;
     h[i].tex = g.d_tex;
   }
+  // Kernels in flight read the table: let them finish before rewriting it.
+  HIPCHK(hipStreamSynchronize(c->stream));
   if (!h.empty())
     HIPCHK(hipMemcpy(c->d_groups, h.data(), h.size() * sizeof(AxrGroupDev),
                      hipMemcpyHostToDevice));
@@ -332,9 +330,7 @@ int axr_destroy(axr_ctx* c) {
   dfree(c->d_b);
   dfree(c->d_rgba);
   dfree(c->d_ids);
-  dfree(c->d_small);
   dfree(c->d_large);
-  dfree(c->d_shrec);
   dfree(c->d_counters);
   dfree(c->d_tile_cnt);
   dfree(c->d_tile_off);
@@ -499,7 +495,8 @@ int axr_clear_zbuffer(axr_ctx* c) {
   int rc;
   if ((rc = check_ctx(c))) return rc;
   Timed tm(c, "clear");
-  HIPCHK(launch_fill_u64(c->d_keys, (size_t)c->W * c->H, key_clear(), c->stream));
+  HIPCHK(launch_clear_frame(c->d_keys, (size_t)c->W * c->H, c->d_counters, c->d_tile_cnt,
+                            c->NT + 1, c->stream));
   c->state = ST_CLEARED;
   return AXR_OK;
 }
@@ -532,13 +529,19 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
   dc.T = c->T;
   dc.ngroups = (uint32_t)ngroups;
   dc.pcf_taps = pcf_taps;
-  bool any_shadow = false;
+  bool any_shadow = false, changed = false;
+  std::vector<uint32_t> nf((size_t)ngroups);
   for (int g = 0; g < ngroups; g++) {
     const uint32_t f = shader_flags[g] & (AXR_SH_MIP | AXR_SH_SHADOW | AXR_SH_PHONG | AXR_SH_CULL);
     if ((f & AXR_SH_MIP) && !c->groups[g].is_mip)
       return fail(AXR_EINVAL, "group %d: mip shader on a texture without mips", g);
     if ((f & AXR_SH_SHADOW) && !(mask && mask[g])) any_shadow = true;
-    dc.gflags[g] = f | ((mask && mask[g]) ? 0u : AXR_GRP_DRAW);
+    nf[g] = f | ((mask && mask[g]) ? 0u : AXR_GRP_DRAW);
+    changed |= nf[g] != c->groups[g].flags;
+  }
+  if (changed) {
+    for (int g = 0; g < ngroups; g++) c->groups[g].flags = nf[g];
+    if ((rc = upload_groups(c))) return rc;
   }
   dc.nshadow = 0;
   if (any_shadow) {
@@ -554,17 +557,10 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
     }
   }
   if (c->T > 0) {
-    HIPCHK(launch_fill_u32(c->d_counters, kCounters, 0, c->stream));
-    HIPCHK(launch_fill_u32(c->d_tile_cnt, c->NT + 1, 0, c->stream));
     {
-      Timed tm(c, "setup");
-      HIPCHK(launch_setup(c->d_vpos, c->d_vattr, c->d_groups, dc, c->d_small, c->d_large,
-                          c->d_counters, c->stream));
-    }
-    {
-      Timed tm(c, "raster_small");
-      HIPCHK(launch_raster_small(c->d_small, c->d_counters, c->d_keys, c->W, c->H, c->T,
-                                 c->stream));
+      Timed tm(c, "setup_raster");
+      HIPCHK(launch_setup_raster(c->d_vpos, c->d_vattr, c->d_groups, dc, c->d_large,
+                                 c->d_counters, c->d_keys, c->stream));
     }
     {
       Timed tm(c, "bin_count");
@@ -629,8 +625,8 @@ int axr_clear_shadow_map(axr_ctx* c, int i) {
   if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists)
     return fail(AXR_EINVAL, "no shadow map %d", i);
   Timed tm(c, "shadow_clear");
-  HIPCHK(launch_fill_u32(c->sm[i].d_map, (size_t)c->sm[i].dim * c->sm[i].dim,
-                         ordf(AXR_DEPTH_CLEAR), c->stream));
+  HIPCHK(launch_clear_shadow(c->sm[i].d_map, (size_t)c->sm[i].dim * c->sm[i].dim, c->d_counters,
+                             c->stream));
   return AXR_OK;
 }
 
@@ -645,18 +641,9 @@ int axr_shadow_map(axr_ctx* c, int i, const uint8_t* cast, int ngroups, float bi
   for (int g = 0; g < ngroups; g++)
     if (cast[g]) m.bits[g >> 5] |= 1u << (g & 31);
   if (c->T == 0) return AXR_OK;
-  HIPCHK(launch_fill_u32(c->d_counters + 2, 1, 0, c->stream));
-  {
-    Timed tm(c, "shadow_setup");
-    HIPCHK(launch_shadow_setup(c->d_vpos, c->d_groups, (uint32_t)c->groups.size(), c->T, m,
-                               c->sm[i].cam, c->sm[i].dim, bias, c->d_shrec, c->d_counters + 2,
-                               c->stream));
-  }
-  {
-    Timed tm(c, "shadow_raster");
-    HIPCHK(launch_shadow_raster(c->d_shrec, c->d_counters + 2, c->sm[i].d_map, c->sm[i].dim, c->T,
-                                c->stream));
-  }
+  Timed tm(c, "shadow");
+  HIPCHK(launch_shadow(c->d_vpos, c->d_groups, (uint32_t)c->groups.size(), c->T, m, c->sm[i].cam,
+                       c->sm[i].dim, bias, c->sm[i].d_map, c->d_counters, c->stream));
   return AXR_OK;
 }
 
@@ -769,10 +756,20 @@ int axr_get_stats(axr_ctx* c, uint32_t out[5]) {
   int rc;
   if ((rc = check_ctx(c))) return rc;
   if (!out) return fail(AXR_EINVAL, "null output");
-  uint32_t h[kCounters];
-  HIPCHK(hipMemcpyAsync(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  std::vector<uint32_t> h(kCounters);
+  HIPCHK(hipMemcpyAsync(h.data(), c->d_counters, kCounters * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (int k = 0; k < 5; k++) out[k] = h[k];
+  uint32_t small = 0, cast = 0;
+  for (int k = 0; k < 64; k++) {
+    small += h[AXR_CTR_SMALL + k * AXR_CTR_STRIDE];
+    cast += h[AXR_CTR_CAST + k * AXR_CTR_STRIDE];
+  }
+  out[0] = small;
+  out[1] = h[AXR_CTR_LARGE];
+  out[2] = cast;
+  out[3] = h[AXR_CTR_BIN_TOTAL];
+  out[4] = h[AXR_CTR_BIN_OVERFLOW];
   return AXR_OK;
 }
 

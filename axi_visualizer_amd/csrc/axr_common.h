@@ -20,6 +20,8 @@
 
 #define AXR_MAX_GROUPS 128
 
+#define AXR_HD __host__ __device__ __forceinline__
+
 #define AXR_MAX_SHADOWS 4
 #define AXR_SHADOW_BIND 2  // cascades sampled by one drawAll
 #define AXR_NONE 0xFFFFFFFFu
@@ -32,11 +34,11 @@ struct AxrCam {
   float scale, cax, cay;
 };
 
-// Static per-group data (device table, uploaded when groups change).
+// Per-group device table; re-uploaded only when a group or its flags change.
 struct AxrGroupDev {
   uint32_t base;   // first global triangle index
   uint32_t ntri;
-  uint32_t flags;  // filled per draw from AxrDrawConst::gflags
+  uint32_t flags;  // AXR_SH_* | AXR_GRP_DRAW of the current drawAll
   int tex_size;    // N, or log2(N) for a mip chain (texSize, Ops.py:371-374)
   float mip_bias;
   int spec_pow;
@@ -63,13 +65,28 @@ struct AxrDrawConst {
   int pcf_taps;
   AxrShadowDev sh[AXR_SHADOW_BIND];
   uint16_t clear_rgb[4];
-  uint32_t gflags[AXR_MAX_GROUPS];  // AXR_SH_* | AXR_GRP_DRAW per group
 };
 
 // Groups casting into one shadow pass (Ops.py:1094-1095 whichCast).
 struct AxrCastMask {
   uint32_t bits[AXR_MAX_GROUPS / 32];
 };
+// Bit g of the mask without dynamic indexing of a by-value argument (which
+// would spill the argument to scratch).
+AXR_HD bool cast_bit(const AxrCastMask& m, int g) {
+  const uint32_t w = g < 32 ? m.bits[0] : g < 64 ? m.bits[1] : g < 96 ? m.bits[2] : m.bits[3];
+  return (w >> (g & 31)) & 1u;
+}
+
+// Device counter block.  Statistics are accumulated per wave into 64 slots
+// one cache line apart, so no address sees more than 1/64 of the atomics.
+#define AXR_CTR_LARGE 0        // large-triangle records (append cursor)
+#define AXR_CTR_BIN_TOTAL 1    // bin entries of the last drawAll
+#define AXR_CTR_BIN_OVERFLOW 2 // entries needed when above capacity, else 0
+#define AXR_CTR_STRIDE 32      // u32 per slot (128 B)
+#define AXR_CTR_SMALL 32       // 64 slots: small triangles drawn
+#define AXR_CTR_CAST (AXR_CTR_SMALL + 64 * AXR_CTR_STRIDE)  // 64 slots: casters
+#define AXR_CTR_WORDS (AXR_CTR_CAST + 64 * AXR_CTR_STRIDE)
 
 // Compact record of one visible triangle from setup.
 struct AxrTriRec {
@@ -84,7 +101,6 @@ struct AxrShRec {
 };
 
 // -------------------------------------------------------------- helpers
-#define AXR_HD __host__ __device__ __forceinline__
 
 AXR_HD float omin(float x, float y) { return y < x ? y : x; }
 AXR_HD float omax(float x, float y) { return x < y ? y : x; }

@@ -7,11 +7,12 @@ namespace axr {
 hipError_t launch_fill_u64(uint64_t* p, size_t n, uint64_t v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
-hipError_t launch_setup(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
-                        const AxrDrawConst& dc, AxrTriRec* small, AxrTriRec* large,
-                        uint32_t* counters, hipStream_t s);
-hipError_t launch_raster_small(const AxrTriRec* rec, const uint32_t* counters, uint64_t* keys,
-                               int W, int H, uint32_t max_n, hipStream_t s);
+hipError_t launch_clear_frame(uint64_t* keys, size_t n, uint32_t* counters, uint32_t* tile_cnt,
+                              int ntile, hipStream_t s);
+hipError_t launch_clear_shadow(uint32_t* map, size_t n, uint32_t* counters, hipStream_t s);
+hipError_t launch_setup_raster(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
+                               const AxrDrawConst& dc, AxrTriRec* large, uint32_t* counters,
+                               uint64_t* keys, hipStream_t s);
 hipError_t launch_bin_count(const AxrTriRec* large, const uint32_t* counters, uint32_t* tile_cnt,
                             int WC, int HC, uint32_t max_n, hipStream_t s);
 hipError_t launch_bin_scan(uint32_t* tile_cnt, uint32_t* tile_off, int NT, uint32_t* counters,
@@ -22,11 +23,9 @@ hipError_t launch_bin_fill(const AxrTriRec* large, const uint32_t* counters, uin
 hipError_t launch_raster_tiles(const AxrTriRec* large, const uint32_t* tile_off,
                                const uint32_t* bins, uint64_t* keys, int W, int WC, int HC,
                                uint32_t T, uint32_t cap, hipStream_t s);
-hipError_t launch_shadow_setup(const float4* vpos, const AxrGroupDev* groups, uint32_t ngroups,
-                               uint32_t T, const AxrCastMask& cast, const AxrCam& cam, int dim,
-                               float bias, AxrShRec* rec, uint32_t* counter, hipStream_t s);
-hipError_t launch_shadow_raster(const AxrShRec* rec, const uint32_t* counter, uint32_t* map,
-                                int dim, uint32_t max_n, hipStream_t s);
+hipError_t launch_shadow(const float4* vpos, const AxrGroupDev* groups, uint32_t ngroups,
+                         uint32_t T, const AxrCastMask& cast, const AxrCam& cam, int dim,
+                         float bias, uint32_t* map, uint32_t* counters, hipStream_t s);
 
 // Shade modes: HDR u16 planes, or gamma fused into RGBA8.
 enum ShadeMode { SHADE_HDR = 0, SHADE_LDR8 = 1 };

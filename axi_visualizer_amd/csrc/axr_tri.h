@@ -614,7 +614,9 @@ __device__ __forceinline__ float pcf_light(const AxrDrawConst& dc, V3 wpos) {
   const int K = dc.pcf_taps == 9 ? 9 : 5;
   const int d = 2;  // shDist
   int shadow = 0;
-  for (int c = 0; c < dc.nshadow; c++) {
+#pragma unroll
+  for (int c = 0; c < AXR_SHADOW_BIND; c++) {
+    if (c >= dc.nshadow) break;
     const AxrShadowDev& sm = dc.sh[c];
     const V3 pos = sub(wpos, v3(sm.cam.pos[0], sm.cam.pos[1], sm.cam.pos[2]));
     const float depth = dot(pos, cam_row(sm.cam, 0));

@@ -1,12 +1,18 @@
-// raster.hip -- depth pass of the gfx950 raster path: triangle setup with
-// wave-aggregated compaction, small-triangle scanline walk, coarse tile
-// binning (count / scan / fill), per-tile walk with LDS-resident 64-bit keys,
-// and the shadow-map depth pass.
+// raster.hip -- depth pass of the gfx950 raster path.
+//
+//   k_clear_frame    depth keys to the clear key, per-frame counters to 0
+//   k_setup_raster   triangle setup of every drawn triangle (a5); a small
+//                    triangle is walked by the same lane at once (a7), a
+//                    large one is appended to the large list
+//   k_bin_count/scan/fill   coarse 16x16 tile binning of large triangles (a8)
+//   k_raster_tiles   one block per tile, keys resident in LDS (a9)
+//   k_clear_shadow / k_shadow   ortho setup + drawmin walk per caster (a11)
 //
 // Every fragment issues min(key) with key = (orderable tz << 32) | ~seq
-// (axr_common.h), so the result does not depend on launch order or on
-// which lane visits a fragment first: bins need no sorting and the
-// reference's per-group launch sequence collapses into single launches.
+// (axr_common.h), so the result does not depend on launch order or on which
+// lane visits a fragment first: bins need no sorting, small triangles need
+// no compaction, and the reference's per-group launch sequence collapses
+// into single launches.
 #include "axr_launch.h"
 #include "axr_tri.h"
 
@@ -34,7 +40,8 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-// Appends `item` for every lane with `take`, one atomic per wave.
+// Appends `item` for every lane with `take`, one atomic per wave that has
+// any taker.
 template <class T>
 __device__ __forceinline__ void wave_append(bool take, const T& item, T* __restrict__ out,
                                             uint32_t* __restrict__ counter, uint32_t cap) {
@@ -50,6 +57,13 @@ __device__ __forceinline__ void wave_append(bool take, const T& item, T* __restr
   }
 }
 
+// Adds the wave's count of `flag` lanes to one of 64 spread stat slots.
+__device__ __forceinline__ void wave_count(bool flag, uint32_t* __restrict__ slots) {
+  const uint64_t m = __ballot(flag);
+  if (m != 0 && __lane_id() == 0)
+    atomicAdd(&slots[(blockIdx.x & 63) * AXR_CTR_STRIDE], (uint32_t)__popcll(m));
+}
+
 __global__ void k_fill_u64(uint64_t* __restrict__ p, size_t n, uint64_t v) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -62,44 +76,55 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
-// a5: one thread per triangle of every drawn group.
-__global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpos,
-                                                  const float4* __restrict__ vattr,
-                                                  const AxrGroupDev* __restrict__ groups,
-                                                  AxrDrawConst dc, AxrTriRec* __restrict__ small,
-                                                  AxrTriRec* __restrict__ large,
-                                                  uint32_t* __restrict__ counters) {
+// clearZBuffer: keys to the clear key; draw counters and tile counts to 0.
+__global__ void k_clear_frame(uint64_t* __restrict__ keys, size_t n, uint64_t v,
+                              uint32_t* __restrict__ counters, uint32_t* __restrict__ tile_cnt,
+                              int ntile) {
+  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = t0; i < n; i += st) keys[i] = v;
+  for (size_t i = t0; i < (size_t)AXR_CTR_CAST; i += st) counters[i] = 0;
+  for (size_t i = t0; i < (size_t)ntile; i += st) tile_cnt[i] = 0;
+}
+
+// clearShadowMap: map to the clear depth key; caster counts to 0.
+__global__ void k_clear_shadow(uint32_t* __restrict__ map, size_t n, uint32_t v,
+                               uint32_t* __restrict__ counters) {
+  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = t0; i < n; i += st) map[i] = v;
+  for (size_t i = AXR_CTR_CAST + t0; i < (size_t)AXR_CTR_WORDS; i += st) counters[i] = 0;
+}
+
+// a5 + a7: one lane per triangle of every drawn group.
+__global__ __launch_bounds__(kBlock) void k_setup_raster(const float4* __restrict__ vpos,
+                                                         const float4* __restrict__ vattr,
+                                                         const AxrGroupDev* __restrict__ groups,
+                                                         AxrDrawConst dc,
+                                                         AxrTriRec* __restrict__ large,
+                                                         uint32_t* __restrict__ counters,
+                                                         unsigned long long* __restrict__ keys) {
   const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
   int cls = 0;
   AxrTriRec r;
   r.gi = gi;
   if (gi < dc.T) {
-    const uint32_t f = dc.gflags[find_group(groups, (int)dc.ngroups, gi)];
+    const uint32_t f = groups[find_group(groups, (int)dc.ngroups, gi)].flags;
     if (f & AXR_GRP_DRAW)
       cls = tri_setup(vpos, vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
                       r.z);
   }
-  wave_append(cls == 1, r, small, &counters[0], dc.T);
-  wave_append(cls == 2, r, large, &counters[1], dc.T);
-}
-
-// a7: small triangles, one lane walks one triangle; keys in global memory.
-__global__ __launch_bounds__(kBlock) void k_raster_small(const AxrTriRec* __restrict__ rec,
-                                                         const uint32_t* __restrict__ counters,
-                                                         unsigned long long* __restrict__ keys,
-                                                         int W, int H) {
-  const uint32_t n = counters[0];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const AxrTriRec r = rec[i];
-    const Geom<int> g = small_geom(r.px, r.py, r.z);
-    const uint32_t order = r.gi;
-    small_walk(g, W, H, [&](int ax, int cy, float tz, int half) {
-      if (!(tz <= AXR_DEPTH_CLEAR)) return;
-      const unsigned long long k = make_key(tz, frag_seq(order, half));
-      unsigned long long* p = &keys[(size_t)cy * W + ax];
-      if (k < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, k);
-    });
-  }
+  wave_count(cls == 1, counters + AXR_CTR_SMALL);
+  wave_append(cls == 2, r, large, &counters[AXR_CTR_LARGE], dc.T);
+  if (cls != 1) return;
+  const Geom<int> g = small_geom(r.px, r.py, r.z);
+  const int W = dc.W;
+  small_walk(g, W, dc.H, [&](int ax, int cy, float tz, int half) {
+    if (!(tz <= AXR_DEPTH_CLEAR)) return;
+    const unsigned long long k = make_key(tz, frag_seq(gi, half));
+    unsigned long long* p = &keys[(size_t)cy * W + ax];
+    if (k < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, k);
+  });
 }
 
 // a8: coarse binning, pass 1 -- entries per tile.
@@ -107,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(const AxrTriRec* __restric
                                                       const uint32_t* __restrict__ counters,
                                                       uint32_t* __restrict__ tile_cnt, int WC,
                                                       int HC) {
-  const uint32_t n = counters[1];
+  const uint32_t n = counters[AXR_CTR_LARGE];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const AxrTriRec r = rec[i];
     coarse_walk(r.px, r.py, WC, HC, [&](int t) { atomicAdd(&tile_cnt[t], 1u); });
@@ -143,8 +168,8 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* __restrict__ tile_c
   }
   if (tid == 1023) {
     tile_off[NT] = part[1023];
-    counters[3] = part[1023];
-    if (part[1023] > cap) counters[4] = part[1023];
+    counters[AXR_CTR_BIN_TOTAL] = part[1023];
+    counters[AXR_CTR_BIN_OVERFLOW] = part[1023] > cap ? part[1023] : 0u;
   }
 }
 
@@ -155,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_fill(const AxrTriRec* __restrict
                                                      const uint32_t* __restrict__ tile_off,
                                                      uint32_t* __restrict__ bins, int WC, int HC,
                                                      uint32_t cap) {
-  const uint32_t n = counters[1];
+  const uint32_t n = counters[AXR_CTR_LARGE];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const AxrTriRec r = rec[i];
     coarse_walk(r.px, r.py, WC, HC, [&](int t) {
@@ -195,38 +220,29 @@ __global__ __launch_bounds__(kBlock) void k_raster_tiles(const AxrTriRec* __rest
   *gk = lk[tid];
 }
 
-// a11: ortho setup of every casting group.
-__global__ __launch_bounds__(kBlock) void k_shadow_setup(const float4* __restrict__ vpos,
-                                                         const AxrGroupDev* __restrict__ groups,
-                                                         uint32_t ngroups, uint32_t T,
-                                                         AxrCastMask cast, AxrCam cam, int dim,
-                                                         float bias,
-                                                         AxrShRec* __restrict__ out,
-                                                         uint32_t* __restrict__ counter) {
+// a11: ortho setup + drawmin walk per caster; the map holds orderable keys
+// of the minimum depth.
+__global__ __launch_bounds__(kBlock) void k_shadow(const float4* __restrict__ vpos,
+                                                   const AxrGroupDev* __restrict__ groups,
+                                                   uint32_t ngroups, uint32_t T, AxrCastMask cast,
+                                                   AxrCam cam, int dim, float bias,
+                                                   uint32_t* __restrict__ map,
+                                                   uint32_t* __restrict__ counters) {
   const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
   bool ok = false;
   AxrShRec r;
   if (gi < T) {
     const int g = find_group(groups, (int)ngroups, gi);
-    if ((cast.bits[g >> 5] >> (g & 31)) & 1u) ok = shadow_setup(vpos, gi, cam, dim, bias, r);
+    if (cast_bit(cast, g)) ok = shadow_setup(vpos, gi, cam, dim, bias, r);
   }
-  wave_append(ok, r, out, counter, T);
-}
-
-// a11: drawmin depth walk; the map holds orderable keys of min depth.
-__global__ __launch_bounds__(kBlock) void k_shadow_raster(const AxrShRec* __restrict__ rec,
-                                                          const uint32_t* __restrict__ counter,
-                                                          uint32_t* __restrict__ map, int dim) {
-  const uint32_t n = *counter;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const AxrShRec r = rec[i];
-    shadow_walk(r, dim, dim, [&](int ax, int cy, float tz) {
-      if (tz != tz) return;
-      const uint32_t k = ordf(tz == 0.0f ? 0.0f : tz);
-      uint32_t* p = &map[(size_t)cy * dim + ax];
-      if (k < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, k);
-    });
-  }
+  wave_count(ok, counters + AXR_CTR_CAST);
+  if (!ok) return;
+  shadow_walk(r, dim, dim, [&](int ax, int cy, float tz) {
+    if (tz != tz) return;
+    const uint32_t k = ordf(tz == 0.0f ? 0.0f : tz);
+    uint32_t* p = &map[(size_t)cy * dim + ax];
+    if (k < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, k);
+  });
 }
 
 inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
@@ -250,20 +266,25 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_setup(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
-                        const AxrDrawConst& dc, AxrTriRec* small, AxrTriRec* large,
-                        uint32_t* counters, hipStream_t s) {
-  if (dc.T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup, dim3((dc.T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos, vattr,
-                     groups, dc, small, large, counters);
+hipError_t launch_clear_frame(uint64_t* keys, size_t n, uint32_t* counters, uint32_t* tile_cnt,
+                              int ntile, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_frame, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s, keys, n,
+                     (uint64_t)key_clear(), counters, tile_cnt, ntile);
   return hipGetLastError();
 }
 
-hipError_t launch_raster_small(const AxrTriRec* rec, const uint32_t* counters, uint64_t* keys,
-                               int W, int H, uint32_t max_n, hipStream_t s) {
-  if (max_n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_raster_small, dim3(grid_for(max_n, kBlock, 65535)), dim3(kBlock), 0, s,
-                     rec, counters, (unsigned long long*)keys, W, H);
+hipError_t launch_clear_shadow(uint32_t* map, size_t n, uint32_t* counters, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_shadow, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s, map, n,
+                     ordf(AXR_DEPTH_CLEAR), counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_setup_raster(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
+                               const AxrDrawConst& dc, AxrTriRec* large, uint32_t* counters,
+                               uint64_t* keys, hipStream_t s) {
+  if (dc.T == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_setup_raster, dim3((dc.T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
+                     vattr, groups, dc, large, counters, (unsigned long long*)keys);
   return hipGetLastError();
 }
 
@@ -298,20 +319,12 @@ hipError_t launch_raster_tiles(const AxrTriRec* large, const uint32_t* tile_off,
   return hipGetLastError();
 }
 
-hipError_t launch_shadow_setup(const float4* vpos, const AxrGroupDev* groups, uint32_t ngroups,
-                               uint32_t T, const AxrCastMask& cast, const AxrCam& cam, int dim,
-                               float bias, AxrShRec* rec, uint32_t* counter, hipStream_t s) {
+hipError_t launch_shadow(const float4* vpos, const AxrGroupDev* groups, uint32_t ngroups,
+                         uint32_t T, const AxrCastMask& cast, const AxrCam& cam, int dim,
+                         float bias, uint32_t* map, uint32_t* counters, hipStream_t s) {
   if (T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_shadow_setup, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
-                     groups, ngroups, T, cast, cam, dim, bias, rec, counter);
-  return hipGetLastError();
-}
-
-hipError_t launch_shadow_raster(const AxrShRec* rec, const uint32_t* counter, uint32_t* map,
-                                int dim, uint32_t max_n, hipStream_t s) {
-  if (max_n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_shadow_raster, dim3(grid_for(max_n, kBlock, 65535)), dim3(kBlock), 0, s,
-                     rec, counter, map, dim);
+  hipLaunchKernelGGL(k_shadow, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos, groups,
+                     ngroups, T, cast, cam, dim, bias, map, counters);
   return hipGetLastError();
 }
 

@@ -28,8 +28,18 @@ __device__ __forceinline__ int find_group(const AxrGroupDev* __restrict__ groups
   return lo;
 }
 
+// Vertex k of three, by selects (a dynamic index would place the array in
+// scratch memory).
+__device__ __forceinline__ Attr pick3(const Attr& a0, const Attr& a1, const Attr& a2, int k) {
+  Attr r;
+#define M(f) r.f = k == 0 ? a0.f : (k == 1 ? a1.f : a2.f);
+  AXR_ATTR_FIELDS(M)
+#undef M
+  return r;
+}
+
 // Rebuilds the winning fragment at (x, y) and shades it.
-__device__ __noinline__ void resolve_pixel(uint64_t key, int x, int y,
+__device__ __forceinline__ void resolve_pixel(uint64_t key, int x, int y,
                                            const float4* __restrict__ vpos,
                                            const float4* __restrict__ vattr,
                                            const AxrGroupDev* __restrict__ groups,
@@ -39,22 +49,21 @@ __device__ __noinline__ void resolve_pixel(uint64_t key, int x, int y,
   const int half = (int)((seq - 1u) & 1u);
   const bool large = order >= dc.T;
   const uint32_t gi = large ? order - dc.T : order;
-  const int gidx = find_group(groups, (int)dc.ngroups, gi);
-  AxrGroupDev grp = groups[gidx];
-  grp.flags = dc.gflags[gidx];
+  const AxrGroupDev grp = groups[find_group(groups, (int)dc.ngroups, gi)];
 
   float px[3], py[3], z[3];
   tri_setup(vpos, vattr, gi, dc.view, dc.W, dc.H, (grp.flags & AXR_SH_CULL) != 0, px, py, z);
-  Attr A[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) A[k] = load_attr(vpos, vattr, 3 * gi + k, z[k]);
+  const Attr A0 = load_attr(vpos, vattr, 3 * gi, z[0]);
+  const Attr A1 = load_attr(vpos, vattr, 3 * gi + 1, z[1]);
+  const Attr A2 = load_attr(vpos, vattr, 3 * gi + 2, z[2]);
 
   float cx1, cx2, cz1, cz2;
   Attr c1, c2;
   MipInfo mi = {0, 0.0f, 0, 0, 0, 0};
   if (!large) {
     const Geom<int> g = small_geom(px, py, z);
-    Attr a1 = A[g.k1], a2 = A[g.k2], a3 = A[g.k3];
+    Attr a1 = pick3(A0, A1, A2, g.k1), a2 = pick3(A0, A1, A2, g.k2),
+         a3 = pick3(A0, A1, A2, g.k3);
     const Attr a4 = a_lerp4(a1, a3, g.yd);
     if (grp.flags & AXR_SH_MIP) mi = mip_setup(a1, a2, a3, g, grp.tex_size, grp.mip_bias);
     const Edges e = small_edges(g, half);
@@ -83,7 +92,8 @@ __device__ __noinline__ void resolve_pixel(uint64_t key, int x, int y,
     }
   } else {
     const Geom<float> g = tile_geom(px, py, z);
-    Attr a1 = A[g.k1], a2 = A[g.k2], a3 = A[g.k3];
+    Attr a1 = pick3(A0, A1, A2, g.k1), a2 = pick3(A0, A1, A2, g.k2),
+         a3 = pick3(A0, A1, A2, g.k3);
     const Attr a4 = a_lerp4(a1, a3, g.yd);
     if (grp.flags & AXR_SH_MIP) mi = mip_setup(a1, a2, a3, g, grp.tex_size, grp.mip_bias);
     const TileBox box = tile_box((y >> 4) * dc.WC + (x >> 4), dc.W);
@@ -123,6 +133,30 @@ __device__ __noinline__ void resolve_pixel(uint64_t key, int x, int y,
   shade(dc, grp, mi, t, tz, c1, c2, rgb);
 }
 
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const unsigned lane = __lane_id();
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+template <int MODE>
+__device__ __forceinline__ void put_px(size_t p, float d, uint16_t rgb[3], float exposure,
+                                       float* __restrict__ depth, uint16_t* __restrict__ ro,
+                                       uint16_t* __restrict__ go, uint16_t* __restrict__ bo,
+                                       uint32_t* __restrict__ rgba) {
+  depth[p] = d;
+  if (MODE == SHADE_LDR8) {
+    gamma_px(exposure, rgb);
+    rgba[p] = (uint32_t)rgb[0] | ((uint32_t)rgb[1] << 8) | ((uint32_t)rgb[2] << 16) | 0xFF000000u;
+  } else {
+    ro[p] = rgb[0];
+    go[p] = rgb[1];
+    bo[p] = rgb[2];
+  }
+}
+
+// One block per 16x16 pixel tile.  Background pixels are written at once;
+// covered pixels are compacted in LDS so the shading work fills the first
+// waves of the block densely instead of idling lanes across all four.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_shade(const unsigned long long* __restrict__ keys,
                                                const float4* __restrict__ vpos,
@@ -133,26 +167,40 @@ __global__ __launch_bounds__(256) void k_shade(const unsigned long long* __restr
                                                uint16_t* __restrict__ ro, uint16_t* __restrict__ go,
                                                uint16_t* __restrict__ bo,
                                                uint32_t* __restrict__ rgba) {
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (x >= dc.W || y >= dc.H) return;
+  __shared__ uint8_t list[256];
+  __shared__ uint32_t wcount[4];
+  const int tid = threadIdx.x;
+  const int x = blockIdx.x * 16 + (tid & 15);
+  const int y = blockIdx.y * 16 + (tid >> 4);
+  const bool inb = x < dc.W && y < dc.H;
   const size_t p = (size_t)y * dc.W + x;
-  const uint64_t key = keys[p];
-  uint16_t rgb[3] = {dc.clear_rgb[0], dc.clear_rgb[1], dc.clear_rgb[2]};
-  float d = AXR_DEPTH_CLEAR;
-  if ((uint32_t)key != 0xFFFFFFFFu) {
-    d = unordf((uint32_t)(key >> 32));
-    resolve_pixel(key, x, y, vpos, vattr, groups, dc, rgb);
+  uint64_t key = inb ? keys[p] : 0xFFFFFFFFull;
+  const bool covered = inb && (uint32_t)key != 0xFFFFFFFFu;
+  if (inb && !covered) {
+    uint16_t rgb[3] = {dc.clear_rgb[0], dc.clear_rgb[1], dc.clear_rgb[2]};
+    put_px<MODE>(p, AXR_DEPTH_CLEAR, rgb, exposure, depth, ro, go, bo, rgba);
   }
-  depth[p] = d;
-  if (MODE == SHADE_LDR8) {
-    gamma_px(exposure, rgb);
-    rgba[p] = (uint32_t)rgb[0] | ((uint32_t)rgb[1] << 8) | ((uint32_t)rgb[2] << 16) | 0xFF000000u;
-  } else {
-    ro[p] = rgb[0];
-    go[p] = rgb[1];
-    bo[p] = rgb[2];
+  const uint64_t m = __ballot(covered);
+  const int wave = tid >> 6;
+  if (__lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t off = 0, n = 0;
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    off += w < wave ? wcount[w] : 0u;
+    n += wcount[w];
   }
+  if (covered) list[off + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)tid;
+  __syncthreads();
+  if ((uint32_t)tid >= n) return;
+  const int lid = list[tid];
+  const int sx = blockIdx.x * 16 + (lid & 15);
+  const int sy = blockIdx.y * 16 + (lid >> 4);
+  const size_t sp = (size_t)sy * dc.W + sx;
+  key = keys[sp];
+  uint16_t rgb[3];
+  resolve_pixel(key, sx, sy, vpos, vattr, groups, dc, rgb);
+  put_px<MODE>(sp, unordf((uint32_t)(key >> 32)), rgb, exposure, depth, ro, go, bo, rgba);
 }
 
 __global__ void k_gamma_planes(uint16_t* __restrict__ r, uint16_t* __restrict__ g,

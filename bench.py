@@ -42,19 +42,17 @@ def algorithmic_bytes(sc, st):
     for c in sc.cascades:
         frame += T * 36 + 2 * c.dim * c.dim * 4
     visible = st["small"] + st["large"]
+    dmax = max((c.dim for c in sc.cascades), default=0)
     per_kernel = {
-        # keys in (8 B/px), RGBA8 + depth out (8 B/px), winning triangles'
-        # attributes (132 B each, bounded by the visible set)
+        # keys in (8 B/px), RGBA8 + depth out (8 B/px), the winning
+        # triangles' vertex data (132 B each, bounded by the visible set)
         "shade_gamma": W * H * 16 + visible * 132,
-        # positions of every triangle in, one 40 B record per visible one out
-        "setup": T * 36 + visible * 40,
-        # records in, keys read-modify-write once per covered pixel
-        "raster_small": st["small"] * 40 + W * H * 16,
-        "shadow_setup": T * 36 + st["casters"] * 36,
-        "shadow_raster": st["casters"] * 36 + 2 * max(c.dim for c in sc.cascades) ** 2 * 4
-        if sc.cascades else 0,
+        # positions of every triangle in (36 B), the key buffer out once
+        "setup_raster": T * 36 + W * H * 8,
+        # positions of every triangle in, the shadow map out once
+        "shadow": T * 36 + dmax * dmax * 4,
         "clear": W * H * 8,
-        "shadow_clear": max((c.dim * c.dim * 4 for c in sc.cascades), default=0),
+        "shadow_clear": dmax * dmax * 4,
     }
     return frame, per_kernel
 
