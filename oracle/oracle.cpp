@@ -828,7 +828,7 @@ void bucket_rows(std::vector<std::vector<uint32_t>>& bands, uint32_t item, int y
   for (int b = b0; b <= b1; b++) bands[b].push_back(item);
 }
 
-void render_shadow(Ctx& cx, int c, int nthreads) {
+int64_t render_shadow(Ctx& cx, int c, int nthreads) {
   const OrcShadowCfg& cfg = cx.fr->shadow[c];
   ShadowMap& sm = cx.sm[c];
   sm.dim = cfg.dim;
@@ -862,14 +862,19 @@ void render_shadow(Ctx& cx, int c, int nthreads) {
   }
   float* F = sm.F.data();
   const int dim = cfg.dim;
+  std::atomic<int64_t> nfrag{0};
   parallel_for(nb, nthreads, [&](int b) {
     int rlo = b * BAND, rhi = std::min(dim, rlo + BAND);
+    int64_t nf = 0;
     for (uint32_t gi : bands[b])
       raster_shadow(tris[gi], dim, dim, rlo, rhi, [&](int ax, int cy, float tz, int) {
+        nf++;
         float& f = F[(int64_t)dim * cy + ax];
         if (f > tz) f = tz;
       });
+    nfrag += nf;
   });
+  return nfrag.load();
 }
 
 }  // namespace
@@ -1059,12 +1064,12 @@ int orc_render(const OrcGroup* groups, int ngroups, const OrcFrame* fr, OrcOut* 
   cx.base.assign(ngroups + 1, 0);
   for (int g = 0; g < ngroups; g++) cx.base[g + 1] = cx.base[g] + groups[g].ntri;
   const int64_t T = cx.base[ngroups];
-  for (int i = 0; i < 8; i++) out->stats[i] = 0;
+  for (int i = 0; i < 8; i++) out->stats[i] = 0;  // 5, 6: shadow fragments per cascade
 
   // Shadow cascades (Ops.py:1089-1141 per map), before the main pass.
   cx.nsm = std::min(std::max(fr->nshadow, 0), 2);
   for (int c = 0; c < cx.nsm; c++) {
-    render_shadow(cx, c, nthreads);
+    out->stats[5 + c] = render_shadow(cx, c, nthreads);
     if (out->shadow[c])
       std::memcpy(out->shadow[c], cx.sm[c].F.data(), cx.sm[c].F.size() * sizeof(float));
   }

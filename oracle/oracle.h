@@ -98,7 +98,8 @@ typedef struct {
   uint16_t* g;
   uint16_t* b;
   float* shadow[2]; /* optional [dim*dim] copies of the shadow maps */
-  int64_t stats[8]; /* 0 small, 1 large, 2 fragments, 3 max bin, 4 bin entries */
+  int64_t stats[8]; /* 0 small, 1 large, 2 fragments, 3 max bin, 4 bin entries,
+                       5/6 shadow fragments of cascade 0/1 */
 } OrcOut;
 
 int orc_render(const OrcGroup* groups, int ngroups, const OrcFrame* fr,
