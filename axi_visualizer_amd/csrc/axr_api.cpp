@@ -1,13 +1,16 @@
 // axr_api.cpp -- C ABI of libaxr.so (declared in include/axr.h).
 //
 // One context = one render target, its scene groups and shadow maps, and one
-// HIP stream.  Per-frame work is enqueued without host synchronisation:
-// triangle counts stay on the device (grid-stride kernels read them), group
-// flags live in a device table rewritten only when they change, and colour
-// is materialised lazily -- the
-// depth pass leaves per-pixel winner keys, and the first call that needs
-// colour (gamma or a readback) runs one resolve+shade launch, with gamma
-// fused when gamma is what asked for it.
+// HIP stream.  Per-frame work is enqueued without host synchronisation and
+// without allocation: triangle counts and bin offsets stay on the device,
+// group flags live in a device table rewritten only when they change.
+//
+// drawAll sets up and bins the triangles; the depth walk and shading run in
+// one launch when colour is first needed (gamma or a readback), with gamma
+// fused when gamma is what asked for it.  Shadow-map clears are applied
+// lazily: the next shadow pass starts its bins from the clear depth instead
+// of reading a cleared map.  Anything that would change the inputs of a
+// draw that has not been shaded yet shades it first.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -57,16 +60,18 @@ struct HostGroup {
 struct HostShadow {
   bool exists = false;
   bool placed = false;
+  bool clear_pending = false;  // cleared since its last pass; map not yet written
   int dim = 0;
+  int nbx = 0, nb = 0;  // 32x32 bins
   float scale = 0.0f;
   AxrCam cam{};
   uint32_t* d_map = nullptr;
 };
 
 enum State {
-  ST_UNCLEARED,  // keys hold nothing usable yet
-  ST_CLEARED,    // keys clear, nothing drawn
-  ST_PENDING,    // keys hold winners; colour not materialised
+  ST_UNCLEARED,  // nothing drawn since the context was made
+  ST_CLEARED,    // cleared, nothing drawn
+  ST_PENDING,    // triangles binned; colour not materialised
   ST_HDR,        // u16 planes hold HDR colour
   ST_LDR8,       // rgba8 holds gamma'd colour
   ST_LDR16       // u16 planes hold gamma'd colour
@@ -77,14 +82,15 @@ struct TimingRec {
   hipEvent_t a, b;
 };
 
-constexpr uint32_t kDefaultBinCap = 8u << 20;
+constexpr uint32_t kDefaultBinCap = 16u << 20;
 constexpr int kCounters = AXR_CTR_WORDS;
 
 }  // namespace
 
 struct axr_ctx {
   int device = 0;
-  int W = 0, H = 0, WC = 0, HC = 0, NT = 0;
+  int W = 0, H = 0, WC = 0, HC = 0;
+  int NBX = 0, NB = 0;  // 32x32 screen bins
   hipStream_t stream = nullptr;
 
   std::vector<HostGroup> groups;
@@ -93,19 +99,24 @@ struct axr_ctx {
   float4* d_vpos = nullptr;   // [3*tri_cap] (x,y,z,0)
   float4* d_vattr = nullptr;  // [6*tri_cap] (nx,ny,nz,u),(v,Ix,Iy,Iz)
   AxrGroupDev* d_groups = nullptr;
+  AxrTriRec* d_rec = nullptr;   // [tri_cap] setup records of the last drawAll
+  uint8_t* d_cls = nullptr;     // [tri_cap] 0 culled, 1 small, 2 large
+  AxrShRec* d_shrec = nullptr;  // [tri_cap] ortho setup of the last shadow pass
+  uint8_t* d_ok = nullptr;      // [tri_cap] caster kept
 
-  uint64_t* d_keys = nullptr;
+  // main bins: counts/offsets/cursors over 2*NB (small, then large lists)
+  uint32_t *d_bcnt = nullptr, *d_boff = nullptr, *d_bcur = nullptr, *d_bent = nullptr;
+  uint32_t bin_cap = 0;
+  // shadow bins, sized for the largest map
+  uint32_t *d_scnt = nullptr, *d_soff = nullptr, *d_scur = nullptr, *d_sent = nullptr;
+  int snb_cap = 0;
+  uint32_t sbin_cap = 0;
+
   float* d_depth = nullptr;
   uint16_t *d_r = nullptr, *d_g = nullptr, *d_b = nullptr;
   uint32_t* d_rgba = nullptr;
   uint32_t* d_ids = nullptr;
-  AxrTriRec* d_large = nullptr;
-  size_t rec_cap = 0;
   uint32_t* d_counters = nullptr;
-  uint32_t* d_tile_cnt = nullptr;
-  uint32_t* d_tile_off = nullptr;
-  uint32_t* d_bins = nullptr;
-  uint32_t bin_cap = 0;
   float* d_light = nullptr;  // vertLight light arrays
   size_t light_cap = 0;
 
@@ -118,6 +129,7 @@ struct axr_ctx {
 
   int state = ST_UNCLEARED;
   AxrDrawConst last{};
+  bool records_valid = false;  // setup records and bins belong to `last`
 
   bool timing = false;
   std::vector<TimingRec> trecs;
@@ -127,7 +139,7 @@ struct axr_ctx {
 
 namespace {
 
-// Scoped event pair around a launch sequence when timing is enabled.
+// Scoped event pair around a launch when timing is enabled.
 struct Timed {
   axr_ctx* c;
   const char* name;
@@ -191,7 +203,7 @@ int check_ctx(axr_ctx* c) {
   return 0;
 }
 
-// Grows the concatenated vertex and record arrays to hold ntri more.
+// Grows the concatenated vertex arrays and the per-triangle records.
 int reserve_tris(axr_ctx* c, size_t extra) {
   const size_t need = (size_t)c->T + extra;
   if (need <= c->tri_cap) return 0;
@@ -213,9 +225,18 @@ int reserve_tris(axr_ctx* c, size_t extra) {
   c->d_vpos = np;
   c->d_vattr = na;
   c->tri_cap = cap;
-  dfree(c->d_large);
-  if ((rc = dalloc(&c->d_large, cap))) return rc;
-  c->rec_cap = cap;
+  dfree(c->d_rec);
+  dfree(c->d_cls);
+  dfree(c->d_shrec);
+  dfree(c->d_ok);
+  if ((rc = dalloc(&c->d_rec, cap)) || (rc = dalloc(&c->d_cls, cap)) ||
+      (rc = dalloc(&c->d_shrec, cap)) || (rc = dalloc(&c->d_ok, cap)))
+    return rc;
+  // cap is a power of two >= 1024: the byte arrays are whole words
+  HIPCHK(launch_fill_u32((uint32_t*)c->d_cls, cap / 4, 0, c->stream));
+  HIPCHK(launch_fill_u32((uint32_t*)c->d_ok, cap / 4, 0, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->records_valid = false;
   return 0;
 }
 
@@ -239,10 +260,11 @@ int upload_groups(axr_ctx* c) {
   return 0;
 }
 
-// Resolve + shade of the pending draw into the requested target.
-int materialize(axr_ctx* c, int mode, float exposure) {
+// Per-bin depth walk + shading of the current draw into the requested target.
+int bin_raster(axr_ctx* c, int mode, float exposure) {
   AxrDrawConst dc = c->last;
-  if (c->state == ST_CLEARED) {
+  const int empty = c->state == ST_CLEARED || c->state == ST_UNCLEARED;
+  if (empty) {
     dc = AxrDrawConst{};
     dc.W = c->W;
     dc.H = c->H;
@@ -252,10 +274,65 @@ int materialize(axr_ctx* c, int mode, float exposure) {
     dc.ngroups = (uint32_t)c->groups.size();
     for (int k = 0; k < 3; k++) dc.clear_rgb[k] = c->clear_rgb[k];
   }
-  Timed tm(c, mode == SHADE_LDR8 ? "shade_gamma" : "shade");
-  HIPCHK(launch_shade(mode, c->d_keys, c->d_vpos, c->d_vattr, c->d_groups, dc, exposure,
-                      c->d_depth, c->d_r, c->d_g, c->d_b, c->d_rgba, c->stream));
+  Timed tm(c, mode == SHADE_LDR8 ? "raster_shade" : mode == SHADE_HDR ? "raster_hdr" : "raster_ids");
+  HIPCHK(launch_bin_raster(mode, c->d_rec, c->d_boff, c->d_bent, c->bin_cap, empty, c->d_vpos,
+                           c->d_vattr, c->d_groups, dc, c->NBX, c->NB, exposure, c->d_depth,
+                           c->d_r, c->d_g, c->d_b, c->d_rgba, c->d_ids, c->stream));
+  return 0;
+}
+
+int materialize(axr_ctx* c, int mode, float exposure) {
+  int rc = bin_raster(c, mode, exposure);
+  if (rc) return rc;
   c->state = mode == SHADE_LDR8 ? ST_LDR8 : ST_HDR;
+  return 0;
+}
+
+// Shades a pending draw before one of its inputs changes (the reference
+// shades inside drawAll, so later calls cannot affect that frame).
+int settle(axr_ctx* c) {
+  if (c->state != ST_PENDING) return 0;
+  return materialize(c, SHADE_HDR, 1.0f);
+}
+
+bool bound_to_pending(axr_ctx* c, int i) {
+  if (c->state != ST_PENDING) return false;
+  for (int k = 0; k < c->last.nshadow; k++)
+    if (c->last.sh[k].map == c->sm[i].d_map) return true;
+  return false;
+}
+
+// Writes the clear depth into a map whose clear is still pending.
+int apply_shadow_clear(axr_ctx* c, int i) {
+  HostShadow& s = c->sm[i];
+  if (!s.clear_pending) return 0;
+  Timed tm(c, "shadow_clear");
+  HIPCHK(launch_fill_u32(s.d_map, (size_t)s.dim * s.dim, ordf(AXR_DEPTH_CLEAR), c->stream));
+  s.clear_pending = false;
+  return 0;
+}
+
+// After a synchronising readback: a bin capacity overflow makes the frame
+// incomplete, so report it and grow the pool for the frames that follow.
+int check_overflow(axr_ctx* c) {
+  uint32_t h[kCounters];
+  HIPCHK(hipMemcpy(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost));
+  if (h[AXR_CTR_BIN_OVERFLOW] > c->bin_cap) {
+    const uint32_t need = h[AXR_CTR_BIN_OVERFLOW];
+    dfree(c->d_bent);
+    c->bin_cap = need + need / 2;
+    int rc = dalloc(&c->d_bent, c->bin_cap);
+    if (rc) return rc;
+    return fail(AXR_ECAP, "screen bins needed %u entries; pool grown, redraw the frame", need);
+  }
+  if (h[AXR_CTR_SH_OVERFLOW] > c->sbin_cap) {
+    const uint32_t need = h[AXR_CTR_SH_OVERFLOW];
+    dfree(c->d_sent);
+    c->sbin_cap = need + need / 2;
+    int rc = dalloc(&c->d_sent, c->sbin_cap);
+    if (rc) return rc;
+    return fail(AXR_ECAP, "shadow bins needed %u entries; pool grown, redraw the map", need);
+  }
   return 0;
 }
 
@@ -278,7 +355,8 @@ int axr_create(int w, int h, axr_ctx** out) {
   c->H = h;
   c->WC = w / 16;
   c->HC = h / 16;
-  c->NT = c->WC * c->HC;
+  c->NBX = (w + AXR_BIN - 1) / AXR_BIN;
+  c->NB = c->NBX * ((h + AXR_BIN - 1) / AXR_BIN);
   const size_t np = (size_t)w * h;
   int rc = 0;
   auto bail = [&](int r) {
@@ -287,16 +365,18 @@ int axr_create(int w, int h, axr_ctx** out) {
   };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(AXR_EHIP, "hipStreamCreate failed"));
-  if ((rc = dalloc(&c->d_keys, np)) || (rc = dalloc(&c->d_depth, np)) ||
-      (rc = dalloc(&c->d_r, np)) || (rc = dalloc(&c->d_g, np)) || (rc = dalloc(&c->d_b, np)) ||
-      (rc = dalloc(&c->d_rgba, np)) || (rc = dalloc(&c->d_ids, np)) ||
-      (rc = dalloc(&c->d_counters, kCounters)) || (rc = dalloc(&c->d_tile_cnt, c->NT + 1)) ||
-      (rc = dalloc(&c->d_tile_off, c->NT + 1)) || (rc = dalloc(&c->d_bins, kDefaultBinCap)) ||
+  const size_t nb2 = 2 * (size_t)c->NB;
+  if ((rc = dalloc(&c->d_depth, np)) || (rc = dalloc(&c->d_r, np)) ||
+      (rc = dalloc(&c->d_g, np)) || (rc = dalloc(&c->d_b, np)) || (rc = dalloc(&c->d_rgba, np)) ||
+      (rc = dalloc(&c->d_ids, np)) || (rc = dalloc(&c->d_counters, kCounters)) ||
+      (rc = dalloc(&c->d_bcnt, nb2)) || (rc = dalloc(&c->d_boff, nb2 + 1)) ||
+      (rc = dalloc(&c->d_bcur, nb2)) || (rc = dalloc(&c->d_bent, kDefaultBinCap)) ||
       (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)))
     return bail(rc);
   c->bin_cap = kDefaultBinCap;
   if (launch_fill_u32(c->d_counters, kCounters, 0, c->stream) != hipSuccess ||
-      launch_fill_u32(c->d_tile_cnt, c->NT + 1, 0, c->stream) != hipSuccess ||
+      launch_fill_u32(c->d_bcnt, nb2, 0, c->stream) != hipSuccess ||
+      launch_fill_u32(c->d_boff, nb2 + 1, 0, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(fail(AXR_EHIP, "initial fill failed"));
   // Default view: identity basis at the origin, 90 degree fovx.
@@ -323,18 +403,25 @@ int axr_destroy(axr_ctx* c) {
   dfree(c->d_vpos);
   dfree(c->d_vattr);
   dfree(c->d_groups);
-  dfree(c->d_keys);
+  dfree(c->d_rec);
+  dfree(c->d_cls);
+  dfree(c->d_shrec);
+  dfree(c->d_ok);
+  dfree(c->d_bcnt);
+  dfree(c->d_boff);
+  dfree(c->d_bcur);
+  dfree(c->d_bent);
+  dfree(c->d_scnt);
+  dfree(c->d_soff);
+  dfree(c->d_scur);
+  dfree(c->d_sent);
   dfree(c->d_depth);
   dfree(c->d_r);
   dfree(c->d_g);
   dfree(c->d_b);
   dfree(c->d_rgba);
   dfree(c->d_ids);
-  dfree(c->d_large);
   dfree(c->d_counters);
-  dfree(c->d_tile_cnt);
-  dfree(c->d_tile_off);
-  dfree(c->d_bins);
   dfree(c->d_light);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -373,6 +460,7 @@ int axr_add_texture_group(axr_ctx* c, const float* xyz, const float* uv, const f
         (int64_t)tex_size * tex_size != ntexels)
       return fail(AXR_EINVAL, "texture must be N*N with N a power of two");
   }
+  if ((rc = settle(c))) return rc;
   if ((rc = reserve_tris(c, (size_t)ntri))) return rc;
   const size_t nv = (size_t)3 * ntri;
   std::vector<float4> hp(nv), ha(2 * nv);
@@ -409,6 +497,7 @@ int axr_set_group_material(axr_ctx* c, int tn, float mip_bias, int spec_pow) {
   if ((rc = check_ctx(c))) return rc;
   if (tn < 0 || tn >= (int)c->groups.size()) return fail(AXR_EINVAL, "no group %d", tn);
   if (spec_pow < 0) return fail(AXR_EINVAL, "spec_pow must be >= 0");
+  if ((rc = settle(c))) return rc;
   c->groups[tn].mip_bias = mip_bias;
   c->groups[tn].spec_pow = spec_pow;
   return upload_groups(c);
@@ -435,6 +524,7 @@ int axr_vert_light(axr_ctx* c, const uint8_t* mask, int ngroups, const float* di
   if ((ndir && (!dirI || !dirD)) || (npoint && (!pointI || !pointP)) ||
       (nspot && (!spotI || !spotD || !spotP)))
     return fail(AXR_EINVAL, "vertLight: missing light arrays");
+  if ((rc = settle(c))) return rc;
   std::vector<float> h;
   auto put = [&](const float* p, int n) {
     size_t off = h.size();
@@ -445,10 +535,12 @@ int axr_vert_light(axr_ctx* c, const uint8_t* mask, int ngroups, const float* di
   const size_t oPI = put(pointI, npoint), oPP = put(pointP, npoint);
   const size_t oSI = put(spotI, nspot), oSD = put(spotD, nspot), oSP = put(spotP, nspot);
   if (h.size() > c->light_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
     dfree(c->d_light);
     if ((rc = dalloc(&c->d_light, h.size()))) return rc;
     c->light_cap = h.size();
   }
+  HIPCHK(hipStreamSynchronize(c->stream));
   if (!h.empty())
     HIPCHK(hipMemcpy(c->d_light, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
   AxrLights L;
@@ -494,9 +586,8 @@ int axr_set_clear_color(axr_ctx* c, const uint16_t rgb[3]) {
 int axr_clear_zbuffer(axr_ctx* c) {
   int rc;
   if ((rc = check_ctx(c))) return rc;
-  Timed tm(c, "clear");
-  HIPCHK(launch_clear_frame(c->d_keys, (size_t)c->W * c->H, c->d_counters, c->d_tile_cnt,
-                            c->NT + 1, c->stream));
+  // The bins of the next drawAll start empty (the scan zeroes the counts),
+  // and the per-bin walk starts from the clear key: nothing to launch.
   c->state = ST_CLEARED;
   return AXR_OK;
 }
@@ -539,16 +630,13 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
     nf[g] = f | ((mask && mask[g]) ? 0u : AXR_GRP_DRAW);
     changed |= nf[g] != c->groups[g].flags;
   }
-  if (changed) {
-    for (int g = 0; g < ngroups; g++) c->groups[g].flags = nf[g];
-    if ((rc = upload_groups(c))) return rc;
-  }
   dc.nshadow = 0;
   if (any_shadow) {
     for (int k = 0; k < nshadow; k++) {
       const int i = shadow_ids ? shadow_ids[k] : k;
       if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists || !c->sm[i].placed)
         return fail(AXR_ESTATE, "drawAll: shadow map %d missing or not placed", i);
+      if ((rc = apply_shadow_clear(c, i))) return rc;
       AxrShadowDev& sd = dc.sh[dc.nshadow++];
       sd.map = c->sm[i].d_map;
       sd.dim = c->sm[i].dim;
@@ -556,35 +644,31 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
       sd.cam = c->sm[i].cam;
     }
   }
+  if (changed) {
+    for (int g = 0; g < ngroups; g++) c->groups[g].flags = nf[g];
+    if ((rc = upload_groups(c))) return rc;
+  }
   if (c->T > 0) {
     {
-      Timed tm(c, "setup_raster");
-      HIPCHK(launch_setup_raster(c->d_vpos, c->d_vattr, c->d_groups, dc, c->d_large,
-                                 c->d_counters, c->d_keys, c->stream));
-    }
-    {
-      Timed tm(c, "bin_count");
-      HIPCHK(launch_bin_count(c->d_large, c->d_counters, c->d_tile_cnt, c->WC, c->HC, c->T,
-                              c->stream));
+      Timed tm(c, "setup_bin");
+      HIPCHK(launch_setup_bin(c->d_vpos, c->d_vattr, c->d_groups, dc, c->d_rec, c->d_cls,
+                              c->d_bcnt, c->NBX, c->NB, c->stream));
     }
     {
       Timed tm(c, "bin_scan");
-      HIPCHK(launch_bin_scan(c->d_tile_cnt, c->d_tile_off, c->NT, c->d_counters, c->bin_cap,
-                             c->stream));
+      HIPCHK(launch_scan(c->d_bcnt, c->d_boff, c->d_bcur, 2 * c->NB, c->bin_cap,
+                         c->d_counters + AXR_CTR_BIN_TOTAL, c->d_counters + AXR_CTR_BIN_OVERFLOW,
+                         c->stream));
     }
     {
       Timed tm(c, "bin_fill");
-      HIPCHK(launch_bin_fill(c->d_large, c->d_counters, c->d_tile_cnt, c->d_tile_off, c->d_bins,
-                             c->WC, c->HC, c->bin_cap, c->T, c->stream));
-    }
-    {
-      Timed tm(c, "raster_tiles");
-      HIPCHK(launch_raster_tiles(c->d_large, c->d_tile_off, c->d_bins, c->d_keys, c->W, c->WC,
-                                 c->HC, c->T, c->bin_cap, c->stream));
+      HIPCHK(launch_bin_fill(c->d_rec, c->d_cls, c->T, c->W, c->H, c->WC, c->HC, c->NBX, c->NB,
+                             c->d_bcur, c->d_bent, c->bin_cap, c->stream));
     }
   }
   c->last = dc;
-  c->state = ST_PENDING;
+  c->records_valid = true;
+  c->state = c->T > 0 ? ST_PENDING : ST_CLEARED;
   return AXR_OK;
 }
 
@@ -593,16 +677,35 @@ int axr_add_shadow_map(axr_ctx* c, int i, int size, float scale, float amb) {
   if ((rc = check_ctx(c))) return rc;
   if (i < 0 || i >= AXR_MAX_SHADOWS) return fail(AXR_EINVAL, "shadow map index %d", i);
   if (size < 8 || size > 16384 || (size & 1)) return fail(AXR_EINVAL, "shadow map size %d", size);
+  if (bound_to_pending(c, i) && (rc = settle(c))) return rc;
   HostShadow& s = c->sm[i];
+  HIPCHK(hipStreamSynchronize(c->stream));
   dfree(s.d_map);
   if ((rc = dalloc(&s.d_map, (size_t)size * size))) return rc;
   s.exists = true;
   s.placed = false;
+  s.clear_pending = true;  // addShadowMap fills 256*256-1 (Ops.py:1049)
   s.dim = size;
+  s.nbx = (size + AXR_BIN - 1) / AXR_BIN;
+  s.nb = s.nbx * s.nbx;
   s.scale = scale;
   s.cam.scale = scale;
   if (amb >= 0) c->amb = amb;
-  HIPCHK(launch_fill_u32(s.d_map, (size_t)size * size, ordf(AXR_DEPTH_CLEAR), c->stream));
+  if (s.nb > c->snb_cap) {
+    dfree(c->d_scnt);
+    dfree(c->d_soff);
+    dfree(c->d_scur);
+    if ((rc = dalloc(&c->d_scnt, s.nb)) || (rc = dalloc(&c->d_soff, s.nb + 1)) ||
+        (rc = dalloc(&c->d_scur, s.nb)))
+      return rc;
+    HIPCHK(launch_fill_u32(c->d_scnt, s.nb, 0, c->stream));
+    c->snb_cap = s.nb;
+  }
+  if (!c->d_sent) {
+    if ((rc = dalloc(&c->d_sent, kDefaultBinCap))) return rc;
+    c->sbin_cap = kDefaultBinCap;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
   return AXR_OK;
 }
 
@@ -624,9 +727,8 @@ int axr_clear_shadow_map(axr_ctx* c, int i) {
   if ((rc = check_ctx(c))) return rc;
   if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists)
     return fail(AXR_EINVAL, "no shadow map %d", i);
-  Timed tm(c, "shadow_clear");
-  HIPCHK(launch_clear_shadow(c->sm[i].d_map, (size_t)c->sm[i].dim * c->sm[i].dim, c->d_counters,
-                             c->stream));
+  if (bound_to_pending(c, i) && (rc = settle(c))) return rc;
+  c->sm[i].clear_pending = true;
   return AXR_OK;
 }
 
@@ -637,13 +739,35 @@ int axr_shadow_map(axr_ctx* c, int i, const uint8_t* cast, int ngroups, float bi
     return fail(AXR_EINVAL, "shadow map %d missing or not placed", i);
   if (ngroups != (int)c->groups.size() || !cast)
     return fail(AXR_EINVAL, "shadowMap: whichCast must cover all groups");
+  if (bound_to_pending(c, i) && (rc = settle(c))) return rc;
   AxrCastMask m{};
   for (int g = 0; g < ngroups; g++)
     if (cast[g]) m.bits[g >> 5] |= 1u << (g & 31);
   if (c->T == 0) return AXR_OK;
-  Timed tm(c, "shadow");
-  HIPCHK(launch_shadow(c->d_vpos, c->d_groups, (uint32_t)c->groups.size(), c->T, m, c->sm[i].cam,
-                       c->sm[i].dim, bias, c->sm[i].d_map, c->d_counters, c->stream));
+  HostShadow& s = c->sm[i];
+  {
+    Timed tm(c, "shadow_setup_bin");
+    HIPCHK(launch_shadow_setup_bin(c->d_vpos, c->d_groups, (uint32_t)c->groups.size(), c->T, m,
+                                   s.cam, s.dim, bias, c->d_shrec, c->d_ok, c->d_scnt, s.nbx,
+                                   c->stream));
+  }
+  {
+    Timed tm(c, "shadow_scan");
+    HIPCHK(launch_scan(c->d_scnt, c->d_soff, c->d_scur, s.nb, c->sbin_cap,
+                       c->d_counters + AXR_CTR_SH_TOTAL, c->d_counters + AXR_CTR_SH_OVERFLOW,
+                       c->stream));
+  }
+  {
+    Timed tm(c, "shadow_fill");
+    HIPCHK(launch_shadow_fill(c->d_shrec, c->d_ok, c->T, s.dim, s.nbx, c->d_scur, c->d_sent,
+                              c->sbin_cap, c->stream));
+  }
+  {
+    Timed tm(c, "shadow_raster");
+    HIPCHK(launch_shadow_bins(c->d_shrec, c->d_soff, c->d_sent, c->sbin_cap, s.d_map, s.dim,
+                              s.nbx, s.nb, s.clear_pending ? 0 : 1, c->stream));
+  }
+  s.clear_pending = false;
   return AXR_OK;
 }
 
@@ -690,7 +814,7 @@ int axr_get_frame(axr_ctx* c, uint16_t* r, uint16_t* g, uint16_t* b, float* dept
   if (b) HIPCHK(hipMemcpyAsync(b, c->d_b, n * 2, hipMemcpyDeviceToHost, c->stream));
   if (depth) HIPCHK(hipMemcpyAsync(depth, c->d_depth, n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  return AXR_OK;
+  return check_overflow(c);
 }
 
 int axr_get_rgba8(axr_ctx* c, uint32_t* rgba) {
@@ -701,7 +825,7 @@ int axr_get_rgba8(axr_ctx* c, uint32_t* rgba) {
   if (c->state == ST_LDR8) {
     HIPCHK(hipMemcpyAsync(rgba, c->d_rgba, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    return AXR_OK;
+    return check_overflow(c);
   }
   if (c->state == ST_LDR16) {
     std::vector<uint16_t> r(n), g(n), b(n);
@@ -712,7 +836,7 @@ int axr_get_rgba8(axr_ctx* c, uint32_t* rgba) {
     for (size_t i = 0; i < n; i++)
       rgba[i] = (uint32_t)(r[i] & 0xFF) | ((uint32_t)(g[i] & 0xFF) << 8) |
                 ((uint32_t)(b[i] & 0xFF) << 16) | 0xFF000000u;
-    return AXR_OK;
+    return check_overflow(c);
   }
   return fail(AXR_ESTATE, "getRGBA8 needs gamma() first");
 }
@@ -722,11 +846,14 @@ int axr_get_ids(axr_ctx* c, uint32_t* ids) {
   if ((rc = check_ctx(c))) return rc;
   if (!ids) return fail(AXR_EINVAL, "null output");
   if (c->state == ST_UNCLEARED) return fail(AXR_ESTATE, "getIds before any clearZBuffer");
+  if (c->state != ST_CLEARED && !c->records_valid)
+    return fail(AXR_ESTATE, "getIds: groups were added since the last drawAll");
   const size_t n = (size_t)c->W * c->H;
-  HIPCHK(launch_ids(c->d_keys, c->T, c->d_ids, n, c->stream));
+  // The winners of the current draw (also after it has been shaded).
+  if ((rc = bin_raster(c, SHADE_IDS, 1.0f))) return rc;
   HIPCHK(hipMemcpyAsync(ids, c->d_ids, n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  return AXR_OK;
+  return check_overflow(c);
 }
 
 int axr_get_shadow_map(axr_ctx* c, int i, float* out) {
@@ -734,6 +861,7 @@ int axr_get_shadow_map(axr_ctx* c, int i, float* out) {
   if ((rc = check_ctx(c))) return rc;
   if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists || !out)
     return fail(AXR_EINVAL, "no shadow map %d", i);
+  if ((rc = apply_shadow_clear(c, i))) return rc;
   const size_t n = (size_t)c->sm[i].dim * c->sm[i].dim;
   float* d = nullptr;
   if ((rc = dalloc(&d, n))) return rc;
@@ -742,7 +870,7 @@ int axr_get_shadow_map(axr_ctx* c, int i, float* out) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   dfree(d);
   if (e != hipSuccess) return fail(AXR_EHIP, "getSHM: %s", hipGetErrorString(e));
-  return AXR_OK;
+  return check_overflow(c);
 }
 
 int axr_sync(axr_ctx* c) {
@@ -756,20 +884,15 @@ int axr_get_stats(axr_ctx* c, uint32_t out[5]) {
   int rc;
   if ((rc = check_ctx(c))) return rc;
   if (!out) return fail(AXR_EINVAL, "null output");
-  std::vector<uint32_t> h(kCounters);
-  HIPCHK(hipMemcpyAsync(h.data(), c->d_counters, kCounters * sizeof(uint32_t),
-                        hipMemcpyDeviceToHost, c->stream));
+  if (c->T) HIPCHK(launch_stats(c->d_cls, c->d_ok, c->T, c->d_counters + AXR_CTR_STATS, c->stream));
+  uint32_t h[kCounters];
+  HIPCHK(hipMemcpyAsync(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  uint32_t small = 0, cast = 0;
-  for (int k = 0; k < 64; k++) {
-    small += h[AXR_CTR_SMALL + k * AXR_CTR_STRIDE];
-    cast += h[AXR_CTR_CAST + k * AXR_CTR_STRIDE];
-  }
-  out[0] = small;
-  out[1] = h[AXR_CTR_LARGE];
-  out[2] = cast;
+  out[0] = c->T ? h[AXR_CTR_STATS] : 0;
+  out[1] = c->T ? h[AXR_CTR_STATS + 1] : 0;
+  out[2] = c->T ? h[AXR_CTR_STATS + 2] : 0;
   out[3] = h[AXR_CTR_BIN_TOTAL];
-  out[4] = h[AXR_CTR_BIN_OVERFLOW];
+  out[4] = h[AXR_CTR_BIN_OVERFLOW] | h[AXR_CTR_SH_OVERFLOW];
   return AXR_OK;
 }
 

@@ -78,23 +78,26 @@ AXR_HD bool cast_bit(const AxrCastMask& m, int g) {
   return (w >> (g & 31)) & 1u;
 }
 
-// Device counter block.  Statistics are accumulated per wave into 64 slots
-// one cache line apart, so no address sees more than 1/64 of the atomics.
-#define AXR_CTR_LARGE 0        // large-triangle records (append cursor)
-#define AXR_CTR_BIN_TOTAL 1    // bin entries of the last drawAll
-#define AXR_CTR_BIN_OVERFLOW 2 // entries needed when above capacity, else 0
-#define AXR_CTR_STRIDE 32      // u32 per slot (128 B)
-#define AXR_CTR_SMALL 32       // 64 slots: small triangles drawn
-#define AXR_CTR_CAST (AXR_CTR_SMALL + 64 * AXR_CTR_STRIDE)  // 64 slots: casters
-#define AXR_CTR_WORDS (AXR_CTR_CAST + 64 * AXR_CTR_STRIDE)
+// Screen and shadow maps are binned into 32 x 32 pixel bins (two by two of
+// the reference's 16-px tiles).  A bin's triangles are walked by one block
+// with the bin's depth keys in LDS.
+#define AXR_BIN 32
+#define AXR_BIN_SHIFT 5
 
-// Compact record of one visible triangle from setup.
+// Device counter block.
+#define AXR_CTR_BIN_TOTAL 0     // bin entries of the last drawAll
+#define AXR_CTR_BIN_OVERFLOW 1  // entries needed when above capacity, else 0
+#define AXR_CTR_SH_TOTAL 2      // the same for the last shadow pass
+#define AXR_CTR_SH_OVERFLOW 3
+#define AXR_CTR_STATS 4         // 4 words filled on request: small, large, casters, -
+#define AXR_CTR_WORDS 8
+
+// Setup output of one visible triangle (indexed by its global index).
 struct AxrTriRec {
   float px[3], py[3], z[3];
-  uint32_t gi;
 };
 
-// Compact record of one shadow caster (int coords).
+// Ortho setup output of one shadow caster (int coords).
 struct AxrShRec {
   int x[3], y[3];
   float z[3];

@@ -4,40 +4,41 @@
 
 namespace axr {
 
-hipError_t launch_fill_u64(uint64_t* p, size_t n, uint64_t v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
-hipError_t launch_clear_frame(uint64_t* keys, size_t n, uint32_t* counters, uint32_t* tile_cnt,
-                              int ntile, hipStream_t s);
-hipError_t launch_clear_shadow(uint32_t* map, size_t n, uint32_t* counters, hipStream_t s);
-hipError_t launch_setup_raster(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
-                               const AxrDrawConst& dc, AxrTriRec* large, uint32_t* counters,
-                               uint64_t* keys, hipStream_t s);
-hipError_t launch_bin_count(const AxrTriRec* large, const uint32_t* counters, uint32_t* tile_cnt,
-                            int WC, int HC, uint32_t max_n, hipStream_t s);
-hipError_t launch_bin_scan(uint32_t* tile_cnt, uint32_t* tile_off, int NT, uint32_t* counters,
+hipError_t launch_setup_bin(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
+                            const AxrDrawConst& dc, AxrTriRec* rec, uint8_t* cls,
+                            uint32_t* cnt, int NBX, int NB, hipStream_t s);
+hipError_t launch_scan(uint32_t* cnt, uint32_t* off, uint32_t* cur, int n, uint32_t cap,
+                       uint32_t* ctr_total, uint32_t* ctr_ovf, hipStream_t s);
+hipError_t launch_bin_fill(const AxrTriRec* rec, const uint8_t* cls, uint32_t T, int W, int H,
+                           int WC, int HC, int NBX, int NB, uint32_t* cur, uint32_t* ent,
                            uint32_t cap, hipStream_t s);
-hipError_t launch_bin_fill(const AxrTriRec* large, const uint32_t* counters, uint32_t* tile_cnt,
-                           const uint32_t* tile_off, uint32_t* bins, int WC, int HC, uint32_t cap,
-                           uint32_t max_n, hipStream_t s);
-hipError_t launch_raster_tiles(const AxrTriRec* large, const uint32_t* tile_off,
-                               const uint32_t* bins, uint64_t* keys, int W, int WC, int HC,
-                               uint32_t T, uint32_t cap, hipStream_t s);
-hipError_t launch_shadow(const float4* vpos, const AxrGroupDev* groups, uint32_t ngroups,
-                         uint32_t T, const AxrCastMask& cast, const AxrCam& cam, int dim,
-                         float bias, uint32_t* map, uint32_t* counters, hipStream_t s);
-
-// Shade modes: HDR u16 planes, or gamma fused into RGBA8.
-enum ShadeMode { SHADE_HDR = 0, SHADE_LDR8 = 1 };
-hipError_t launch_shade(int mode, const uint64_t* keys, const float4* vpos, const float4* vattr,
-                        const AxrGroupDev* groups, const AxrDrawConst& dc, float exposure,
-                        float* depth, uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
+hipError_t launch_shadow_setup_bin(const float4* vpos, const AxrGroupDev* groups,
+                                   uint32_t ngroups, uint32_t T, const AxrCastMask& cast,
+                                   const AxrCam& cam, int dim, float bias, AxrShRec* rec,
+                                   uint8_t* ok, uint32_t* cnt, int NBX, hipStream_t s);
+hipError_t launch_shadow_fill(const AxrShRec* rec, const uint8_t* ok, uint32_t T, int dim,
+                              int NBX, uint32_t* cur, uint32_t* ent, uint32_t cap,
+                              hipStream_t s);
+hipError_t launch_shadow_bins(const AxrShRec* rec, const uint32_t* off, const uint32_t* ent,
+                              uint32_t cap, uint32_t* map, int dim, int NBX, int NB, int from_map,
+                              hipStream_t s);
+hipError_t launch_stats(const uint8_t* cls, const uint8_t* ok, uint32_t T, uint32_t* out,
                         hipStream_t s);
+
+// Bin raster modes: HDR u16 planes, gamma fused into RGBA8, or winner ids.
+enum ShadeMode { SHADE_HDR = 0, SHADE_LDR8 = 1, SHADE_IDS = 2 };
+hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const uint32_t* off,
+                             const uint32_t* ent, uint32_t cap, int empty, const float4* vpos,
+                             const float4* vattr, const AxrGroupDev* groups,
+                             const AxrDrawConst& dc, int NBX, int NB, float exposure,
+                             float* depth, uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
+                             uint32_t* ids, hipStream_t s);
 hipError_t launch_gamma_planes(uint16_t* r, uint16_t* g, uint16_t* b, size_t n, float ex,
                                hipStream_t s);
 hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, uint16_t* b,
                                size_t n, hipStream_t s);
-hipError_t launch_ids(const uint64_t* keys, uint32_t T, uint32_t* ids, size_t n, hipStream_t s);
 hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipStream_t s);
 
 struct AxrLights {

@@ -282,16 +282,20 @@ __device__ __forceinline__ float span_tz(float t, float cz1, float cz2) {
 }
 
 // ------------------------------------------------- a7: small-path walk
-// Templates.py:250-321.  frag(ax, cy, tz, half) for on-screen pixels.
+// Templates.py:250-321.  frag(ax, cy, tz, half) for the pixels inside the
+// clip rectangle [xlo, xhi) x [ylo, yhi), which the caller keeps inside the
+// screen (the reference's own bounds test).  Every row still accumulates,
+// so clipping never changes a fragment's value.
 template <class F>
-__device__ __forceinline__ void small_walk(const Geom<int>& g, int W, int H, F&& frag) {
+__device__ __forceinline__ void small_walk(const Geom<int>& g, int xlo, int ylo, int xhi, int yhi,
+                                           F&& frag) {
   {
     Edges e = small_edges(g, 0);
     float cx1 = (float)g.x1, cx2 = cx1, cz1 = g.z1, cz2 = g.z1;
     for (int cy = g.y1; cy <= g.y2; cy++) {
-      if (cy >= H) break;
-      if (cy >= 0) {
-        for (int ax = imax(f2i(cx2), 0); ax < W && (float)ax <= cx1; ax++) {
+      if (cy >= yhi) break;
+      if (cy >= ylo) {
+        for (int ax = imax(f2i(cx2), xlo); ax < xhi && (float)ax <= cx1; ax++) {
           float t = span_t(ax, cx1, cx2);
           frag(ax, cy, span_tz(t, cz1, cz2), 0);
         }
@@ -306,9 +310,9 @@ __device__ __forceinline__ void small_walk(const Geom<int>& g, int W, int H, F&&
     Edges e = small_edges(g, 1);
     float cx1 = (float)g.x3, cx2 = cx1, cz1 = g.z3, cz2 = g.z3;
     for (int cy = g.y3; cy >= g.y2; cy--) {
-      if (cy < 0) break;
-      if (cy < H) {
-        for (int ax = imax(f2i(cx1), 0); ax < W && (float)ax <= cx2; ax++) {
+      if (cy < ylo) break;
+      if (cy < yhi) {
+        for (int ax = imax(f2i(cx1), xlo); ax < xhi && (float)ax <= cx2; ax++) {
           float t = span_t(ax, cx1, cx2);
           frag(ax, cy, span_tz(t, cz1, cz2), 1);
         }
@@ -319,6 +323,20 @@ __device__ __forceinline__ void small_walk(const Geom<int>& g, int W, int H, F&&
       cz2 -= e.sz2;
     }
   }
+}
+
+// Pixel rectangle that contains every fragment of a small triangle: rows
+// y1..y3 exactly; columns within one pixel of the snapped vertices (span
+// ends accumulate rounding of well under a pixel over < 96 rows, and the
+// left end truncates), widened to two for margin.
+__device__ __forceinline__ void small_bounds(const float px[3], const float py[3], int& x0,
+                                             int& y0, int& x1, int& y1) {
+  const int a = f2i(px[0]), b = f2i(px[1]), c = f2i(px[2]);
+  const int d = f2i(py[0]), e = f2i(py[1]), f = f2i(py[2]);
+  x0 = imin(a, imin(b, c)) - 2;
+  x1 = imax(a, imax(b, c)) + 2;
+  y0 = imin(d, imin(e, f));
+  y1 = imax(d, imax(e, f));
 }
 
 // ------------------------------------------------- a9: tiled-path walk
@@ -454,15 +472,17 @@ __device__ __forceinline__ void coarse_walk(const float px[3], const float py[3]
 }
 
 // ------------------------------------------------ a11: shadow depth walk
-// Shaders/drawmin.c:86-176 (int coordinates, strict top loop).
+// Shaders/drawmin.c:86-176 (int coordinates, strict top loop) on a dim x dim
+// map, emitting only the texels inside [xlo, xhi) x [ylo, yhi).
 template <class F>
-__device__ __forceinline__ void shadow_walk(const AxrShRec& r, int wF, int hF, F&& frag) {
+__device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo, int ylo, int xhi,
+                                            int yhi, F&& frag) {
   int x1 = r.x[0], y1 = r.y[0], x2 = r.x[1], y2 = r.y[1], x3 = r.x[2], y3 = r.y[2];
   float z1 = r.z[0], z2 = r.z[1], z3 = r.z[2];
   if (y1 > y2) { swp(y1, y2); swp(x1, x2); swp(z1, z2); }
   if (y2 > y3) { swp(y2, y3); swp(x2, x3); swp(z2, z3); }
   if (y1 > y2) { swp(y1, y2); swp(x1, x2); swp(z1, z2); }
-  if (!(y1 < hF && y3 >= 0)) return;
+  if (!(y1 < dim && y3 >= 0)) return;
   const int x4 = f2i((float)x1 + ((float)(y2 - y1) / (float)(y3 - y1)) * (float)(x3 - x1));
   const int y4 = y2;
   const float z4 = z1 + (float)(y2 - y1) / (float)(y3 - y1) * (z3 - z1);
@@ -477,10 +497,10 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int wF, int hF, F
       swp(slopez1, slopez2);
     }
     for (int cy = y1; cy <= y2; cy++) {
-      if (cy >= hF) break;
-      if (cy >= 0) {
-        const int xe = imin(f2i(cx1), wF - 1);
-        for (int ax = imax(f2i(cx2), 0); ax <= xe; ax++) {
+      if (cy >= yhi) break;
+      if (cy >= ylo) {
+        const int xe = imin(f2i(cx1), xhi - 1);
+        for (int ax = imax(f2i(cx2), xlo); ax <= xe; ax++) {
           float t = span_t(ax, cx1, cx2);
           frag(ax, cy, span_tz(t, cz1, cz2));
         }
@@ -502,10 +522,10 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int wF, int hF, F
       swp(slopez1, slopez2);
     }
     for (int cy = y3; cy > y2; cy--) {
-      if (cy < 0) break;
-      if (cy < hF) {
-        const int xe = imin(f2i(cx2), wF - 1);
-        for (int ax = imax(f2i(cx1), 0); ax <= xe; ax++) {
+      if (cy < ylo) break;
+      if (cy < yhi) {
+        const int xe = imin(f2i(cx2), xhi - 1);
+        for (int ax = imax(f2i(cx1), xlo); ax <= xe; ax++) {
           float t = span_t(ax, cx1, cx2);
           frag(ax, cy, span_tz(t, cz1, cz2));
         }
@@ -516,6 +536,17 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int wF, int hF, F
       cz2 -= slopez2;
     }
   }
+}
+
+// Texel rectangle containing every fragment of a caster: rows y1..y3;
+// columns within one texel of the vertices (the int x4 and the truncated
+// span ends), widened to two.
+__device__ __forceinline__ void shadow_bounds(const AxrShRec& r, int& x0, int& y0, int& x1,
+                                              int& y1) {
+  x0 = imin(r.x[0], imin(r.x[1], r.x[2])) - 2;
+  x1 = imax(r.x[0], imax(r.x[1], r.x[2])) + 2;
+  y0 = imin(r.y[0], imin(r.y[1], r.y[2]));
+  y1 = imax(r.y[0], imax(r.y[1], r.y[2]));
 }
 
 // Pipe/trisetupOrtho.c:20-59.

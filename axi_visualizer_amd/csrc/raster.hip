@@ -1,18 +1,22 @@
-// raster.hip -- depth pass of the gfx950 raster path.
+// raster.hip -- setup and binning of the gfx950 raster path, and the
+// shadow-map depth pass.
 //
-//   k_clear_frame    depth keys to the clear key, per-frame counters to 0
-//   k_setup_raster   triangle setup of every drawn triangle (a5); a small
-//                    triangle is walked by the same lane at once (a7), a
-//                    large one is appended to the large list
-//   k_bin_count/scan/fill   coarse 16x16 tile binning of large triangles (a8)
-//   k_raster_tiles   one block per tile, keys resident in LDS (a9)
-//   k_clear_shadow / k_shadow   ortho setup + drawmin walk per caster (a11)
+// Main pass (drawAll), three launches:
+//   k_setup_bin  triangle setup (a5) of every drawn triangle; the setup
+//                record is stored by global index; small triangles count the
+//                32x32 bins their pixel rectangle touches, large triangles the
+//                bins of their coarse 16-px tiles (a8, Pipe/coarse.c)
+//   k_scan       bin offsets; zeroes the counts for the next frame
+//   k_bin_fill   writes bin entries (same bins, same order of rounds)
+// The per-bin depth walk and shading run later in one launch (shade.hip).
 //
-// Every fragment issues min(key) with key = (orderable tz << 32) | ~seq
-// (axr_common.h), so the result does not depend on launch order or on which
-// lane visits a fragment first: bins need no sorting, small triangles need
-// no compaction, and the reference's per-group launch sequence collapses
-// into single launches.
+// Shadow pass (shadowMap), four launches: k_shadow_setup_bin, k_scan,
+// k_shadow_fill and k_shadow_bins, which walks each bin's casters (a11,
+// Shaders/drawmin.c) with the bin's texels in LDS and writes them once.
+//
+// Bin counting aggregates within a wave: lanes of one wave come from one
+// mesh region and mostly share a bin, so one atomic serves all of them.
+// Depth minima live in LDS; no global atomics touch pixels or texels.
 #include "axr_launch.h"
 #include "axr_tri.h"
 
@@ -21,14 +25,6 @@ using namespace axr;
 namespace {
 
 constexpr int kBlock = 256;
-
-// Posted (no-return) device-scope minimum: the lane never waits on it.
-// Measured faster than reading the current value first to skip occluded
-// fragments, whose dependent load stalls every fragment of the walk.
-template <class T>
-__device__ __forceinline__ void atomic_min_posted(T* p, T k) {
-  __hip_atomic_fetch_min(p, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 __device__ __forceinline__ int find_group(const AxrGroupDev* __restrict__ groups, int ng,
                                           uint32_t gi) {
@@ -48,34 +44,53 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-// Appends `item` for every lane with `take`, one atomic per wave that has
-// any taker.
-template <class T>
-__device__ __forceinline__ void wave_append(bool take, const T& item, T* __restrict__ out,
-                                            uint32_t* __restrict__ counter, uint32_t cap) {
-  const uint64_t m = __ballot(take);
-  if (m == 0) return;
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t base = 0;
-  if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  if (take) {
-    const uint32_t slot = base + (uint32_t)__popcll(m & lanemask_lt());
-    if (slot < cap) out[slot] = item;
+// Adds one per active lane to ctr[bin] with one atomic per distinct bin in
+// the wave; returns the counter's previous value plus the lane's rank among
+// the lanes that share its bin.  All lanes of the wave must call it.
+__device__ __forceinline__ uint32_t wave_bin_add(uint32_t* __restrict__ ctr, bool active,
+                                                 uint32_t bin) {
+  uint64_t pending = __ballot(active);
+  uint32_t result = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t b = __shfl(bin, leader);
+    const uint64_t grp = __ballot(active && bin == b) & pending;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(&ctr[b], (uint32_t)__popcll(grp));
+    base = __shfl(base, leader);
+    if ((grp >> __lane_id()) & 1ull) result = base + (uint32_t)__popcll(grp & lanemask_lt());
+    pending &= ~grp;
   }
+  return result;
 }
 
-// Adds the wave's count of `flag` lanes to one of 64 spread stat slots.
-__device__ __forceinline__ void wave_count(bool flag, uint32_t* __restrict__ slots) {
-  const uint64_t m = __ballot(flag);
-  if (m != 0 && __lane_id() == 0)
-    atomicAdd(&slots[(blockIdx.x & 63) * AXR_CTR_STRIDE], (uint32_t)__popcll(m));
+// Bin rectangle of a pixel rectangle clipped to [0,w) x [0,h).
+struct BinRect {
+  int bx0, by0, nbx, n;
+};
+__device__ __forceinline__ BinRect bin_rect(int x0, int y0, int x1, int y1, int w, int h) {
+  BinRect r = {0, 0, 1, 0};
+  x0 = imax(x0, 0);
+  y0 = imax(y0, 0);
+  x1 = imin(x1, w - 1);
+  y1 = imin(y1, h - 1);
+  if (x0 <= x1 && y0 <= y1) {
+    r.bx0 = x0 >> AXR_BIN_SHIFT;
+    r.by0 = y0 >> AXR_BIN_SHIFT;
+    r.nbx = (x1 >> AXR_BIN_SHIFT) - r.bx0 + 1;
+    r.n = r.nbx * ((y1 >> AXR_BIN_SHIFT) - r.by0 + 1);
+  }
+  return r;
+}
+__device__ __forceinline__ uint32_t rect_bin(const BinRect& r, int k, int nbx_total) {
+  return (uint32_t)((r.by0 + k / r.nbx) * nbx_total + r.bx0 + k % r.nbx);
 }
 
-__global__ void k_fill_u64(uint64_t* __restrict__ p, size_t n, uint64_t v) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    p[i] = v;
+// 32-px bin and quadrant of a reference 16-px tile index.
+__device__ __forceinline__ void tile_to_bin(int t, int WC, int NBX, uint32_t& bin, uint32_t& sub) {
+  const int tx = t % WC, ty = t / WC;
+  bin = (uint32_t)((ty >> 1) * NBX + (tx >> 1));
+  sub = (uint32_t)(((ty & 1) << 1) | (tx & 1));
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
@@ -84,171 +99,223 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
-// clearZBuffer: keys to the clear key; draw counters and tile counts to 0.
-__global__ void k_clear_frame(uint64_t* __restrict__ keys, size_t n, uint64_t v,
-                              uint32_t* __restrict__ counters, uint32_t* __restrict__ tile_cnt,
-                              int ntile) {
-  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t st = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = t0; i < n; i += st) keys[i] = v;
-  for (size_t i = t0; i < (size_t)AXR_CTR_CAST; i += st) counters[i] = 0;
-  for (size_t i = t0; i < (size_t)ntile; i += st) tile_cnt[i] = 0;
-}
-
-// clearShadowMap: map to the clear depth key; caster counts to 0.
-__global__ void k_clear_shadow(uint32_t* __restrict__ map, size_t n, uint32_t v,
-                               uint32_t* __restrict__ counters) {
-  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t st = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = t0; i < n; i += st) map[i] = v;
-  for (size_t i = AXR_CTR_CAST + t0; i < (size_t)AXR_CTR_WORDS; i += st) counters[i] = 0;
-}
-
-// a5 + a7: one lane per triangle of every drawn group.
-__global__ __launch_bounds__(kBlock) void k_setup_raster(const float4* __restrict__ vpos,
-                                                         const float4* __restrict__ vattr,
-                                                         const AxrGroupDev* __restrict__ groups,
-                                                         AxrDrawConst dc,
-                                                         AxrTriRec* __restrict__ large,
-                                                         uint32_t* __restrict__ counters,
-                                                         unsigned long long* __restrict__ keys) {
+// K1: setup + bin counts.  cnt[0, NB): small entries per bin; cnt[NB, 2NB):
+// large (coarse tile) entries per bin.
+__global__ __launch_bounds__(kBlock) void k_setup_bin(const float4* __restrict__ vpos,
+                                                      const float4* __restrict__ vattr,
+                                                      const AxrGroupDev* __restrict__ groups,
+                                                      AxrDrawConst dc, AxrTriRec* __restrict__ rec,
+                                                      uint8_t* __restrict__ cls_out,
+                                                      uint32_t* __restrict__ cnt, int NBX,
+                                                      int NB) {
   const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
   int cls = 0;
   AxrTriRec r;
-  r.gi = gi;
   if (gi < dc.T) {
     const uint32_t f = groups[find_group(groups, (int)dc.ngroups, gi)].flags;
     if (f & AXR_GRP_DRAW)
       cls = tri_setup(vpos, vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
                       r.z);
+    cls_out[gi] = (uint8_t)cls;
+    if (cls) rec[gi] = r;
   }
-  wave_count(cls == 1, counters + AXR_CTR_SMALL);
-  wave_append(cls == 2, r, large, &counters[AXR_CTR_LARGE], dc.T);
-  if (cls != 1) return;
-  const Geom<int> g = small_geom(r.px, r.py, r.z);
-  const int W = dc.W;
-  small_walk(g, W, dc.H, [&](int ax, int cy, float tz, int half) {
-    if (!(tz <= AXR_DEPTH_CLEAR)) return;
-    const unsigned long long k = make_key(tz, frag_seq(gi, half));
-    atomic_min_posted(&keys[(size_t)cy * W + ax], k);
-  });
-}
-
-// a8: coarse binning, pass 1 -- entries per tile.
-__global__ __launch_bounds__(kBlock) void k_bin_count(const AxrTriRec* __restrict__ rec,
-                                                      const uint32_t* __restrict__ counters,
-                                                      uint32_t* __restrict__ tile_cnt, int WC,
-                                                      int HC) {
-  const uint32_t n = counters[AXR_CTR_LARGE];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const AxrTriRec r = rec[i];
-    coarse_walk(r.px, r.py, WC, HC, [&](int t) { atomicAdd(&tile_cnt[t], 1u); });
+  BinRect br = {0, 0, 1, 0};
+  if (cls == 1) {
+    int x0, y0, x1, y1;
+    small_bounds(r.px, r.py, x0, y0, x1, y1);
+    br = bin_rect(x0, y0, x1, y1, dc.W, dc.H);
+  }
+  for (int k = 0;; k++) {
+    const bool act = k < br.n;
+    if (__ballot(act) == 0) break;
+    wave_bin_add(cnt, act, act ? rect_bin(br, k, NBX) : 0u);
+  }
+  if (cls == 2) {
+    coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
+      uint32_t bin, sub;
+      tile_to_bin(t, dc.WC, NBX, bin, sub);
+      atomicAdd(&cnt[NB + bin], 1u);
+    });
   }
 }
 
-// Exclusive scan of the tile counts (single block); zeroes the counts so
-// they serve as fill cursors; flags capacity overflow.
-__global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* __restrict__ tile_cnt,
-                                                   uint32_t* __restrict__ tile_off, int NT,
-                                                   uint32_t* __restrict__ counters,
-                                                   uint32_t cap) {
-  __shared__ uint32_t part[1024];
+// K3: bin entries.  Small: global index; large: (global index << 2) | the
+// 16-px tile's quadrant in its bin.
+__global__ __launch_bounds__(kBlock) void k_bin_fill(const AxrTriRec* __restrict__ rec,
+                                                     const uint8_t* __restrict__ cls_in,
+                                                     uint32_t T, int W, int H, int WC, int HC,
+                                                     int NBX, int NB, uint32_t* __restrict__ cur,
+                                                     uint32_t* __restrict__ ent, uint32_t cap) {
+  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cls = gi < T ? cls_in[gi] : 0;
+  AxrTriRec r;
+  if (cls) r = rec[gi];
+  BinRect br = {0, 0, 1, 0};
+  if (cls == 1) {
+    int x0, y0, x1, y1;
+    small_bounds(r.px, r.py, x0, y0, x1, y1);
+    br = bin_rect(x0, y0, x1, y1, W, H);
+  }
+  for (int k = 0;; k++) {
+    const bool act = k < br.n;
+    if (__ballot(act) == 0) break;
+    const uint32_t pos = wave_bin_add(cur, act, act ? rect_bin(br, k, NBX) : 0u);
+    if (act && pos < cap) ent[pos] = gi;
+  }
+  if (cls == 2) {
+    coarse_walk(r.px, r.py, WC, HC, [&](int t) {
+      uint32_t bin, sub;
+      tile_to_bin(t, WC, NBX, bin, sub);
+      const uint32_t pos = atomicAdd(&cur[NB + bin], 1u);
+      if (pos < cap) ent[pos] = (gi << 2) | sub;
+    });
+  }
+}
+
+// K2: exclusive scan of n bin counts (one block).  off[0..n], cur = off,
+// counts zeroed for the next frame; total and overflow into ctr.
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt,
+                                               uint32_t* __restrict__ off,
+                                               uint32_t* __restrict__ cur, int n, uint32_t cap,
+                                               uint32_t* __restrict__ ctr_total,
+                                               uint32_t* __restrict__ ctr_ovf) {
+  __shared__ uint32_t wsum[16];
   const int tid = threadIdx.x;
-  const int per = (NT + 1023) / 1024;
-  const int b0 = tid * per, b1 = imin(NT, b0 + per);
+  const int lane = tid & 63, w = tid >> 6;
+  const int per = (n + 1023) / 1024;
+  const int b0 = imin(n, tid * per), b1 = imin(n, b0 + per);
   uint32_t s = 0;
-  for (int i = b0; i < b1; i++) s += tile_cnt[i];
-  part[tid] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = tid >= off ? part[tid - off] : 0u;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+  for (int i = b0; i < b1; i++) s += cnt[i];
+  uint32_t v = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o);
+    if (lane >= o) v += t;
   }
-  uint32_t run = part[tid] - s;
+  if (lane == 63) wsum[w] = v;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t x = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const uint32_t t = __shfl_up(x, o);
+      if (lane >= o) x += t;
+    }
+    if (lane < 16) wsum[lane] = x;
+  }
+  __syncthreads();
+  uint32_t run = v - s + (w > 0 ? wsum[w - 1] : 0u);
   for (int i = b0; i < b1; i++) {
-    const uint32_t c = tile_cnt[i];
-    tile_off[i] = run;
+    const uint32_t c = cnt[i];
+    off[i] = run;
+    cur[i] = run;
     run += c;
-    tile_cnt[i] = 0;
+    cnt[i] = 0;
   }
   if (tid == 1023) {
-    tile_off[NT] = part[1023];
-    counters[AXR_CTR_BIN_TOTAL] = part[1023];
-    counters[AXR_CTR_BIN_OVERFLOW] = part[1023] > cap ? part[1023] : 0u;
+    const uint32_t total = wsum[15];
+    off[n] = total;
+    *ctr_total = total;
+    *ctr_ovf = total > cap ? total : 0u;
   }
 }
 
-// a8: coarse binning, pass 2 -- scatter record indices into the bins.
-__global__ __launch_bounds__(kBlock) void k_bin_fill(const AxrTriRec* __restrict__ rec,
-                                                     const uint32_t* __restrict__ counters,
-                                                     uint32_t* __restrict__ cursor,
-                                                     const uint32_t* __restrict__ tile_off,
-                                                     uint32_t* __restrict__ bins, int WC, int HC,
-                                                     uint32_t cap) {
-  const uint32_t n = counters[AXR_CTR_LARGE];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const AxrTriRec r = rec[i];
-    coarse_walk(r.px, r.py, WC, HC, [&](int t) {
-      const uint32_t pos = tile_off[t] + atomicAdd(&cursor[t], 1u);
-      if (pos < cap) bins[pos] = i;
-    });
-  }
-}
-
-// a9: one block per 16x16 tile; the tile's keys live in LDS for the pass.
-__global__ __launch_bounds__(kBlock) void k_raster_tiles(const AxrTriRec* __restrict__ rec,
-                                                         const uint32_t* __restrict__ tile_off,
-                                                         const uint32_t* __restrict__ bins,
-                                                         unsigned long long* __restrict__ keys,
-                                                         int W, uint32_t T, uint32_t cap) {
-  __shared__ unsigned long long lk[256];
-  const int bx = blockIdx.x;
-  const uint32_t b0 = imin(tile_off[bx], cap), b1 = imin(tile_off[bx + 1], cap);
-  if (b0 >= b1) return;
-  const TileBox box = tile_box(bx, W);
-  const int x0 = (int)box.xMin, y0 = (int)box.yMin;
-  const int tid = threadIdx.x;
-  unsigned long long* gk = &keys[(size_t)(y0 + (tid >> 4)) * W + x0 + (tid & 15)];
-  lk[tid] = *gk;
-  __syncthreads();
-  for (uint32_t e = b0 + tid; e < b1; e += blockDim.x) {
-    const AxrTriRec r = rec[bins[e]];
-    const Geom<float> g = tile_geom(r.px, r.py, r.z);
-    const uint32_t order = T + r.gi;
-    tile_walk(g, box, [&](int ax, int cy, float tz, int half) {
-      if (!(tz <= AXR_DEPTH_CLEAR)) return;
-      const unsigned lc = (unsigned)((cy - y0) * 16 + (ax - x0));
-      if (lc < 256u) atomicMin(&lk[lc], (unsigned long long)make_key(tz, frag_seq(order, half)));
-    });
-  }
-  __syncthreads();
-  *gk = lk[tid];
-}
-
-// a11: ortho setup + drawmin walk per caster; the map holds orderable keys
-// of the minimum depth.
-__global__ __launch_bounds__(kBlock) void k_shadow(const float4* __restrict__ vpos,
-                                                   const AxrGroupDev* __restrict__ groups,
-                                                   uint32_t ngroups, uint32_t T, AxrCastMask cast,
-                                                   AxrCam cam, int dim, float bias,
-                                                   uint32_t* __restrict__ map,
-                                                   uint32_t* __restrict__ counters) {
+// S1: ortho setup of every casting group + bin counts.
+__global__ __launch_bounds__(kBlock) void k_shadow_setup_bin(
+    const float4* __restrict__ vpos, const AxrGroupDev* __restrict__ groups, uint32_t ngroups,
+    uint32_t T, AxrCastMask cast, AxrCam cam, int dim, float bias, AxrShRec* __restrict__ rec,
+    uint8_t* __restrict__ ok_out, uint32_t* __restrict__ cnt, int NBX) {
   const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
   bool ok = false;
   AxrShRec r;
   if (gi < T) {
-    const int g = find_group(groups, (int)ngroups, gi);
-    if (cast_bit(cast, g)) ok = shadow_setup(vpos, gi, cam, dim, bias, r);
+    if (cast_bit(cast, find_group(groups, (int)ngroups, gi)))
+      ok = shadow_setup(vpos, gi, cam, dim, bias, r);
+    ok_out[gi] = ok ? 1 : 0;
+    if (ok) rec[gi] = r;
   }
-  wave_count(ok, counters + AXR_CTR_CAST);
-  if (!ok) return;
-  shadow_walk(r, dim, dim, [&](int ax, int cy, float tz) {
-    if (tz != tz) return;
-    const uint32_t k = ordf(tz == 0.0f ? 0.0f : tz);
-    atomic_min_posted(&map[(size_t)cy * dim + ax], k);
-  });
+  BinRect br = {0, 0, 1, 0};
+  if (ok) {
+    int x0, y0, x1, y1;
+    shadow_bounds(r, x0, y0, x1, y1);
+    br = bin_rect(x0, y0, x1, y1, dim, dim);
+  }
+  for (int k = 0;; k++) {
+    const bool act = k < br.n;
+    if (__ballot(act) == 0) break;
+    wave_bin_add(cnt, act, act ? rect_bin(br, k, NBX) : 0u);
+  }
+}
+
+// S3: shadow bin entries (global index of the caster).
+__global__ __launch_bounds__(kBlock) void k_shadow_fill(const AxrShRec* __restrict__ rec,
+                                                        const uint8_t* __restrict__ ok_in,
+                                                        uint32_t T, int dim, int NBX,
+                                                        uint32_t* __restrict__ cur,
+                                                        uint32_t* __restrict__ ent, uint32_t cap) {
+  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = gi < T && ok_in[gi];
+  BinRect br = {0, 0, 1, 0};
+  if (ok) {
+    const AxrShRec r = rec[gi];
+    int x0, y0, x1, y1;
+    shadow_bounds(r, x0, y0, x1, y1);
+    br = bin_rect(x0, y0, x1, y1, dim, dim);
+  }
+  for (int k = 0;; k++) {
+    const bool act = k < br.n;
+    if (__ballot(act) == 0) break;
+    const uint32_t pos = wave_bin_add(cur, act, act ? rect_bin(br, k, NBX) : 0u);
+    if (act && pos < cap) ent[pos] = gi;
+  }
+}
+
+// S4: one block per 32x32 texel bin; minimum depth keys in LDS, written
+// once.  The bin starts from the map (from_map) or from the clear depth.
+__global__ __launch_bounds__(kBlock) void k_shadow_bins(const AxrShRec* __restrict__ rec,
+                                                        const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ ent,
+                                                        uint32_t cap, uint32_t* __restrict__ map,
+                                                        int dim, int NBX, int from_map) {
+  __shared__ uint32_t lk[AXR_BIN * AXR_BIN];
+  const int b = blockIdx.x;
+  const uint32_t s0 = imin(off[b], cap), s1 = imin(off[b + 1], cap);
+  if (from_map && s0 >= s1) return;
+  const int x0 = (b % NBX) << AXR_BIN_SHIFT, y0 = (b / NBX) << AXR_BIN_SHIFT;
+  const int xe = imin(x0 + AXR_BIN, dim), ye = imin(y0 + AXR_BIN, dim);
+  const uint32_t clear = ordf(AXR_DEPTH_CLEAR);
+  for (int i = threadIdx.x; i < AXR_BIN * AXR_BIN; i += blockDim.x) {
+    const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
+    lk[i] = (from_map && x < xe && y < ye) ? map[(size_t)y * dim + x] : clear;
+  }
+  __syncthreads();
+  for (uint32_t e = s0 + threadIdx.x; e < s1; e += blockDim.x) {
+    const AxrShRec r = rec[ent[e]];
+    shadow_walk(r, dim, x0, y0, xe, ye, [&](int ax, int cy, float tz) {
+      if (tz != tz) return;
+      const unsigned lc = (unsigned)(((cy - y0) << AXR_BIN_SHIFT) + (ax - x0));
+      if (lc < (unsigned)(AXR_BIN * AXR_BIN)) atomicMin(&lk[lc], ordf(tz == 0.0f ? 0.0f : tz));
+    });
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < AXR_BIN * AXR_BIN; i += blockDim.x) {
+    const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
+    if (x < xe && y < ye) map[(size_t)y * dim + x] = lk[i];
+  }
+}
+
+// Counts small / large / casting triangles on request (statistics only).
+__global__ void k_stats(const uint8_t* __restrict__ cls, const uint8_t* __restrict__ ok,
+                        uint32_t T, uint32_t* __restrict__ out) {
+  uint32_t s = 0, l = 0, c = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
+    s += cls[i] == 1;
+    l += cls[i] == 2;
+    c += ok[i] != 0;
+  }
+  atomicAdd(&out[0], s);
+  atomicAdd(&out[1], l);
+  atomicAdd(&out[2], c);
 }
 
 inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
@@ -262,75 +329,69 @@ inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
 
 namespace axr {
 
-hipError_t launch_fill_u64(uint64_t* p, size_t n, uint64_t v, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s, p, n, v);
-  return hipGetLastError();
-}
-
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s, p, n, v);
   return hipGetLastError();
 }
 
-hipError_t launch_clear_frame(uint64_t* keys, size_t n, uint32_t* counters, uint32_t* tile_cnt,
-                              int ntile, hipStream_t s) {
-  hipLaunchKernelGGL(k_clear_frame, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s, keys, n,
-                     (uint64_t)key_clear(), counters, tile_cnt, ntile);
-  return hipGetLastError();
-}
-
-hipError_t launch_clear_shadow(uint32_t* map, size_t n, uint32_t* counters, hipStream_t s) {
-  hipLaunchKernelGGL(k_clear_shadow, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, s, map, n,
-                     ordf(AXR_DEPTH_CLEAR), counters);
-  return hipGetLastError();
-}
-
-hipError_t launch_setup_raster(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
-                               const AxrDrawConst& dc, AxrTriRec* large, uint32_t* counters,
-                               uint64_t* keys, hipStream_t s) {
+hipError_t launch_setup_bin(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
+                            const AxrDrawConst& dc, AxrTriRec* rec, uint8_t* cls,
+                            uint32_t* cnt, int NBX, int NB, hipStream_t s) {
   if (dc.T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup_raster, dim3((dc.T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
-                     vattr, groups, dc, large, counters, (unsigned long long*)keys);
+  hipLaunchKernelGGL(k_setup_bin, dim3((dc.T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
+                     vattr, groups, dc, rec, cls, cnt, NBX, NB);
   return hipGetLastError();
 }
 
-hipError_t launch_bin_count(const AxrTriRec* large, const uint32_t* counters, uint32_t* tile_cnt,
-                            int WC, int HC, uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_count, dim3(grid_for(max_n, kBlock, 2048)), dim3(kBlock), 0, s, large,
-                     counters, tile_cnt, WC, HC);
+hipError_t launch_scan(uint32_t* cnt, uint32_t* off, uint32_t* cur, int n, uint32_t cap,
+                       uint32_t* ctr_total, uint32_t* ctr_ovf, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, cnt, off, cur, n, cap, ctr_total,
+                     ctr_ovf);
   return hipGetLastError();
 }
 
-hipError_t launch_bin_scan(uint32_t* tile_cnt, uint32_t* tile_off, int NT, uint32_t* counters,
+hipError_t launch_bin_fill(const AxrTriRec* rec, const uint8_t* cls, uint32_t T, int W, int H,
+                           int WC, int HC, int NBX, int NB, uint32_t* cur, uint32_t* ent,
                            uint32_t cap, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, tile_cnt, tile_off, NT, counters, cap);
-  return hipGetLastError();
-}
-
-hipError_t launch_bin_fill(const AxrTriRec* large, const uint32_t* counters, uint32_t* tile_cnt,
-                           const uint32_t* tile_off, uint32_t* bins, int WC, int HC, uint32_t cap,
-                           uint32_t max_n, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_fill, dim3(grid_for(max_n, kBlock, 2048)), dim3(kBlock), 0, s, large,
-                     counters, tile_cnt, tile_off, bins, WC, HC, cap);
-  return hipGetLastError();
-}
-
-hipError_t launch_raster_tiles(const AxrTriRec* large, const uint32_t* tile_off,
-                               const uint32_t* bins, uint64_t* keys, int W, int WC, int HC,
-                               uint32_t T, uint32_t cap, hipStream_t s) {
-  const int NT = WC * HC;
-  if (NT == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_raster_tiles, dim3(NT), dim3(kBlock), 0, s, large, tile_off, bins,
-                     (unsigned long long*)keys, W, T, cap);
-  return hipGetLastError();
-}
-
-hipError_t launch_shadow(const float4* vpos, const AxrGroupDev* groups, uint32_t ngroups,
-                         uint32_t T, const AxrCastMask& cast, const AxrCam& cam, int dim,
-                         float bias, uint32_t* map, uint32_t* counters, hipStream_t s) {
   if (T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_shadow, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos, groups,
-                     ngroups, T, cast, cam, dim, bias, map, counters);
+  hipLaunchKernelGGL(k_bin_fill, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rec, cls, T,
+                     W, H, WC, HC, NBX, NB, cur, ent, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_setup_bin(const float4* vpos, const AxrGroupDev* groups,
+                                   uint32_t ngroups, uint32_t T, const AxrCastMask& cast,
+                                   const AxrCam& cam, int dim, float bias, AxrShRec* rec,
+                                   uint8_t* ok, uint32_t* cnt, int NBX, hipStream_t s) {
+  if (T == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shadow_setup_bin, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     vpos, groups, ngroups, T, cast, cam, dim, bias, rec, ok, cnt, NBX);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_fill(const AxrShRec* rec, const uint8_t* ok, uint32_t T, int dim,
+                              int NBX, uint32_t* cur, uint32_t* ent, uint32_t cap,
+                              hipStream_t s) {
+  if (T == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shadow_fill, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rec, ok,
+                     T, dim, NBX, cur, ent, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow_bins(const AxrShRec* rec, const uint32_t* off, const uint32_t* ent,
+                              uint32_t cap, uint32_t* map, int dim, int NBX, int NB, int from_map,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_shadow_bins, dim3(NB), dim3(kBlock), 0, s, rec, off, ent, cap, map, dim,
+                     NBX, from_map);
+  return hipGetLastError();
+}
+
+hipError_t launch_stats(const uint8_t* cls, const uint8_t* ok, uint32_t T, uint32_t* out,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, out, (size_t)4, 0u);
+  if (T)
+    hipLaunchKernelGGL(k_stats, dim3(grid_for(T, kBlock, 1024)), dim3(kBlock), 0, s, cls, ok, T,
+                       out);
   return hipGetLastError();
 }
 

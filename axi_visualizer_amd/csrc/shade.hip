@@ -1,13 +1,18 @@
-// shade.hip -- resolve + shade pass of the gfx950 raster path, and the small
-// per-pixel / per-vertex kernels around it (gamma, readback helpers, vertL).
+// shade.hip -- per-bin depth walk and shading of the gfx950 raster path,
+// and the small per-pixel / per-vertex kernels around it (gamma, readback
+// helpers, vertL).
 //
-// The depth pass leaves one 64-bit key per pixel naming the winning
-// fragment (triangle, pass, scanline half).  Here each pixel is shaded
-// exactly once: the winner's setup and scanline walk are replayed up to the
-// pixel's row -- the same accumulation sequence the walk performed, so the
-// interpolants are bit-identical to the reference's at that fragment -- and
-// the colour is written with fully coalesced stores (RGBA8 with gamma fused,
-// or HDR u16 planes).  Overdraw is never shaded.
+// One block per 32x32 screen bin: the bin's 64-bit depth keys start at the
+// clear key in LDS, every small triangle binned there walks its scanlines
+// clipped to the bin (a7) and every large-triangle tile entry walks the
+// reference's 16-px tile (a9); each fragment issues an LDS min of
+// (orderable tz << 32) | ~seq.  The surviving key names the winning fragment
+// (triangle, pass, scanline half).  Each pixel is then shaded exactly once:
+// the winner's scanline walk is replayed up to the pixel's row -- the same
+// accumulation sequence, so the interpolants are bit-identical to the
+// reference's at that fragment -- and the colour is stored coalesced (RGBA8
+// with gamma fused, or HDR u16 planes).  Keys never leave LDS and overdraw
+// is never shaded.
 #include "axr_launch.h"
 #include "axr_tri.h"
 
@@ -40,7 +45,8 @@ __device__ __forceinline__ Attr pick3(const Attr& a0, const Attr& a1, const Attr
 
 // Rebuilds the winning fragment at (x, y) and shades it.
 __device__ __forceinline__ void resolve_pixel(uint64_t key, int x, int y,
-                                           const float4* __restrict__ vpos,
+                                              const AxrTriRec* __restrict__ rec,
+                                              const float4* __restrict__ vpos,
                                            const float4* __restrict__ vattr,
                                            const AxrGroupDev* __restrict__ groups,
                                            const AxrDrawConst& dc, uint16_t rgb[3]) {
@@ -51,8 +57,10 @@ __device__ __forceinline__ void resolve_pixel(uint64_t key, int x, int y,
   const uint32_t gi = large ? order - dc.T : order;
   const AxrGroupDev grp = groups[find_group(groups, (int)dc.ngroups, gi)];
 
-  float px[3], py[3], z[3];
-  tri_setup(vpos, vattr, gi, dc.view, dc.W, dc.H, (grp.flags & AXR_SH_CULL) != 0, px, py, z);
+  const AxrTriRec tr = rec[gi];
+  const float* px = tr.px;
+  const float* py = tr.py;
+  const float* z = tr.z;
   const Attr A0 = load_attr(vpos, vattr, 3 * gi, z[0]);
   const Attr A1 = load_attr(vpos, vattr, 3 * gi + 1, z[1]);
   const Attr A2 = load_attr(vpos, vattr, 3 * gi + 2, z[2]);
@@ -133,11 +141,6 @@ __device__ __forceinline__ void resolve_pixel(uint64_t key, int x, int y,
   shade(dc, grp, mi, t, tz, c1, c2, rgb);
 }
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-  const unsigned lane = __lane_id();
-  return lane ? (~0ull >> (64 - lane)) : 0ull;
-}
-
 template <int MODE>
 __device__ __forceinline__ void put_px(size_t p, float d, uint16_t rgb[3], float exposure,
                                        float* __restrict__ depth, uint16_t* __restrict__ ro,
@@ -154,53 +157,91 @@ __device__ __forceinline__ void put_px(size_t p, float d, uint16_t rgb[3], float
   }
 }
 
-// One block per 16x16 pixel tile.  Background pixels are written at once;
-// covered pixels are compacted in LDS so the shading work fills the first
-// waves of the block densely instead of idling lanes across all four.
+constexpr int kBinPx = AXR_BIN * AXR_BIN;
+
 template <int MODE>
-__global__ __launch_bounds__(256) void k_shade(const unsigned long long* __restrict__ keys,
-                                               const float4* __restrict__ vpos,
-                                               const float4* __restrict__ vattr,
-                                               const AxrGroupDev* __restrict__ groups,
-                                               AxrDrawConst dc, float exposure,
-                                               float* __restrict__ depth,
-                                               uint16_t* __restrict__ ro, uint16_t* __restrict__ go,
-                                               uint16_t* __restrict__ bo,
-                                               uint32_t* __restrict__ rgba) {
-  __shared__ uint8_t list[256];
-  __shared__ uint32_t wcount[4];
+__global__ __launch_bounds__(256) void k_bin_raster(
+    const AxrTriRec* __restrict__ rec, const uint32_t* __restrict__ off,
+    const uint32_t* __restrict__ ent, uint32_t cap, int empty, const float4* __restrict__ vpos,
+    const float4* __restrict__ vattr, const AxrGroupDev* __restrict__ groups, AxrDrawConst dc,
+    int NBX, int NB, float exposure, float* __restrict__ depth, uint16_t* __restrict__ ro,
+    uint16_t* __restrict__ go, uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba,
+    uint32_t* __restrict__ ids) {
+  __shared__ unsigned long long lk[kBinPx];
+  __shared__ uint16_t list[kBinPx];
+  __shared__ uint32_t ncov;
+  const int b = blockIdx.x;
+  const int bx = b % NBX, by = b / NBX;
+  const int x0 = bx << AXR_BIN_SHIFT, y0 = by << AXR_BIN_SHIFT;
+  const int xe = imin(x0 + AXR_BIN, dc.W), ye = imin(y0 + AXR_BIN, dc.H);
   const int tid = threadIdx.x;
-  const int x = blockIdx.x * 16 + (tid & 15);
-  const int y = blockIdx.y * 16 + (tid >> 4);
-  const bool inb = x < dc.W && y < dc.H;
-  const size_t p = (size_t)y * dc.W + x;
-  uint64_t key = inb ? keys[p] : 0xFFFFFFFFull;
-  const bool covered = inb && (uint32_t)key != 0xFFFFFFFFu;
-  if (inb && !covered) {
-    uint16_t rgb[3] = {dc.clear_rgb[0], dc.clear_rgb[1], dc.clear_rgb[2]};
-    put_px<MODE>(p, AXR_DEPTH_CLEAR, rgb, exposure, depth, ro, go, bo, rgba);
-  }
-  const uint64_t m = __ballot(covered);
-  const int wave = tid >> 6;
-  if (__lane_id() == 0) wcount[wave] = (uint32_t)__popcll(m);
+  for (int i = tid; i < kBinPx; i += blockDim.x) lk[i] = key_clear();
+  if (tid == 0) ncov = 0;
   __syncthreads();
-  uint32_t off = 0, n = 0;
-#pragma unroll
-  for (int w = 0; w < 4; w++) {
-    off += w < wave ? wcount[w] : 0u;
-    n += wcount[w];
+  if (!empty) {
+    // a7: small triangles, walk clipped to the bin
+    const uint32_t s0 = imin(off[b], cap), s1 = imin(off[b + 1], cap);
+    for (uint32_t e = s0 + tid; e < s1; e += blockDim.x) {
+      const uint32_t gi = ent[e];
+      const AxrTriRec r = rec[gi];
+      const Geom<int> g = small_geom(r.px, r.py, r.z);
+      small_walk(g, x0, y0, xe, ye, [&](int ax, int cy, float tz, int half) {
+        if (!(tz <= AXR_DEPTH_CLEAR)) return;
+        const unsigned lc = (unsigned)(((cy - y0) << AXR_BIN_SHIFT) + (ax - x0));
+        if (lc < (unsigned)kBinPx) atomicMin(&lk[lc], (unsigned long long)make_key(tz, frag_seq(gi, half)));
+      });
+    }
+    // a9: large triangles, one reference 16-px tile per entry
+    const uint32_t l0 = imin(off[NB + b], cap), l1 = imin(off[NB + b + 1], cap);
+    for (uint32_t e = l0 + tid; e < l1; e += blockDim.x) {
+      const uint32_t v = ent[e];
+      const uint32_t gi = v >> 2, sub = v & 3u;
+      const int t16 = ((by << 1) + (int)(sub >> 1)) * dc.WC + (bx << 1) + (int)(sub & 1u);
+      const TileBox box = tile_box(t16, dc.W);
+      const AxrTriRec r = rec[gi];
+      const Geom<float> g = tile_geom(r.px, r.py, r.z);
+      const uint32_t order = dc.T + gi;
+      tile_walk(g, box, [&](int ax, int cy, float tz, int half) {
+        if (!(tz <= AXR_DEPTH_CLEAR)) return;
+        const unsigned lc = (unsigned)(((cy - y0) << AXR_BIN_SHIFT) + (ax - x0));
+        if (lc < (unsigned)kBinPx) atomicMin(&lk[lc], (unsigned long long)make_key(tz, frag_seq(order, half)));
+      });
+    }
   }
-  if (covered) list[off + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)tid;
   __syncthreads();
-  if ((uint32_t)tid >= n) return;
-  const int lid = list[tid];
-  const int sx = blockIdx.x * 16 + (lid & 15);
-  const int sy = blockIdx.y * 16 + (lid >> 4);
-  const size_t sp = (size_t)sy * dc.W + sx;
-  key = keys[sp];
-  uint16_t rgb[3];
-  resolve_pixel(key, sx, sy, vpos, vattr, groups, dc, rgb);
-  put_px<MODE>(sp, unordf((uint32_t)(key >> 32)), rgb, exposure, depth, ro, go, bo, rgba);
+  for (int i = tid; i < kBinPx; i += blockDim.x) {
+    const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
+    if (x >= xe || y >= ye) continue;
+    const size_t p = (size_t)y * dc.W + x;
+    const uint64_t key = lk[i];
+    const uint32_t low = (uint32_t)key;
+    if (MODE == SHADE_IDS) {
+      uint32_t id = AXR_NONE;
+      if (low != 0xFFFFFFFFu) {
+        const uint32_t order = ((~low) - 1u) >> 1;
+        id = order >= dc.T ? order - dc.T : order;
+      }
+      ids[p] = id;
+      depth[p] = unordf((uint32_t)(key >> 32));
+    } else if (low == 0xFFFFFFFFu) {
+      uint16_t rgb[3] = {dc.clear_rgb[0], dc.clear_rgb[1], dc.clear_rgb[2]};
+      put_px<MODE>(p, AXR_DEPTH_CLEAR, rgb, exposure, depth, ro, go, bo, rgba);
+    } else {
+      list[atomicAdd(&ncov, 1u)] = (uint16_t)i;
+    }
+  }
+  if (MODE == SHADE_IDS) return;
+  __syncthreads();
+  const uint32_t n = ncov;
+  for (uint32_t j = tid; j < n; j += blockDim.x) {
+    const int i = list[j];
+    const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
+    const uint64_t key = lk[i];
+    uint16_t rgb[3];
+    resolve_pixel(key, x, y, rec, vpos, vattr, groups, dc, rgb);
+    put_px<MODE>((size_t)y * dc.W + x, unordf((uint32_t)(key >> 32)), rgb, exposure, depth, ro,
+                 go, bo, rgba);
+  }
 }
 
 __global__ void k_gamma_planes(uint16_t* __restrict__ r, uint16_t* __restrict__ g,
@@ -223,20 +264,6 @@ __global__ void k_expand_rgba8(const uint32_t* __restrict__ rgba, uint16_t* __re
     r[i] = (uint16_t)(v & 0xFF);
     g[i] = (uint16_t)((v >> 8) & 0xFF);
     b[i] = (uint16_t)((v >> 16) & 0xFF);
-  }
-}
-
-__global__ void k_ids(const unsigned long long* __restrict__ keys, uint32_t T,
-                      uint32_t* __restrict__ ids, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t low = (uint32_t)keys[i];
-    uint32_t id = AXR_NONE;
-    if (low != 0xFFFFFFFFu) {
-      const uint32_t order = ((~low) - 1u) >> 1;
-      id = order >= T ? order - T : order;
-    }
-    ids[i] = id;
   }
 }
 
@@ -300,18 +327,23 @@ inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
 
 namespace axr {
 
-hipError_t launch_shade(int mode, const uint64_t* keys, const float4* vpos, const float4* vattr,
-                        const AxrGroupDev* groups, const AxrDrawConst& dc, float exposure,
-                        float* depth, uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
-                        hipStream_t s) {
-  dim3 grid((dc.W + 15) / 16, (dc.H + 15) / 16);
-  const unsigned long long* k = (const unsigned long long*)keys;
+hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const uint32_t* off,
+                             const uint32_t* ent, uint32_t cap, int empty, const float4* vpos,
+                             const float4* vattr, const AxrGroupDev* groups,
+                             const AxrDrawConst& dc, int NBX, int NB, float exposure,
+                             float* depth, uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
+                             uint32_t* ids, hipStream_t s) {
+  if (NB == 0) return hipSuccess;
+#define AXR_LAUNCH_BIN(M)                                                                      \
+  hipLaunchKernelGGL(k_bin_raster<M>, dim3(NB), dim3(256), 0, s, rec, off, ent, cap, empty,    \
+                     vpos, vattr, groups, dc, NBX, NB, exposure, depth, r, g, b, rgba, ids)
   if (mode == SHADE_LDR8)
-    hipLaunchKernelGGL(k_shade<SHADE_LDR8>, grid, dim3(256), 0, s, k, vpos, vattr, groups, dc,
-                       exposure, depth, r, g, b, rgba);
+    AXR_LAUNCH_BIN(SHADE_LDR8);
+  else if (mode == SHADE_HDR)
+    AXR_LAUNCH_BIN(SHADE_HDR);
   else
-    hipLaunchKernelGGL(k_shade<SHADE_HDR>, grid, dim3(256), 0, s, k, vpos, vattr, groups, dc,
-                       exposure, depth, r, g, b, rgba);
+    AXR_LAUNCH_BIN(SHADE_IDS);
+#undef AXR_LAUNCH_BIN
   return hipGetLastError();
 }
 
@@ -326,12 +358,6 @@ hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, u
                                size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_expand_rgba8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rgba, r, g,
                      b, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_ids(const uint64_t* keys, uint32_t T, uint32_t* ids, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_ids, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s,
-                     (const unsigned long long*)keys, T, ids, n);
   return hipGetLastError();
 }
 

@@ -42,19 +42,42 @@ def algorithmic_bytes(sc, st):
     for c in sc.cascades:
         frame += T * 36 + 2 * c.dim * c.dim * 4
     visible = st["small"] + st["large"]
+    casters = st["casters"]
+    entries = st["bin_entries"]
     dmax = max((c.dim for c in sc.cascades), default=0)
     per_kernel = {
-        # keys in (8 B/px), RGBA8 + depth out (8 B/px), the winning
-        # triangles' vertex data (132 B each, bounded by the visible set)
-        "shade_gamma": W * H * 16 + visible * 132,
-        # positions of every triangle in (36 B), the key buffer out once
-        "setup_raster": T * 36 + W * H * 8,
-        # positions of every triangle in, the shadow map out once
-        "shadow": T * 36 + dmax * dmax * 4,
-        "clear": W * H * 8,
-        "shadow_clear": dmax * dmax * 4,
+        # bin entries and the setup records they name in; RGBA8 + depth out
+        # (8 B/px); the winning triangles' vertex data (132 B each, bounded
+        # by the visible set)
+        "raster_shade": entries * (4 + 36) + W * H * 8 + visible * 132,
+        # positions of every triangle (36 B) and its class byte; one 36 B
+        # setup record per visible triangle
+        "setup_bin": T * 37 + visible * 36,
+        "bin_fill": T + visible * 36 + entries * 4,
+        "shadow_setup_bin": T * 37 + casters * 36,
+        "shadow_fill": T + casters * (36 + 4),
+        # caster records via their bin entries; the map written once
+        "shadow_raster": casters * (36 + 4) + dmax * dmax * 4,
     }
     return frame, per_kernel
+
+
+def calibrate_copy(mb):
+    """One device-to-device copy of `mb` MiB through the HIP runtime: a plain
+    copy kernel of known traffic (mb MiB read + mb MiB written) that gives
+    the scale of the HBM byte counters (FETCH_SIZE, WRITE_SIZE) in a
+    profiled run."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def cpu_baseline(sc, budget_s=12.0):
@@ -92,6 +115,8 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--latency-frames", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -159,6 +184,8 @@ def main():
                 "algorithmic_bytes": int(dom_bytes), "avg_us": round(dom_us, 2)}
     frame_gbs = frame_bytes * fps / 1e9
 
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
     line = {
         "metric": "frames/s", "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1,

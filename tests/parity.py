@@ -24,6 +24,7 @@ def render_gpu(scene, hdr=False):
 
 def compare(scene, gpu, ref, label=""):
     """Asserts the bit-exact and tolerance contracts; returns a report."""
+    assert gpu["stats"]["bin_overflow"] == 0, f"{label}: bin pool overflow {gpu['stats']}"
     ids_eq = np.array_equal(gpu["id"], ref["id"])
     if not ids_eq:
         bad = np.argwhere(gpu["id"] != ref["id"])

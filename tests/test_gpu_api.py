@@ -1,0 +1,74 @@
+"""CLDraw call sequences on the GPU: lazy colour, lazy shadow clears,
+readbacks in any order."""
+import numpy as np
+import pytest
+
+from axi_visualizer_amd import scenes
+from axi_visualizer_amd.render import Renderer
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def small_spheres():
+    return scenes.spheres_scene(W=320, H=240, n_inst=60, tex_side=64, shadow_dim=256,
+                                cam_dist=150.0, name="api_spheres")
+
+
+def test_ids_same_before_and_after_shading():
+    r = Renderer(small_spheres())
+    try:
+        r.frame(gamma=False)
+        before = r.draw.getIds()
+        r.draw.gamma(1.0)
+        after = r.draw.getIds()
+        rgba = r.draw.getRGBA8()
+        planes = r.draw.getFrame()
+        assert np.array_equal(before, after)
+        for c in range(3):
+            assert np.array_equal(planes[c], rgba[..., c].astype(np.uint16))
+    finally:
+        r.close()
+
+
+def test_shadow_clear_is_lazy_and_map_rerender_is_idempotent():
+    sc = small_spheres()
+    r = Renderer(sc)
+    try:
+        d = r.draw
+        d.clearShadowMap(0)
+        cleared = d.getSHM(0)
+        assert (cleared == 65535.0).all()
+        d.clearShadowMap(0)
+        d.shadowMap(0, r.cast, r.shaders, sc.cascades[0].bias)
+        once = d.getSHM(0)
+        d.shadowMap(0, r.cast, r.shaders, sc.cascades[0].bias)  # min with itself
+        twice = d.getSHM(0)
+        assert np.array_equal(once.view(np.uint32), twice.view(np.uint32))
+        ref = pyoracle.render(sc, with_shadows_out=True)["shadow"][0]
+        assert np.array_equal(once.view(np.uint32), ref.view(np.uint32))
+        assert (once < 65535.0).any()
+    finally:
+        r.close()
+
+
+def test_second_draw_without_clear_is_refused():
+    r = Renderer(small_spheres())
+    try:
+        r.frame()
+        with pytest.raises(RuntimeError, match="clearZBuffer"):
+            r.draw.drawAll(r.shaders, mask=r.mask)
+    finally:
+        r.close()
+
+
+def test_frame_twice_gives_identical_pixels():
+    r = Renderer(small_spheres())
+    try:
+        r.frame()
+        a = r.draw.getRGBA8().copy()
+        r.frame()
+        b = r.draw.getRGBA8()
+        assert np.array_equal(a, b)
+    finally:
+        r.close()
