@@ -106,9 +106,11 @@ struct axr_ctx {
 
   // main bins: counts/offsets/cursors over 2*NB (small, then large lists)
   uint32_t *d_bcnt = nullptr, *d_boff = nullptr, *d_bcur = nullptr, *d_bent = nullptr;
+  uint32_t* d_border = nullptr;  // [NB] heavy-first bin order
   uint32_t bin_cap = 0;
   // shadow bins, sized for the largest map
   uint32_t *d_scnt = nullptr, *d_soff = nullptr, *d_scur = nullptr, *d_sent = nullptr;
+  uint32_t* d_sorder = nullptr;
   int snb_cap = 0;
   uint32_t sbin_cap = 0;
 
@@ -188,6 +190,15 @@ int dalloc(T** p, size_t n) {
   *p = nullptr;
   if (n == 0) n = 1;
   HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+  return 0;
+}
+
+// Identity launch order, valid before the first scan writes one.
+int identity_order(uint32_t* d, int n, hipStream_t s) {
+  std::vector<uint32_t> h((size_t)n);
+  for (int i = 0; i < n; i++) h[(size_t)i] = (uint32_t)i;
+  HIPCHK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
   return 0;
 }
 
@@ -276,7 +287,8 @@ int bin_raster(axr_ctx* c, int mode, float exposure) {
   }
   Timed tm(c, mode == SHADE_LDR8 ? "raster_shade" : mode == SHADE_HDR ? "raster_hdr" : "raster_ids");
   HIPCHK(launch_bin_raster(mode, c->d_rec, c->d_boff, c->d_bent, c->bin_cap, empty, c->d_vpos,
-                           c->d_vattr, c->d_groups, dc, c->NBX, c->NB, exposure, c->d_depth,
+                           c->d_vattr, c->d_groups, dc, c->NBX, c->NB, c->d_border, exposure,
+                           c->d_depth,
                            c->d_r, c->d_g, c->d_b, c->d_rgba, c->d_ids, c->stream));
   return 0;
 }
@@ -371,6 +383,7 @@ int axr_create(int w, int h, axr_ctx** out) {
       (rc = dalloc(&c->d_ids, np)) || (rc = dalloc(&c->d_counters, kCounters)) ||
       (rc = dalloc(&c->d_bcnt, nb2)) || (rc = dalloc(&c->d_boff, nb2 + 1)) ||
       (rc = dalloc(&c->d_bcur, nb2)) || (rc = dalloc(&c->d_bent, kDefaultBinCap)) ||
+      (rc = dalloc(&c->d_border, (size_t)c->NB)) ||
       (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)))
     return bail(rc);
   c->bin_cap = kDefaultBinCap;
@@ -379,6 +392,7 @@ int axr_create(int w, int h, axr_ctx** out) {
       launch_fill_u32(c->d_boff, nb2 + 1, 0, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(fail(AXR_EHIP, "initial fill failed"));
+  if ((rc = identity_order(c->d_border, c->NB, c->stream))) return bail(rc);
   // Default view: identity basis at the origin, 90 degree fovx.
   const float vm[9] = {0, 0, 1, -1, 0, 0, 0, -1, 0};
   std::memcpy(c->view.vm, vm, sizeof vm);
@@ -411,6 +425,8 @@ int axr_destroy(axr_ctx* c) {
   dfree(c->d_boff);
   dfree(c->d_bcur);
   dfree(c->d_bent);
+  dfree(c->d_border);
+  dfree(c->d_sorder);
   dfree(c->d_scnt);
   dfree(c->d_soff);
   dfree(c->d_scur);
@@ -656,9 +672,9 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
     }
     {
       Timed tm(c, "bin_scan");
-      HIPCHK(launch_scan(c->d_bcnt, c->d_boff, c->d_bcur, 2 * c->NB, c->bin_cap,
+      HIPCHK(launch_scan(c->d_bcnt, c->d_boff, c->d_bcur, c->NB, 2, c->bin_cap,
                          c->d_counters + AXR_CTR_BIN_TOTAL, c->d_counters + AXR_CTR_BIN_OVERFLOW,
-                         c->stream));
+                         c->d_border, c->stream));
     }
     {
       Timed tm(c, "bin_fill");
@@ -695,10 +711,12 @@ int axr_add_shadow_map(axr_ctx* c, int i, int size, float scale, float amb) {
     dfree(c->d_scnt);
     dfree(c->d_soff);
     dfree(c->d_scur);
+    dfree(c->d_sorder);
     if ((rc = dalloc(&c->d_scnt, s.nb)) || (rc = dalloc(&c->d_soff, s.nb + 1)) ||
-        (rc = dalloc(&c->d_scur, s.nb)))
+        (rc = dalloc(&c->d_scur, s.nb)) || (rc = dalloc(&c->d_sorder, s.nb)))
       return rc;
     HIPCHK(launch_fill_u32(c->d_scnt, s.nb, 0, c->stream));
+    if ((rc = identity_order(c->d_sorder, s.nb, c->stream))) return rc;
     c->snb_cap = s.nb;
   }
   if (!c->d_sent) {
@@ -753,9 +771,9 @@ int axr_shadow_map(axr_ctx* c, int i, const uint8_t* cast, int ngroups, float bi
   }
   {
     Timed tm(c, "shadow_scan");
-    HIPCHK(launch_scan(c->d_scnt, c->d_soff, c->d_scur, s.nb, c->sbin_cap,
+    HIPCHK(launch_scan(c->d_scnt, c->d_soff, c->d_scur, s.nb, 1, c->sbin_cap,
                        c->d_counters + AXR_CTR_SH_TOTAL, c->d_counters + AXR_CTR_SH_OVERFLOW,
-                       c->stream));
+                       c->d_sorder, c->stream));
   }
   {
     Timed tm(c, "shadow_fill");
@@ -765,7 +783,7 @@ int axr_shadow_map(axr_ctx* c, int i, const uint8_t* cast, int ngroups, float bi
   {
     Timed tm(c, "shadow_raster");
     HIPCHK(launch_shadow_bins(c->d_shrec, c->d_soff, c->d_sent, c->sbin_cap, s.d_map, s.dim,
-                              s.nbx, s.nb, s.clear_pending ? 0 : 1, c->stream));
+                              s.nbx, s.nb, s.clear_pending ? 0 : 1, c->d_sorder, c->stream));
   }
   s.clear_pending = false;
   return AXR_OK;

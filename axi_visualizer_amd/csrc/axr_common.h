@@ -84,6 +84,14 @@ AXR_HD bool cast_bit(const AxrCastMask& m, int g) {
 #define AXR_BIN 32
 #define AXR_BIN_SHIFT 5
 
+// Threads of one bin's block in the per-bin depth walks (screen and shadow).
+#ifndef AXR_RASTER_THREADS
+#define AXR_RASTER_THREADS 512
+#endif
+#ifndef AXR_SHADOW_THREADS
+#define AXR_SHADOW_THREADS 512
+#endif
+
 // Device counter block.
 #define AXR_CTR_BIN_TOTAL 0     // bin entries of the last drawAll
 #define AXR_CTR_BIN_OVERFLOW 1  // entries needed when above capacity, else 0

@@ -172,13 +172,25 @@ __global__ __launch_bounds__(kBlock) void k_bin_fill(const AxrTriRec* __restrict
   }
 }
 
-// K2: exclusive scan of n bin counts (one block).  off[0..n], cur = off,
-// counts zeroed for the next frame; total and overflow into ctr.
+constexpr int kOrderClasses = 66;
+// 0 for an empty bin, else 2*floor(log2 w) + (next bit) + 2.
+__device__ __forceinline__ int order_class(uint32_t w) {
+  if (w == 0) return 0;
+  const int l = 31 - __clz(w);
+  const int half = l > 0 ? (int)((w >> (l - 1)) & 1u) : 0;
+  return 2 * l + half + 2;
+}
+
+// K2: exclusive scan of n = nbins * nseg bin counts (one block).  off[0..n],
+// cur = off, counts zeroed for the next frame; total and overflow into ctr;
+// order[0..nbins) = bins by descending weight.
 __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt,
                                                uint32_t* __restrict__ off,
-                                               uint32_t* __restrict__ cur, int n, uint32_t cap,
-                                               uint32_t* __restrict__ ctr_total,
-                                               uint32_t* __restrict__ ctr_ovf) {
+                                               uint32_t* __restrict__ cur, int nbins, int nseg,
+                                               uint32_t cap, uint32_t* __restrict__ ctr_total,
+                                               uint32_t* __restrict__ ctr_ovf,
+                                               uint32_t* __restrict__ order) {
+  const int n = nbins * nseg;
   __shared__ uint32_t wsum[16];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
@@ -217,6 +229,34 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt,
     off[n] = total;
     *ctr_total = total;
     *ctr_ovf = total > cap ? total : 0u;
+  }
+  // Heavy-first launch order of the nbins bins (weight = entries over the
+  // nseg segments), so the longest bins start first and the short ones
+  // fill the tail.  Classes are half-octaves of the weight; order within a
+  // class is arbitrary (every bin is computed by one block, so the result
+  // does not depend on it).
+  __shared__ uint32_t hist[kOrderClasses];
+  for (int i = tid; i < kOrderClasses; i += 1024) hist[i] = 0;
+  __syncthreads();  // also publishes off[] to the whole block
+  for (int i = tid; i < nbins; i += 1024) {
+    uint32_t wgt = 0;
+    for (int sg = 0; sg < nseg; sg++) wgt += off[sg * nbins + i + 1] - off[sg * nbins + i];
+    atomicAdd(&hist[order_class(wgt)], 1u);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int k = kOrderClasses - 1; k >= 0; k--) {
+      const uint32_t c = hist[k];
+      hist[k] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nbins; i += 1024) {
+    uint32_t wgt = 0;
+    for (int sg = 0; sg < nseg; sg++) wgt += off[sg * nbins + i + 1] - off[sg * nbins + i];
+    order[atomicAdd(&hist[order_class(wgt)], 1u)] = (uint32_t)i;
   }
 }
 
@@ -272,13 +312,14 @@ __global__ __launch_bounds__(kBlock) void k_shadow_fill(const AxrShRec* __restri
 
 // S4: one block per 32x32 texel bin; minimum depth keys in LDS, written
 // once.  The bin starts from the map (from_map) or from the clear depth.
-__global__ __launch_bounds__(kBlock) void k_shadow_bins(const AxrShRec* __restrict__ rec,
+__global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(const AxrShRec* __restrict__ rec,
                                                         const uint32_t* __restrict__ off,
                                                         const uint32_t* __restrict__ ent,
                                                         uint32_t cap, uint32_t* __restrict__ map,
-                                                        int dim, int NBX, int from_map) {
+                                                        int dim, int NBX, int from_map,
+                                                        const uint32_t* __restrict__ order) {
   __shared__ uint32_t lk[AXR_BIN * AXR_BIN];
-  const int b = blockIdx.x;
+  const int b = (int)order[blockIdx.x];
   const uint32_t s0 = imin(off[b], cap), s1 = imin(off[b + 1], cap);
   if (from_map && s0 >= s1) return;
   const int x0 = (b % NBX) << AXR_BIN_SHIFT, y0 = (b / NBX) << AXR_BIN_SHIFT;
@@ -343,10 +384,11 @@ hipError_t launch_setup_bin(const float4* vpos, const float4* vattr, const AxrGr
   return hipGetLastError();
 }
 
-hipError_t launch_scan(uint32_t* cnt, uint32_t* off, uint32_t* cur, int n, uint32_t cap,
-                       uint32_t* ctr_total, uint32_t* ctr_ovf, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, cnt, off, cur, n, cap, ctr_total,
-                     ctr_ovf);
+hipError_t launch_scan(uint32_t* cnt, uint32_t* off, uint32_t* cur, int nbins, int nseg,
+                       uint32_t cap, uint32_t* ctr_total, uint32_t* ctr_ovf, uint32_t* order,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, cnt, off, cur, nbins, nseg, cap,
+                     ctr_total, ctr_ovf, order);
   return hipGetLastError();
 }
 
@@ -380,9 +422,9 @@ hipError_t launch_shadow_fill(const AxrShRec* rec, const uint8_t* ok, uint32_t T
 
 hipError_t launch_shadow_bins(const AxrShRec* rec, const uint32_t* off, const uint32_t* ent,
                               uint32_t cap, uint32_t* map, int dim, int NBX, int NB, int from_map,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow_bins, dim3(NB), dim3(kBlock), 0, s, rec, off, ent, cap, map, dim,
-                     NBX, from_map);
+                              const uint32_t* order, hipStream_t s) {
+  hipLaunchKernelGGL(k_shadow_bins, dim3(NB), dim3(AXR_SHADOW_THREADS), 0, s, rec, off, ent, cap, map, dim,
+                     NBX, from_map, order);
   return hipGetLastError();
 }
 

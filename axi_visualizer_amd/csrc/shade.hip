@@ -160,17 +160,19 @@ __device__ __forceinline__ void put_px(size_t p, float d, uint16_t rgb[3], float
 constexpr int kBinPx = AXR_BIN * AXR_BIN;
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_bin_raster(
+__global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
     const AxrTriRec* __restrict__ rec, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ ent, uint32_t cap, int empty, const float4* __restrict__ vpos,
     const float4* __restrict__ vattr, const AxrGroupDev* __restrict__ groups, AxrDrawConst dc,
-    int NBX, int NB, float exposure, float* __restrict__ depth, uint16_t* __restrict__ ro,
+    int NBX, int NB, const uint32_t* __restrict__ order, float exposure,
+    float* __restrict__ depth, uint16_t* __restrict__ ro,
     uint16_t* __restrict__ go, uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba,
     uint32_t* __restrict__ ids) {
   __shared__ unsigned long long lk[kBinPx];
   __shared__ uint16_t list[kBinPx];
   __shared__ uint32_t ncov;
-  const int b = blockIdx.x;
+  // Heavy bins first (order from the scan); an empty draw has no order.
+  const int b = empty ? (int)blockIdx.x : (int)order[blockIdx.x];
   const int bx = b % NBX, by = b / NBX;
   const int x0 = bx << AXR_BIN_SHIFT, y0 = by << AXR_BIN_SHIFT;
   const int xe = imin(x0 + AXR_BIN, dc.W), ye = imin(y0 + AXR_BIN, dc.H);
@@ -330,13 +332,13 @@ namespace axr {
 hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const uint32_t* off,
                              const uint32_t* ent, uint32_t cap, int empty, const float4* vpos,
                              const float4* vattr, const AxrGroupDev* groups,
-                             const AxrDrawConst& dc, int NBX, int NB, float exposure,
-                             float* depth, uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
-                             uint32_t* ids, hipStream_t s) {
+                             const AxrDrawConst& dc, int NBX, int NB, const uint32_t* order,
+                             float exposure, float* depth, uint16_t* r, uint16_t* g, uint16_t* b,
+                             uint32_t* rgba, uint32_t* ids, hipStream_t s) {
   if (NB == 0) return hipSuccess;
 #define AXR_LAUNCH_BIN(M)                                                                      \
-  hipLaunchKernelGGL(k_bin_raster<M>, dim3(NB), dim3(256), 0, s, rec, off, ent, cap, empty,    \
-                     vpos, vattr, groups, dc, NBX, NB, exposure, depth, r, g, b, rgba, ids)
+  hipLaunchKernelGGL(k_bin_raster<M>, dim3(NB), dim3(AXR_RASTER_THREADS), 0, s, rec, off, ent, cap, empty,    \
+                     vpos, vattr, groups, dc, NBX, NB, order, exposure, depth, r, g, b, rgba, ids)
   if (mode == SHADE_LDR8)
     AXR_LAUNCH_BIN(SHADE_LDR8);
   else if (mode == SHADE_HDR)
