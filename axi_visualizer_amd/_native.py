@@ -44,6 +44,8 @@ SIGNATURES = {
     "axr_get_shadow_map": (_i, [_ctx, _i, _p]),
     "axr_sync": (_i, [_ctx]),
     "axr_get_stats": (_i, [_ctx, _p]),
+    "axr_set_bin_capacity": (_i, [_ctx, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.c_uint32]),
     "axr_set_timing": (_i, [_ctx, _i]),
     "axr_get_timing": (_i, [_ctx, _p, _i, _p, _p, _i]),
 }

@@ -82,8 +82,28 @@ struct TimingRec {
   hipEvent_t a, b;
 };
 
-constexpr uint32_t kDefaultBinCap = 16u << 20;
 constexpr int kCounters = AXR_CTR_WORDS;
+
+// Bucket capacities (entries per bin slot).  A pass appends into fixed
+// buckets; what does not fit goes to a spill list that the raster scans for
+// the affected bins only, and a pass that spilled grows the buckets before
+// the next pass of its kind.
+constexpr uint32_t kMainCap0 = 4096;   // small triangles per screen bin
+constexpr uint32_t kMainCap1 = 1024;   // large-triangle tiles per screen bin
+constexpr uint32_t kShadowCap = 8192;  // casters per shadow bin
+constexpr uint32_t kMinCap = 256;
+constexpr size_t kBucketBudget = (size_t)256 << 20;      // bytes for the initial buckets
+constexpr size_t kBucketGrowLimit = (size_t)4 << 30;     // bytes a growth may reach
+constexpr uint32_t kSpillCap = 1u << 20;                 // spill entries
+
+// Device bin storage of one pass kind.
+struct BinStore {
+  AxrBins B{};            // pointers and capacities as kernels see them
+  int nseg = 1;
+  int nb_alloc = 0;       // bins the cursor/count/order arrays hold
+  size_t ent_alloc = 0;   // bucket entries allocated
+  uint32_t want0 = 0, want1 = 0, want_spill = 0;  // capacities for the next pass
+};
 
 }  // namespace
 
@@ -104,15 +124,8 @@ struct axr_ctx {
   AxrShRec* d_shrec = nullptr;  // [tri_cap] ortho setup of the last shadow pass
   uint8_t* d_ok = nullptr;      // [tri_cap] caster kept
 
-  // main bins: counts/offsets/cursors over 2*NB (small, then large lists)
-  uint32_t *d_bcnt = nullptr, *d_boff = nullptr, *d_bcur = nullptr, *d_bent = nullptr;
-  uint32_t* d_border = nullptr;  // [NB] heavy-first bin order
-  uint32_t bin_cap = 0;
-  // shadow bins, sized for the largest map
-  uint32_t *d_scnt = nullptr, *d_soff = nullptr, *d_scur = nullptr, *d_sent = nullptr;
-  uint32_t* d_sorder = nullptr;
-  int snb_cap = 0;
-  uint32_t sbin_cap = 0;
+  BinStore mbins;  // screen bins: slots [0, NB) small, [NB, 2NB) large tiles
+  BinStore sbins;  // shadow bins, sized for the largest map
 
   float* d_depth = nullptr;
   uint16_t *d_r = nullptr, *d_g = nullptr, *d_b = nullptr;
@@ -193,19 +206,98 @@ int dalloc(T** p, size_t n) {
   return 0;
 }
 
-// Identity launch order, valid before the first scan writes one.
-int identity_order(uint32_t* d, int n, hipStream_t s) {
-  std::vector<uint32_t> h((size_t)n);
-  for (int i = 0; i < n; i++) h[(size_t)i] = (uint32_t)i;
-  HIPCHK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return 0;
-}
-
 template <class T>
 void dfree(T*& p) {
   if (p) (void)hipFree((void*)p);
   p = nullptr;
+}
+
+void bins_free(BinStore& S) {
+  dfree(S.B.cnt);
+  dfree(S.B.num);
+  dfree(S.B.ent);
+  dfree(S.B.order);
+  dfree(S.B.spill);
+  dfree(S.B.spill_ctr);
+  S.nb_alloc = 0;
+  S.ent_alloc = 0;
+}
+
+// Initial bucket capacity: `cap`, reduced to fit the budget for nb bins.
+uint32_t fit_cap(uint32_t cap, size_t nb, int nseg) {
+  const size_t per = kBucketBudget / (sizeof(uint32_t) * (size_t)nseg * (nb ? nb : 1));
+  if (per < cap) cap = per < kMinCap ? kMinCap : (uint32_t)per;
+  return cap;
+}
+
+// Makes S hold nb bins with the wanted capacities.  Only called when the
+// bins of the previous pass of this kind are no longer needed: their
+// entries are dropped if storage has to grow.  The stream is idle after.
+int bins_prepare(BinStore& S, int nb, hipStream_t st) {
+  const size_t slots = (size_t)S.nseg * nb;
+  const size_t ents = (size_t)nb * S.want0 + (S.nseg > 1 ? (size_t)nb * S.want1 : 0);
+  const bool grow_slots = nb > S.nb_alloc;
+  const bool grow_ents = ents > S.ent_alloc || S.want0 != S.B.cap0 || S.want1 != S.B.cap1;
+  const bool grow_spill = S.want_spill != S.B.spill_cap;
+  if (!grow_slots && !grow_ents && !grow_spill && S.B.cnt) {
+    S.B.nb = nb;
+    return 0;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  int rc;
+  if (grow_slots || !S.B.cnt) {
+    dfree(S.B.cnt);
+    dfree(S.B.num);
+    dfree(S.B.order);
+    const size_t all = (size_t)S.nseg * nb;
+    if ((rc = dalloc(&S.B.cnt, all)) || (rc = dalloc(&S.B.num, all)) ||
+        (rc = dalloc(&S.B.order, (size_t)nb)))
+      return rc;
+    HIPCHK(launch_fill_u32(S.B.cnt, all, 0, st));
+    HIPCHK(launch_fill_u32(S.B.num, all, 0, st));
+    std::vector<uint32_t> h((size_t)nb);
+    for (int i = 0; i < nb; i++) h[(size_t)i] = (uint32_t)i;
+    HIPCHK(hipMemcpyAsync(S.B.order, h.data(), h.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, st));
+    S.nb_alloc = nb;
+  }
+  if (grow_ents || !S.B.ent) {
+    dfree(S.B.ent);
+    if ((rc = dalloc(&S.B.ent, ents))) return rc;
+    S.ent_alloc = ents;
+  }
+  if (grow_spill || !S.B.spill) {
+    dfree(S.B.spill);
+    dfree(S.B.spill_ctr);
+    if ((rc = dalloc(&S.B.spill, (size_t)S.want_spill)) || (rc = dalloc(&S.B.spill_ctr, 2)))
+      return rc;
+    HIPCHK(launch_fill_u32(S.B.spill_ctr, 2, 0, st));
+  }
+  S.B.cap0 = S.want0;
+  S.B.cap1 = S.nseg > 1 ? S.want1 : 0;
+  S.B.spill_cap = S.want_spill;
+  S.B.nb = nb;
+  (void)slots;
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+// Raises the wanted capacities after a pass of this kind spilled or lost
+// entries (applied by the next bins_prepare).
+void bins_grow(BinStore& S, uint32_t spilled, uint32_t lost) {
+  if (spilled) {
+    const size_t cur = (size_t)S.nb_alloc * (S.want0 + S.want1) * sizeof(uint32_t);
+    if (2 * cur <= kBucketGrowLimit) {
+      S.want0 *= 2;
+      if (S.nseg > 1) S.want1 *= 2;
+    }
+  }
+  if (lost) {
+    uint64_t need = (uint64_t)S.want_spill + lost;
+    uint64_t cap = 2 * (uint64_t)S.want_spill;
+    while (cap < need) cap *= 2;
+    S.want_spill = cap > 0x7FFFFFFFu ? 0x7FFFFFFFu : (uint32_t)cap;
+  }
 }
 
 int check_ctx(axr_ctx* c) {
@@ -286,10 +378,11 @@ int bin_raster(axr_ctx* c, int mode, float exposure) {
     for (int k = 0; k < 3; k++) dc.clear_rgb[k] = c->clear_rgb[k];
   }
   Timed tm(c, mode == SHADE_LDR8 ? "raster_shade" : mode == SHADE_HDR ? "raster_hdr" : "raster_ids");
-  HIPCHK(launch_bin_raster(mode, c->d_rec, c->d_boff, c->d_bent, c->bin_cap, empty, c->d_vpos,
-                           c->d_vattr, c->d_groups, dc, c->NBX, c->NB, c->d_border, exposure,
-                           c->d_depth,
-                           c->d_r, c->d_g, c->d_b, c->d_rgba, c->d_ids, c->stream));
+  AxrBins B = c->mbins.B;
+  B.nb = c->NB;
+  HIPCHK(launch_bin_raster(mode, c->d_rec, B, empty, c->d_vpos, c->d_vattr, c->d_groups, dc,
+                           c->NBX, exposure, c->d_depth, c->d_r, c->d_g, c->d_b, c->d_rgba,
+                           c->d_ids, c->stream));
   return 0;
 }
 
@@ -324,27 +417,24 @@ int apply_shadow_clear(axr_ctx* c, int i) {
   return 0;
 }
 
-// After a synchronising readback: a bin capacity overflow makes the frame
-// incomplete, so report it and grow the pool for the frames that follow.
+// After a synchronising readback.  Spilled entries were still drawn (the
+// buckets grow for later passes); entries lost because the spill list was
+// full make the frame incomplete: report it and grow the list.
 int check_overflow(axr_ctx* c) {
   uint32_t h[kCounters];
   HIPCHK(hipMemcpy(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost));
-  if (h[AXR_CTR_BIN_OVERFLOW] > c->bin_cap) {
-    const uint32_t need = h[AXR_CTR_BIN_OVERFLOW];
-    dfree(c->d_bent);
-    c->bin_cap = need + need / 2;
-    int rc = dalloc(&c->d_bent, c->bin_cap);
-    if (rc) return rc;
-    return fail(AXR_ECAP, "screen bins needed %u entries; pool grown, redraw the frame", need);
-  }
-  if (h[AXR_CTR_SH_OVERFLOW] > c->sbin_cap) {
-    const uint32_t need = h[AXR_CTR_SH_OVERFLOW];
-    dfree(c->d_sent);
-    c->sbin_cap = need + need / 2;
-    int rc = dalloc(&c->d_sent, c->sbin_cap);
-    if (rc) return rc;
-    return fail(AXR_ECAP, "shadow bins needed %u entries; pool grown, redraw the map", need);
-  }
+  const uint32_t msp = h[AXR_CTR_BIN_SPILL], ssp = h[AXR_CTR_SH_SPILL], lost = h[AXR_CTR_LOST];
+  if (!msp && !ssp && !lost) return 0;
+  // A loss is not attributed to a kind: grow both spill lists.
+  bins_grow(c->mbins, msp, lost);
+  bins_grow(c->sbins, ssp, lost);
+  const uint32_t zero[3] = {0, 0, 0};
+  HIPCHK(hipMemcpy(c->d_counters + AXR_CTR_BIN_SPILL, zero, 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_counters + AXR_CTR_SH_SPILL, zero, 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_counters + AXR_CTR_LOST, zero, 4, hipMemcpyHostToDevice));
+  if (lost)
+    return fail(AXR_ECAP, "%u bin entries did not fit the spill list; it has been grown, redraw",
+                lost);
   return 0;
 }
 
@@ -377,22 +467,21 @@ int axr_create(int w, int h, axr_ctx** out) {
   };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(AXR_EHIP, "hipStreamCreate failed"));
-  const size_t nb2 = 2 * (size_t)c->NB;
   if ((rc = dalloc(&c->d_depth, np)) || (rc = dalloc(&c->d_r, np)) ||
       (rc = dalloc(&c->d_g, np)) || (rc = dalloc(&c->d_b, np)) || (rc = dalloc(&c->d_rgba, np)) ||
       (rc = dalloc(&c->d_ids, np)) || (rc = dalloc(&c->d_counters, kCounters)) ||
-      (rc = dalloc(&c->d_bcnt, nb2)) || (rc = dalloc(&c->d_boff, nb2 + 1)) ||
-      (rc = dalloc(&c->d_bcur, nb2)) || (rc = dalloc(&c->d_bent, kDefaultBinCap)) ||
-      (rc = dalloc(&c->d_border, (size_t)c->NB)) ||
       (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)))
     return bail(rc);
-  c->bin_cap = kDefaultBinCap;
   if (launch_fill_u32(c->d_counters, kCounters, 0, c->stream) != hipSuccess ||
-      launch_fill_u32(c->d_bcnt, nb2, 0, c->stream) != hipSuccess ||
-      launch_fill_u32(c->d_boff, nb2 + 1, 0, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(fail(AXR_EHIP, "initial fill failed"));
-  if ((rc = identity_order(c->d_border, c->NB, c->stream))) return bail(rc);
+  c->mbins.nseg = 2;
+  c->mbins.want0 = fit_cap(kMainCap0, (size_t)c->NB, 2);
+  c->mbins.want1 = fit_cap(kMainCap1, (size_t)c->NB, 2);
+  c->mbins.want_spill = kSpillCap;
+  c->sbins.nseg = 1;
+  c->sbins.want_spill = kSpillCap;
+  if ((rc = bins_prepare(c->mbins, c->NB, c->stream))) return bail(rc);
   // Default view: identity basis at the origin, 90 degree fovx.
   const float vm[9] = {0, 0, 1, -1, 0, 0, 0, -1, 0};
   std::memcpy(c->view.vm, vm, sizeof vm);
@@ -421,16 +510,8 @@ int axr_destroy(axr_ctx* c) {
   dfree(c->d_cls);
   dfree(c->d_shrec);
   dfree(c->d_ok);
-  dfree(c->d_bcnt);
-  dfree(c->d_boff);
-  dfree(c->d_bcur);
-  dfree(c->d_bent);
-  dfree(c->d_border);
-  dfree(c->d_sorder);
-  dfree(c->d_scnt);
-  dfree(c->d_soff);
-  dfree(c->d_scur);
-  dfree(c->d_sent);
+  bins_free(c->mbins);
+  bins_free(c->sbins);
   dfree(c->d_depth);
   dfree(c->d_r);
   dfree(c->d_g);
@@ -602,7 +683,7 @@ int axr_set_clear_color(axr_ctx* c, const uint16_t rgb[3]) {
 int axr_clear_zbuffer(axr_ctx* c) {
   int rc;
   if ((rc = check_ctx(c))) return rc;
-  // The bins of the next drawAll start empty (the scan zeroes the counts),
+  // The bins of the next drawAll start empty (the order pass zeroes them),
   // and the per-bin walk starts from the clear key: nothing to launch.
   c->state = ST_CLEARED;
   return AXR_OK;
@@ -665,21 +746,17 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
     if ((rc = upload_groups(c))) return rc;
   }
   if (c->T > 0) {
+    if ((rc = bins_prepare(c->mbins, c->NB, c->stream))) return rc;
     {
       Timed tm(c, "setup_bin");
       HIPCHK(launch_setup_bin(c->d_vpos, c->d_vattr, c->d_groups, dc, c->d_rec, c->d_cls,
-                              c->d_bcnt, c->NBX, c->NB, c->stream));
+                              c->mbins.B, c->NBX, c->stream));
     }
     {
-      Timed tm(c, "bin_scan");
-      HIPCHK(launch_scan(c->d_bcnt, c->d_boff, c->d_bcur, c->NB, 2, c->bin_cap,
-                         c->d_counters + AXR_CTR_BIN_TOTAL, c->d_counters + AXR_CTR_BIN_OVERFLOW,
-                         c->d_border, c->stream));
-    }
-    {
-      Timed tm(c, "bin_fill");
-      HIPCHK(launch_bin_fill(c->d_rec, c->d_cls, c->T, c->W, c->H, c->WC, c->HC, c->NBX, c->NB,
-                             c->d_bcur, c->d_bent, c->bin_cap, c->stream));
+      Timed tm(c, "bin_order");
+      HIPCHK(launch_bin_order(c->mbins.B, 2, c->d_counters + AXR_CTR_BIN_TOTAL,
+                              c->d_counters + AXR_CTR_BIN_SPILL, c->d_counters + AXR_CTR_LOST,
+                              c->stream));
     }
   }
   c->last = dc;
@@ -707,21 +784,11 @@ int axr_add_shadow_map(axr_ctx* c, int i, int size, float scale, float amb) {
   s.scale = scale;
   s.cam.scale = scale;
   if (amb >= 0) c->amb = amb;
-  if (s.nb > c->snb_cap) {
-    dfree(c->d_scnt);
-    dfree(c->d_soff);
-    dfree(c->d_scur);
-    dfree(c->d_sorder);
-    if ((rc = dalloc(&c->d_scnt, s.nb)) || (rc = dalloc(&c->d_soff, s.nb + 1)) ||
-        (rc = dalloc(&c->d_scur, s.nb)) || (rc = dalloc(&c->d_sorder, s.nb)))
-      return rc;
-    HIPCHK(launch_fill_u32(c->d_scnt, s.nb, 0, c->stream));
-    if ((rc = identity_order(c->d_sorder, s.nb, c->stream))) return rc;
-    c->snb_cap = s.nb;
-  }
-  if (!c->d_sent) {
-    if ((rc = dalloc(&c->d_sent, kDefaultBinCap))) return rc;
-    c->sbin_cap = kDefaultBinCap;
+  // Shadow buckets are sized for the largest map; a larger map re-fits them.
+  if (s.nb > c->sbins.nb_alloc) {
+    const uint32_t cap = fit_cap(kShadowCap, (size_t)s.nb, 1);
+    if (!c->sbins.want0 || cap < c->sbins.want0) c->sbins.want0 = cap;
+    if ((rc = bins_prepare(c->sbins, s.nb, c->stream))) return rc;
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   return AXR_OK;
@@ -763,27 +830,24 @@ int axr_shadow_map(axr_ctx* c, int i, const uint8_t* cast, int ngroups, float bi
     if (cast[g]) m.bits[g >> 5] |= 1u << (g & 31);
   if (c->T == 0) return AXR_OK;
   HostShadow& s = c->sm[i];
+  // The previous shadow pass is complete (its map written): its bins may go.
+  if ((rc = bins_prepare(c->sbins, s.nb, c->stream))) return rc;
   {
     Timed tm(c, "shadow_setup_bin");
     HIPCHK(launch_shadow_setup_bin(c->d_vpos, c->d_groups, (uint32_t)c->groups.size(), c->T, m,
-                                   s.cam, s.dim, bias, c->d_shrec, c->d_ok, c->d_scnt, s.nbx,
+                                   s.cam, s.dim, bias, c->d_shrec, c->d_ok, c->sbins.B, s.nbx,
                                    c->stream));
   }
   {
-    Timed tm(c, "shadow_scan");
-    HIPCHK(launch_scan(c->d_scnt, c->d_soff, c->d_scur, s.nb, 1, c->sbin_cap,
-                       c->d_counters + AXR_CTR_SH_TOTAL, c->d_counters + AXR_CTR_SH_OVERFLOW,
-                       c->d_sorder, c->stream));
-  }
-  {
-    Timed tm(c, "shadow_fill");
-    HIPCHK(launch_shadow_fill(c->d_shrec, c->d_ok, c->T, s.dim, s.nbx, c->d_scur, c->d_sent,
-                              c->sbin_cap, c->stream));
+    Timed tm(c, "shadow_order");
+    HIPCHK(launch_bin_order(c->sbins.B, 1, c->d_counters + AXR_CTR_SH_TOTAL,
+                            c->d_counters + AXR_CTR_SH_SPILL, c->d_counters + AXR_CTR_LOST,
+                            c->stream));
   }
   {
     Timed tm(c, "shadow_raster");
-    HIPCHK(launch_shadow_bins(c->d_shrec, c->d_soff, c->d_sent, c->sbin_cap, s.d_map, s.dim,
-                              s.nbx, s.nb, s.clear_pending ? 0 : 1, c->d_sorder, c->stream));
+    HIPCHK(launch_shadow_bins(c->d_shrec, c->sbins.B, s.d_map, s.dim, s.nbx,
+                              s.clear_pending ? 0 : 1, c->stream));
   }
   s.clear_pending = false;
   return AXR_OK;
@@ -898,7 +962,7 @@ int axr_sync(axr_ctx* c) {
   return AXR_OK;
 }
 
-int axr_get_stats(axr_ctx* c, uint32_t out[5]) {
+int axr_get_stats(axr_ctx* c, uint32_t out[6]) {
   int rc;
   if ((rc = check_ctx(c))) return rc;
   if (!out) return fail(AXR_EINVAL, "null output");
@@ -910,7 +974,22 @@ int axr_get_stats(axr_ctx* c, uint32_t out[5]) {
   out[1] = c->T ? h[AXR_CTR_STATS + 1] : 0;
   out[2] = c->T ? h[AXR_CTR_STATS + 2] : 0;
   out[3] = h[AXR_CTR_BIN_TOTAL];
-  out[4] = h[AXR_CTR_BIN_OVERFLOW] | h[AXR_CTR_SH_OVERFLOW];
+  out[4] = h[AXR_CTR_LOST];
+  out[5] = h[AXR_CTR_SH_TOTAL];
+  return AXR_OK;
+}
+
+int axr_set_bin_capacity(axr_ctx* c, uint32_t small, uint32_t large, uint32_t shadow,
+                         uint32_t spill) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (small) c->mbins.want0 = small;
+  if (large) c->mbins.want1 = large;
+  if (shadow) c->sbins.want0 = shadow;
+  if (spill) {
+    c->mbins.want_spill = spill;
+    c->sbins.want_spill = spill;
+  }
   return AXR_OK;
 }
 

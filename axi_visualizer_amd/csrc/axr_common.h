@@ -93,12 +93,64 @@ AXR_HD bool cast_bit(const AxrCastMask& m, int g) {
 #endif
 
 // Device counter block.
-#define AXR_CTR_BIN_TOTAL 0     // bin entries of the last drawAll
-#define AXR_CTR_BIN_OVERFLOW 1  // entries needed when above capacity, else 0
-#define AXR_CTR_SH_TOTAL 2      // the same for the last shadow pass
-#define AXR_CTR_SH_OVERFLOW 3
-#define AXR_CTR_STATS 4         // 4 words filled on request: small, large, casters, -
+#define AXR_CTR_BIN_TOTAL 0  // bin entries of the last drawAll
+#define AXR_CTR_BIN_SPILL 1  // most screen-bin entries spilled by one pass since the last check
+#define AXR_CTR_SH_TOTAL 2   // bin entries of the last shadow pass
+#define AXR_CTR_SH_SPILL 3   // the same as BIN_SPILL for shadow bins
+#define AXR_CTR_STATS 4      // 3 words filled on request: small, large, casters
+#define AXR_CTR_LOST 7       // entries dropped since the last check (spill list full)
 #define AXR_CTR_WORDS 8
+
+// Bin storage of one pass kind (screen or shadow).  A bin slot s is
+// seg * nb + bin; segment 0 holds small triangles (or casters), segment 1
+// the large triangles' 16-px tiles.  Setup appends straight into fixed
+// capacity buckets (cap[seg] entries per slot); entries past a bucket's
+// capacity go to the spill list as (slot, entry) pairs, which the raster
+// reads only for slots whose count exceeds the capacity.
+struct AxrBins {
+  uint32_t* cnt;        // [nseg*nb] append cursors (zeroed by the order pass)
+  uint32_t* num;        // [nseg*nb] entries of the current pass
+  uint32_t* ent;        // bucket storage, segment 0 then segment 1
+  uint32_t* order;      // [nb] heavy-first launch order
+  uint2* spill;         // [spill_cap] (slot, entry)
+  uint32_t* spill_ctr;  // [2]: append cursor, entries of the current pass
+  uint32_t cap0, cap1, spill_cap;
+  int nb;
+};
+
+AXR_HD size_t bin_base(const AxrBins& B, uint32_t s) {
+  return s < (uint32_t)B.nb ? (size_t)s * B.cap0
+                            : (size_t)B.nb * B.cap0 + (size_t)(s - (uint32_t)B.nb) * B.cap1;
+}
+
+// Entry v at append position pos of slot s.
+__device__ __forceinline__ void bin_store(const AxrBins& B, uint32_t s, uint32_t pos, uint32_t v) {
+  const uint32_t cap = s < (uint32_t)B.nb ? B.cap0 : B.cap1;
+  if (pos < cap) {
+    B.ent[bin_base(B, s) + pos] = v;
+  } else {
+    const uint32_t q = atomicAdd(&B.spill_ctr[0], 1u);
+    if (q < B.spill_cap) B.spill[q] = make_uint2(s, v);
+  }
+}
+
+// f(entry) for the n entries of slot s, strided over the block's threads:
+// the bucket, then -- only when it overflowed -- the slot's spilled entries.
+template <class F>
+__device__ __forceinline__ void for_bin_entries(const AxrBins& B, uint32_t s, uint32_t n, int tid,
+                                                int nthr, F&& f) {
+  const uint32_t cap = s < (uint32_t)B.nb ? B.cap0 : B.cap1;
+  const uint32_t* __restrict__ ent = B.ent + bin_base(B, s);
+  const uint32_t nb = n < cap ? n : cap;
+  for (uint32_t e = (uint32_t)tid; e < nb; e += (uint32_t)nthr) f(ent[e]);
+  if (n > cap) {
+    const uint32_t ns = B.spill_ctr[1];
+    for (uint32_t q = (uint32_t)tid; q < ns; q += (uint32_t)nthr) {
+      const uint2 p = B.spill[q];
+      if (p.x == s) f(p.y);
+    }
+  }
+}
 
 // Setup output of one visible triangle (indexed by its global index).
 struct AxrTriRec {

@@ -8,35 +8,26 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 hipError_t launch_setup_bin(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                             const AxrDrawConst& dc, AxrTriRec* rec, uint8_t* cls,
-                            uint32_t* cnt, int NBX, int NB, hipStream_t s);
-// Offsets of nbins * nseg counts, and the bins' heavy-first launch order.
-hipError_t launch_scan(uint32_t* cnt, uint32_t* off, uint32_t* cur, int nbins, int nseg,
-                       uint32_t cap, uint32_t* ctr_total, uint32_t* ctr_ovf, uint32_t* order,
-                       hipStream_t s);
-hipError_t launch_bin_fill(const AxrTriRec* rec, const uint8_t* cls, uint32_t T, int W, int H,
-                           int WC, int HC, int NBX, int NB, uint32_t* cur, uint32_t* ent,
-                           uint32_t cap, hipStream_t s);
+                            const AxrBins& B, int NBX, hipStream_t s);
+// Counts of the pass, heavy-first order, cursor reset (nseg segments).
+hipError_t launch_bin_order(const AxrBins& B, int nseg, uint32_t* ctr_total,
+                            uint32_t* ctr_spill, uint32_t* ctr_lost, hipStream_t s);
 hipError_t launch_shadow_setup_bin(const float4* vpos, const AxrGroupDev* groups,
                                    uint32_t ngroups, uint32_t T, const AxrCastMask& cast,
                                    const AxrCam& cam, int dim, float bias, AxrShRec* rec,
-                                   uint8_t* ok, uint32_t* cnt, int NBX, hipStream_t s);
-hipError_t launch_shadow_fill(const AxrShRec* rec, const uint8_t* ok, uint32_t T, int dim,
-                              int NBX, uint32_t* cur, uint32_t* ent, uint32_t cap,
-                              hipStream_t s);
-hipError_t launch_shadow_bins(const AxrShRec* rec, const uint32_t* off, const uint32_t* ent,
-                              uint32_t cap, uint32_t* map, int dim, int NBX, int NB, int from_map,
-                              const uint32_t* order, hipStream_t s);
+                                   uint8_t* ok, const AxrBins& B, int NBX, hipStream_t s);
+hipError_t launch_shadow_bins(const AxrShRec* rec, const AxrBins& B, uint32_t* map, int dim,
+                              int NBX, int from_map, hipStream_t s);
 hipError_t launch_stats(const uint8_t* cls, const uint8_t* ok, uint32_t T, uint32_t* out,
                         hipStream_t s);
 
 // Bin raster modes: HDR u16 planes, gamma fused into RGBA8, or winner ids.
 enum ShadeMode { SHADE_HDR = 0, SHADE_LDR8 = 1, SHADE_IDS = 2 };
-hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const uint32_t* off,
-                             const uint32_t* ent, uint32_t cap, int empty, const float4* vpos,
-                             const float4* vattr, const AxrGroupDev* groups,
-                             const AxrDrawConst& dc, int NBX, int NB, const uint32_t* order,
-                             float exposure, float* depth, uint16_t* r, uint16_t* g, uint16_t* b,
-                             uint32_t* rgba, uint32_t* ids, hipStream_t s);
+hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const AxrBins& B, int empty,
+                             const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
+                             const AxrDrawConst& dc, int NBX, float exposure, float* depth,
+                             uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
+                             uint32_t* ids, hipStream_t s);
 hipError_t launch_gamma_planes(uint16_t* r, uint16_t* g, uint16_t* b, size_t n, float ex,
                                hipStream_t s);
 hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, uint16_t* b,

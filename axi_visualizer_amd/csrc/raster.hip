@@ -1,22 +1,23 @@
 // raster.hip -- setup and binning of the gfx950 raster path, and the
 // shadow-map depth pass.
 //
-// Main pass (drawAll), three launches:
+// Main pass (drawAll), two launches:
 //   k_setup_bin  triangle setup (a5) of every drawn triangle; the setup
-//                record is stored by global index; small triangles count the
-//                32x32 bins their pixel rectangle touches, large triangles the
-//                bins of their coarse 16-px tiles (a8, Pipe/coarse.c)
-//   k_scan       bin offsets; zeroes the counts for the next frame
-//   k_bin_fill   writes bin entries (same bins, same order of rounds)
+//                record is stored by global index; small triangles are
+//                appended to the 32x32 bins their pixel rectangle touches,
+//                large triangles to the bins of their coarse 16-px tiles
+//                (a8, Pipe/coarse.c)
+//   k_bin_order  entry counts of the pass, heavy-first bin order; resets
+//                the append cursors for the next pass
 // The per-bin depth walk and shading run later in one launch (shade.hip).
 //
-// Shadow pass (shadowMap), four launches: k_shadow_setup_bin, k_scan,
-// k_shadow_fill and k_shadow_bins, which walks each bin's casters (a11,
-// Shaders/drawmin.c) with the bin's texels in LDS and writes them once.
+// Shadow pass (shadowMap), three launches: k_shadow_setup_bin, k_bin_order
+// and k_shadow_bins, which walks each bin's casters (a11, Shaders/drawmin.c)
+// with the bin's texels in LDS and writes them once.
 //
-// Bin counting aggregates within a wave: lanes of one wave come from one
-// mesh region and mostly share a bin, so one atomic serves all of them.
-// Depth minima live in LDS; no global atomics touch pixels or texels.
+// Appends aggregate within a wave: lanes of one wave come from one mesh
+// region and mostly share a bin, so one atomic serves all of them.  Depth
+// minima live in LDS; no global atomics touch pixels or texels.
 #include "axr_launch.h"
 #include "axr_tri.h"
 
@@ -99,15 +100,15 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
-// K1: setup + bin counts.  cnt[0, NB): small entries per bin; cnt[NB, 2NB):
-// large (coarse tile) entries per bin.
+// K1: setup + bin appends.  Slot b: small triangles (global index); slot
+// NB + b: large triangles' coarse tiles, (global index << 2) | the 16-px
+// tile's quadrant in bin b.
 __global__ __launch_bounds__(kBlock) void k_setup_bin(const float4* __restrict__ vpos,
                                                       const float4* __restrict__ vattr,
                                                       const AxrGroupDev* __restrict__ groups,
                                                       AxrDrawConst dc, AxrTriRec* __restrict__ rec,
-                                                      uint8_t* __restrict__ cls_out,
-                                                      uint32_t* __restrict__ cnt, int NBX,
-                                                      int NB) {
+                                                      uint8_t* __restrict__ cls_out, AxrBins B,
+                                                      int NBX) {
   const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
   int cls = 0;
   AxrTriRec r;
@@ -128,46 +129,16 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(const float4* __restrict__
   for (int k = 0;; k++) {
     const bool act = k < br.n;
     if (__ballot(act) == 0) break;
-    wave_bin_add(cnt, act, act ? rect_bin(br, k, NBX) : 0u);
+    const uint32_t slot = act ? rect_bin(br, k, NBX) : 0u;
+    const uint32_t pos = wave_bin_add(B.cnt, act, slot);
+    if (act) bin_store(B, slot, pos, gi);
   }
   if (cls == 2) {
     coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
       uint32_t bin, sub;
       tile_to_bin(t, dc.WC, NBX, bin, sub);
-      atomicAdd(&cnt[NB + bin], 1u);
-    });
-  }
-}
-
-// K3: bin entries.  Small: global index; large: (global index << 2) | the
-// 16-px tile's quadrant in its bin.
-__global__ __launch_bounds__(kBlock) void k_bin_fill(const AxrTriRec* __restrict__ rec,
-                                                     const uint8_t* __restrict__ cls_in,
-                                                     uint32_t T, int W, int H, int WC, int HC,
-                                                     int NBX, int NB, uint32_t* __restrict__ cur,
-                                                     uint32_t* __restrict__ ent, uint32_t cap) {
-  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
-  const int cls = gi < T ? cls_in[gi] : 0;
-  AxrTriRec r;
-  if (cls) r = rec[gi];
-  BinRect br = {0, 0, 1, 0};
-  if (cls == 1) {
-    int x0, y0, x1, y1;
-    small_bounds(r.px, r.py, x0, y0, x1, y1);
-    br = bin_rect(x0, y0, x1, y1, W, H);
-  }
-  for (int k = 0;; k++) {
-    const bool act = k < br.n;
-    if (__ballot(act) == 0) break;
-    const uint32_t pos = wave_bin_add(cur, act, act ? rect_bin(br, k, NBX) : 0u);
-    if (act && pos < cap) ent[pos] = gi;
-  }
-  if (cls == 2) {
-    coarse_walk(r.px, r.py, WC, HC, [&](int t) {
-      uint32_t bin, sub;
-      tile_to_bin(t, WC, NBX, bin, sub);
-      const uint32_t pos = atomicAdd(&cur[NB + bin], 1u);
-      if (pos < cap) ent[pos] = (gi << 2) | sub;
+      const uint32_t slot = (uint32_t)B.nb + bin;
+      bin_store(B, slot, atomicAdd(&B.cnt[slot], 1u), (gi << 2) | sub);
     });
   }
 }
@@ -181,68 +152,38 @@ __device__ __forceinline__ int order_class(uint32_t w) {
   return 2 * l + half + 2;
 }
 
-// K2: exclusive scan of n = nbins * nseg bin counts (one block).  off[0..n],
-// cur = off, counts zeroed for the next frame; total and overflow into ctr;
-// order[0..nbins) = bins by descending weight.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt,
-                                               uint32_t* __restrict__ off,
-                                               uint32_t* __restrict__ cur, int nbins, int nseg,
-                                               uint32_t cap, uint32_t* __restrict__ ctr_total,
-                                               uint32_t* __restrict__ ctr_ovf,
-                                               uint32_t* __restrict__ order) {
-  const int n = nbins * nseg;
-  __shared__ uint32_t wsum[16];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
-  const int per = (n + 1023) / 1024;
-  const int b0 = imin(n, tid * per), b1 = imin(n, b0 + per);
-  uint32_t s = 0;
-  for (int i = b0; i < b1; i++) s += cnt[i];
-  uint32_t v = s;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = __shfl_up(v, o);
-    if (lane >= o) v += t;
-  }
-  if (lane == 63) wsum[w] = v;
-  __syncthreads();
-  if (w == 0) {
-    uint32_t x = lane < 16 ? wsum[lane] : 0u;
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-      const uint32_t t = __shfl_up(x, o);
-      if (lane >= o) x += t;
-    }
-    if (lane < 16) wsum[lane] = x;
-  }
-  __syncthreads();
-  uint32_t run = v - s + (w > 0 ? wsum[w - 1] : 0u);
-  for (int i = b0; i < b1; i++) {
-    const uint32_t c = cnt[i];
-    off[i] = run;
-    cur[i] = run;
-    run += c;
-    cnt[i] = 0;
-  }
-  if (tid == 1023) {
-    const uint32_t total = wsum[15];
-    off[n] = total;
-    *ctr_total = total;
-    *ctr_ovf = total > cap ? total : 0u;
-  }
-  // Heavy-first launch order of the nbins bins (weight = entries over the
-  // nseg segments), so the longest bins start first and the short ones
-  // fill the tail.  Classes are half-octaves of the weight; order within a
-  // class is arbitrary (every bin is computed by one block, so the result
-  // does not depend on it).
+// K2 (one block): num = cnt, cnt = 0 for the next pass; the pass's entry
+// total, spill and loss into the counter block; bins in heavy-first launch
+// order (weight = small entries + 16 x large-tile entries), so the longest
+// bins start first and short ones fill the tail.  Order within a weight
+// class is arbitrary: each bin is computed by one block, so results do not
+// depend on it.
+__global__ __launch_bounds__(1024) void k_bin_order(AxrBins B, int nseg,
+                                                    uint32_t* __restrict__ ctr_total,
+                                                    uint32_t* __restrict__ ctr_spill,
+                                                    uint32_t* __restrict__ ctr_lost) {
   __shared__ uint32_t hist[kOrderClasses];
+  __shared__ uint32_t total;
+  const int tid = threadIdx.x;
   for (int i = tid; i < kOrderClasses; i += 1024) hist[i] = 0;
-  __syncthreads();  // also publishes off[] to the whole block
-  for (int i = tid; i < nbins; i += 1024) {
-    uint32_t wgt = 0;
-    for (int sg = 0; sg < nseg; sg++) wgt += off[sg * nbins + i + 1] - off[sg * nbins + i];
-    atomicAdd(&hist[order_class(wgt)], 1u);
+  if (tid == 0) total = 0;
+  __syncthreads();
+  uint32_t tot = 0;
+  for (int b = tid; b < B.nb; b += 1024) {
+    uint32_t w = 0;
+    for (int sg = 0; sg < nseg; sg++) {
+      const uint32_t slot = (uint32_t)(sg * B.nb + b);
+      const uint32_t c = B.cnt[slot];
+      B.num[slot] = c;
+      B.cnt[slot] = 0;
+      tot += c;
+      w += sg ? 16u * c : c;
+    }
+    atomicAdd(&hist[order_class(w)], 1u);
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+  if ((tid & 63) == 0) atomicAdd(&total, tot);
   __syncthreads();
   if (tid == 0) {
     uint32_t run = 0;
@@ -251,20 +192,29 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt,
       hist[k] = run;
       run += c;
     }
+    *ctr_total = total;
+    const uint32_t used = B.spill_ctr[0];
+    B.spill_ctr[1] = used < B.spill_cap ? used : B.spill_cap;
+    B.spill_ctr[0] = 0;
+    if (used) atomicMax(ctr_spill, used);
+    if (used > B.spill_cap) atomicAdd(ctr_lost, used - B.spill_cap);
   }
   __syncthreads();
-  for (int i = tid; i < nbins; i += 1024) {
-    uint32_t wgt = 0;
-    for (int sg = 0; sg < nseg; sg++) wgt += off[sg * nbins + i + 1] - off[sg * nbins + i];
-    order[atomicAdd(&hist[order_class(wgt)], 1u)] = (uint32_t)i;
+  for (int b = tid; b < B.nb; b += 1024) {
+    uint32_t w = 0;
+    for (int sg = 0; sg < nseg; sg++) {
+      const uint32_t c = B.num[sg * B.nb + b];  // written by this thread above
+      w += sg ? 16u * c : c;
+    }
+    B.order[atomicAdd(&hist[order_class(w)], 1u)] = (uint32_t)b;
   }
 }
 
-// S1: ortho setup of every casting group + bin counts.
+// S1: ortho setup of every casting group + bin appends (global index).
 __global__ __launch_bounds__(kBlock) void k_shadow_setup_bin(
     const float4* __restrict__ vpos, const AxrGroupDev* __restrict__ groups, uint32_t ngroups,
     uint32_t T, AxrCastMask cast, AxrCam cam, int dim, float bias, AxrShRec* __restrict__ rec,
-    uint8_t* __restrict__ ok_out, uint32_t* __restrict__ cnt, int NBX) {
+    uint8_t* __restrict__ ok_out, AxrBins B, int NBX) {
   const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
   bool ok = false;
   AxrShRec r;
@@ -283,45 +233,22 @@ __global__ __launch_bounds__(kBlock) void k_shadow_setup_bin(
   for (int k = 0;; k++) {
     const bool act = k < br.n;
     if (__ballot(act) == 0) break;
-    wave_bin_add(cnt, act, act ? rect_bin(br, k, NBX) : 0u);
+    const uint32_t slot = act ? rect_bin(br, k, NBX) : 0u;
+    const uint32_t pos = wave_bin_add(B.cnt, act, slot);
+    if (act) bin_store(B, slot, pos, gi);
   }
 }
 
-// S3: shadow bin entries (global index of the caster).
-__global__ __launch_bounds__(kBlock) void k_shadow_fill(const AxrShRec* __restrict__ rec,
-                                                        const uint8_t* __restrict__ ok_in,
-                                                        uint32_t T, int dim, int NBX,
-                                                        uint32_t* __restrict__ cur,
-                                                        uint32_t* __restrict__ ent, uint32_t cap) {
-  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = gi < T && ok_in[gi];
-  BinRect br = {0, 0, 1, 0};
-  if (ok) {
-    const AxrShRec r = rec[gi];
-    int x0, y0, x1, y1;
-    shadow_bounds(r, x0, y0, x1, y1);
-    br = bin_rect(x0, y0, x1, y1, dim, dim);
-  }
-  for (int k = 0;; k++) {
-    const bool act = k < br.n;
-    if (__ballot(act) == 0) break;
-    const uint32_t pos = wave_bin_add(cur, act, act ? rect_bin(br, k, NBX) : 0u);
-    if (act && pos < cap) ent[pos] = gi;
-  }
-}
-
-// S4: one block per 32x32 texel bin; minimum depth keys in LDS, written
-// once.  The bin starts from the map (from_map) or from the clear depth.
-__global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(const AxrShRec* __restrict__ rec,
-                                                        const uint32_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ ent,
-                                                        uint32_t cap, uint32_t* __restrict__ map,
-                                                        int dim, int NBX, int from_map,
-                                                        const uint32_t* __restrict__ order) {
+// S3: one block per 32x32 texel bin, heavy bins first; minimum depth keys
+// in LDS, written once.  The bin starts from the map (from_map) or from the
+// clear depth.
+__global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(
+    const AxrShRec* __restrict__ rec, AxrBins B, uint32_t* __restrict__ map, int dim, int NBX,
+    int from_map) {
   __shared__ uint32_t lk[AXR_BIN * AXR_BIN];
-  const int b = (int)order[blockIdx.x];
-  const uint32_t s0 = imin(off[b], cap), s1 = imin(off[b + 1], cap);
-  if (from_map && s0 >= s1) return;
+  const int b = (int)B.order[blockIdx.x];
+  const uint32_t n = B.num[b];
+  if (from_map && n == 0) return;
   const int x0 = (b % NBX) << AXR_BIN_SHIFT, y0 = (b / NBX) << AXR_BIN_SHIFT;
   const int xe = imin(x0 + AXR_BIN, dim), ye = imin(y0 + AXR_BIN, dim);
   const uint32_t clear = ordf(AXR_DEPTH_CLEAR);
@@ -330,14 +257,14 @@ __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(const AxrShR
     lk[i] = (from_map && x < xe && y < ye) ? map[(size_t)y * dim + x] : clear;
   }
   __syncthreads();
-  for (uint32_t e = s0 + threadIdx.x; e < s1; e += blockDim.x) {
-    const AxrShRec r = rec[ent[e]];
+  for_bin_entries(B, (uint32_t)b, n, (int)threadIdx.x, (int)blockDim.x, [&](uint32_t gi) {
+    const AxrShRec r = rec[gi];
     shadow_walk(r, dim, x0, y0, xe, ye, [&](int ax, int cy, float tz) {
       if (tz != tz) return;
       const unsigned lc = (unsigned)(((cy - y0) << AXR_BIN_SHIFT) + (ax - x0));
       if (lc < (unsigned)(AXR_BIN * AXR_BIN)) atomicMin(&lk[lc], ordf(tz == 0.0f ? 0.0f : tz));
     });
-  }
+  });
   __syncthreads();
   for (int i = threadIdx.x; i < AXR_BIN * AXR_BIN; i += blockDim.x) {
     const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
@@ -377,60 +304,40 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
 
 hipError_t launch_setup_bin(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                             const AxrDrawConst& dc, AxrTriRec* rec, uint8_t* cls,
-                            uint32_t* cnt, int NBX, int NB, hipStream_t s) {
+                            const AxrBins& B, int NBX, hipStream_t s) {
   if (dc.T == 0) return hipSuccess;
   hipLaunchKernelGGL(k_setup_bin, dim3((dc.T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
-                     vattr, groups, dc, rec, cls, cnt, NBX, NB);
+                     vattr, groups, dc, rec, cls, B, NBX);
   return hipGetLastError();
 }
 
-hipError_t launch_scan(uint32_t* cnt, uint32_t* off, uint32_t* cur, int nbins, int nseg,
-                       uint32_t cap, uint32_t* ctr_total, uint32_t* ctr_ovf, uint32_t* order,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, cnt, off, cur, nbins, nseg, cap,
-                     ctr_total, ctr_ovf, order);
-  return hipGetLastError();
-}
-
-hipError_t launch_bin_fill(const AxrTriRec* rec, const uint8_t* cls, uint32_t T, int W, int H,
-                           int WC, int HC, int NBX, int NB, uint32_t* cur, uint32_t* ent,
-                           uint32_t cap, hipStream_t s) {
-  if (T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bin_fill, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rec, cls, T,
-                     W, H, WC, HC, NBX, NB, cur, ent, cap);
+hipError_t launch_bin_order(const AxrBins& B, int nseg, uint32_t* ctr_total,
+                            uint32_t* ctr_spill, uint32_t* ctr_lost, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_order, dim3(1), dim3(1024), 0, s, B, nseg, ctr_total, ctr_spill,
+                     ctr_lost);
   return hipGetLastError();
 }
 
 hipError_t launch_shadow_setup_bin(const float4* vpos, const AxrGroupDev* groups,
                                    uint32_t ngroups, uint32_t T, const AxrCastMask& cast,
                                    const AxrCam& cam, int dim, float bias, AxrShRec* rec,
-                                   uint8_t* ok, uint32_t* cnt, int NBX, hipStream_t s) {
+                                   uint8_t* ok, const AxrBins& B, int NBX, hipStream_t s) {
   if (T == 0) return hipSuccess;
   hipLaunchKernelGGL(k_shadow_setup_bin, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                     vpos, groups, ngroups, T, cast, cam, dim, bias, rec, ok, cnt, NBX);
+                     vpos, groups, ngroups, T, cast, cam, dim, bias, rec, ok, B, NBX);
   return hipGetLastError();
 }
 
-hipError_t launch_shadow_fill(const AxrShRec* rec, const uint8_t* ok, uint32_t T, int dim,
-                              int NBX, uint32_t* cur, uint32_t* ent, uint32_t cap,
-                              hipStream_t s) {
-  if (T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_shadow_fill, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rec, ok,
-                     T, dim, NBX, cur, ent, cap);
-  return hipGetLastError();
-}
-
-hipError_t launch_shadow_bins(const AxrShRec* rec, const uint32_t* off, const uint32_t* ent,
-                              uint32_t cap, uint32_t* map, int dim, int NBX, int NB, int from_map,
-                              const uint32_t* order, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow_bins, dim3(NB), dim3(AXR_SHADOW_THREADS), 0, s, rec, off, ent, cap, map, dim,
-                     NBX, from_map, order);
+hipError_t launch_shadow_bins(const AxrShRec* rec, const AxrBins& B, uint32_t* map, int dim,
+                              int NBX, int from_map, hipStream_t s) {
+  hipLaunchKernelGGL(k_shadow_bins, dim3(B.nb), dim3(AXR_SHADOW_THREADS), 0, s, rec, B, map, dim,
+                     NBX, from_map);
   return hipGetLastError();
 }
 
 hipError_t launch_stats(const uint8_t* cls, const uint8_t* ok, uint32_t T, uint32_t* out,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, out, (size_t)4, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, out, (size_t)3, 0u);
   if (T)
     hipLaunchKernelGGL(k_stats, dim3(grid_for(T, kBlock, 1024)), dim3(kBlock), 0, s, cls, ok, T,
                        out);

@@ -161,18 +161,16 @@ constexpr int kBinPx = AXR_BIN * AXR_BIN;
 
 template <int MODE>
 __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
-    const AxrTriRec* __restrict__ rec, const uint32_t* __restrict__ off,
-    const uint32_t* __restrict__ ent, uint32_t cap, int empty, const float4* __restrict__ vpos,
+    const AxrTriRec* __restrict__ rec, AxrBins B, int empty, const float4* __restrict__ vpos,
     const float4* __restrict__ vattr, const AxrGroupDev* __restrict__ groups, AxrDrawConst dc,
-    int NBX, int NB, const uint32_t* __restrict__ order, float exposure,
-    float* __restrict__ depth, uint16_t* __restrict__ ro,
+    int NBX, float exposure, float* __restrict__ depth, uint16_t* __restrict__ ro,
     uint16_t* __restrict__ go, uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba,
     uint32_t* __restrict__ ids) {
   __shared__ unsigned long long lk[kBinPx];
   __shared__ uint16_t list[kBinPx];
   __shared__ uint32_t ncov;
-  // Heavy bins first (order from the scan); an empty draw has no order.
-  const int b = empty ? (int)blockIdx.x : (int)order[blockIdx.x];
+  // Heavy bins first (order from the order pass); an empty draw has none.
+  const int b = empty ? (int)blockIdx.x : (int)B.order[blockIdx.x];
   const int bx = b % NBX, by = b / NBX;
   const int x0 = bx << AXR_BIN_SHIFT, y0 = by << AXR_BIN_SHIFT;
   const int xe = imin(x0 + AXR_BIN, dc.W), ye = imin(y0 + AXR_BIN, dc.H);
@@ -182,9 +180,7 @@ __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
   __syncthreads();
   if (!empty) {
     // a7: small triangles, walk clipped to the bin
-    const uint32_t s0 = imin(off[b], cap), s1 = imin(off[b + 1], cap);
-    for (uint32_t e = s0 + tid; e < s1; e += blockDim.x) {
-      const uint32_t gi = ent[e];
+    for_bin_entries(B, (uint32_t)b, B.num[b], tid, (int)blockDim.x, [&](uint32_t gi) {
       const AxrTriRec r = rec[gi];
       const Geom<int> g = small_geom(r.px, r.py, r.z);
       small_walk(g, x0, y0, xe, ye, [&](int ax, int cy, float tz, int half) {
@@ -192,11 +188,10 @@ __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
         const unsigned lc = (unsigned)(((cy - y0) << AXR_BIN_SHIFT) + (ax - x0));
         if (lc < (unsigned)kBinPx) atomicMin(&lk[lc], (unsigned long long)make_key(tz, frag_seq(gi, half)));
       });
-    }
+    });
     // a9: large triangles, one reference 16-px tile per entry
-    const uint32_t l0 = imin(off[NB + b], cap), l1 = imin(off[NB + b + 1], cap);
-    for (uint32_t e = l0 + tid; e < l1; e += blockDim.x) {
-      const uint32_t v = ent[e];
+    const uint32_t ls = (uint32_t)B.nb + (uint32_t)b;
+    for_bin_entries(B, ls, B.num[ls], tid, (int)blockDim.x, [&](uint32_t v) {
       const uint32_t gi = v >> 2, sub = v & 3u;
       const int t16 = ((by << 1) + (int)(sub >> 1)) * dc.WC + (bx << 1) + (int)(sub & 1u);
       const TileBox box = tile_box(t16, dc.W);
@@ -208,7 +203,7 @@ __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
         const unsigned lc = (unsigned)(((cy - y0) << AXR_BIN_SHIFT) + (ax - x0));
         if (lc < (unsigned)kBinPx) atomicMin(&lk[lc], (unsigned long long)make_key(tz, frag_seq(order, half)));
       });
-    }
+    });
   }
   __syncthreads();
   for (int i = tid; i < kBinPx; i += blockDim.x) {
@@ -329,16 +324,15 @@ inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
 
 namespace axr {
 
-hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const uint32_t* off,
-                             const uint32_t* ent, uint32_t cap, int empty, const float4* vpos,
-                             const float4* vattr, const AxrGroupDev* groups,
-                             const AxrDrawConst& dc, int NBX, int NB, const uint32_t* order,
-                             float exposure, float* depth, uint16_t* r, uint16_t* g, uint16_t* b,
-                             uint32_t* rgba, uint32_t* ids, hipStream_t s) {
-  if (NB == 0) return hipSuccess;
-#define AXR_LAUNCH_BIN(M)                                                                      \
-  hipLaunchKernelGGL(k_bin_raster<M>, dim3(NB), dim3(AXR_RASTER_THREADS), 0, s, rec, off, ent, cap, empty,    \
-                     vpos, vattr, groups, dc, NBX, NB, order, exposure, depth, r, g, b, rgba, ids)
+hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const AxrBins& B, int empty,
+                             const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
+                             const AxrDrawConst& dc, int NBX, float exposure, float* depth,
+                             uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
+                             uint32_t* ids, hipStream_t s) {
+  if (B.nb == 0) return hipSuccess;
+#define AXR_LAUNCH_BIN(M)                                                                     \
+  hipLaunchKernelGGL(k_bin_raster<M>, dim3(B.nb), dim3(AXR_RASTER_THREADS), 0, s, rec, B, empty, \
+                     vpos, vattr, groups, dc, NBX, exposure, depth, r, g, b, rgba, ids)
   if (mode == SHADE_LDR8)
     AXR_LAUNCH_BIN(SHADE_LDR8);
   else if (mode == SHADE_HDR)

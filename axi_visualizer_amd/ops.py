@@ -276,10 +276,16 @@ class CLDraw:
         self._chk(self._L.axr_sync(self._ctx), "sync")
 
     def stats(self):
-        s = np.zeros(5, np.uint32)
+        s = np.zeros(6, np.uint32)
         self._chk(self._L.axr_get_stats(self._ctx, ptr(s)), "stats")
         return {"small": int(s[0]), "large": int(s[1]), "casters": int(s[2]),
-                "bin_entries": int(s[3]), "bin_overflow": int(s[4])}
+                "bin_entries": int(s[3]), "bin_overflow": int(s[4]),
+                "shadow_bin_entries": int(s[5])}
+
+    def setBinCapacity(self, small=0, large=0, shadow=0, spill=0):
+        """Bucket capacities (entries per bin) and spill-list size; 0 keeps."""
+        self._chk(self._L.axr_set_bin_capacity(self._ctx, int(small), int(large), int(shadow),
+                                               int(spill)), "setBinCapacity")
 
     def setTiming(self, on: bool):
         self._chk(self._L.axr_set_timing(self._ctx, 1 if on else 0), "setTiming")

@@ -36,11 +36,14 @@ class Renderer:
         self.mask = [g.skip for g in sc.groups]
         self.cast = [g.cast for g in sc.groups]
 
-    def frame(self, gamma=True):
+    def frame(self, gamma=True, after_cascade=None):
+        """One frame; ``after_cascade(i)`` runs after shadow pass i (probes)."""
         sc, d = self.scene, self.draw
         for i, c in enumerate(sc.cascades):
             d.clearShadowMap(i)
             d.shadowMap(i, self.cast, self.shaders, c.bias)
+            if after_cascade is not None:
+                after_cascade(i)
         d.setPos(sc.view.pos)
         d.setVM(sc.view.vm)
         d.clearZBuffer()

@@ -42,8 +42,10 @@ def algorithmic_bytes(sc, st):
     for c in sc.cascades:
         frame += T * 36 + 2 * c.dim * c.dim * 4
     visible = st["small"] + st["large"]
-    casters = st["casters"]
+    # shadow kernels: per-launch means over the cascades of one frame
+    casters = float(np.mean(st["cascade_casters"])) if sc.cascades else 0.0
     entries = st["bin_entries"]
+    sh_entries = float(np.mean(st["cascade_entries"])) if sc.cascades else 0.0
     dmax = max((c.dim for c in sc.cascades), default=0)
     per_kernel = {
         # bin entries and the setup records they name in; RGBA8 + depth out
@@ -51,13 +53,11 @@ def algorithmic_bytes(sc, st):
         # by the visible set)
         "raster_shade": entries * (4 + 36) + W * H * 8 + visible * 132,
         # positions of every triangle (36 B) and its class byte; one 36 B
-        # setup record per visible triangle
-        "setup_bin": T * 37 + visible * 36,
-        "bin_fill": T + visible * 36 + entries * 4,
-        "shadow_setup_bin": T * 37 + casters * 36,
-        "shadow_fill": T + casters * (36 + 4),
+        # setup record per visible triangle; one 4 B entry per bin it touches
+        "setup_bin": T * 37 + visible * 36 + entries * 4,
+        "shadow_setup_bin": T * 37 + casters * 36 + sh_entries * 4,
         # caster records via their bin entries; the map written once
-        "shadow_raster": casters * (36 + 4) + dmax * dmax * 4,
+        "shadow_raster": sh_entries * (36 + 4) + dmax * dmax * 4,
     }
     return frame, per_kernel
 
@@ -132,7 +132,13 @@ def main():
     for _ in range(args.warmup):
         r.frame()
     d.sync()
+    # one probe frame for the per-pass counters (casters and shadow-bin
+    # entries of each cascade, screen-bin entries of the draw)
+    per_cascade = []
+    r.frame(after_cascade=lambda i: per_cascade.append(d.stats()))
     st = d.stats()
+    st["cascade_casters"] = [p["casters"] for p in per_cascade]
+    st["cascade_entries"] = [p["shadow_bin_entries"] for p in per_cascade]
     if st["bin_overflow"]:
         raise SystemExit(f"bin capacity exceeded: {st}")
 

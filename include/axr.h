@@ -118,10 +118,22 @@ int axr_get_shadow_map(axr_ctx* ctx, int i, float* out);
 /* Waits for the context's stream. */
 int axr_sync(axr_ctx* ctx);
 
-/* Counters of the last drawAll: [0] small tris, [1] large tris,
- * [2] shadow casters of the last shadowMap, [3] bin entries, [4] bin
- * overflow (0 if none).  Synchronises. */
-int axr_get_stats(axr_ctx* ctx, uint32_t out[5]);
+/* Counters: [0] small tris and [1] large tris of the last drawAll, [2]
+ * shadow casters of the last shadowMap, [3] screen-bin entries of the last
+ * drawAll, [4] bin entries lost since the last readback (0 unless the spill
+ * list overflowed; the frame is then incomplete), [5] shadow-bin entries of
+ * the last shadowMap.  Synchronises. */
+int axr_get_stats(axr_ctx* ctx, uint32_t out[6]);
+
+/* Bin bucket capacities, in entries per bin: small triangles and large-
+ * triangle tiles per 32x32 screen bin, casters per 32x32 shadow bin; and
+ * the spill list of each pass kind (entries past a full bucket).  0 keeps a
+ * value.  Takes effect at the next pass of each kind; readbacks grow the
+ * buckets after a pass spilled and the spill list after entries were lost.
+ * No reference counterpart (the reference's tile bins hold 128 entries and
+ * overflow silently, Pipe/coarse.c:81-82); used to tune and to test. */
+int axr_set_bin_capacity(axr_ctx* ctx, uint32_t small, uint32_t large, uint32_t shadow,
+                         uint32_t spill);
 
 /* Per-kernel timing with HIP events on the context's stream.
  * axr_set_timing(ctx, 1) starts recording; axr_get_timing copies up to n

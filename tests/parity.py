@@ -8,9 +8,11 @@ RGB_TOL = 1          # per channel, in 8-bit units after gamma
 RGB_MIN_FRAC = 0.999  # fraction of pixels within RGB_TOL
 
 
-def render_gpu(scene, hdr=False):
+def render_gpu(scene, hdr=False, caps=None):
     r = Renderer(scene)
     try:
+        if caps:
+            r.draw.setBinCapacity(**caps)
         r.frame(gamma=not hdr)
         out = {"id": r.draw.getIds()}
         rr, gg, bb, d = r.draw.getFrame()
@@ -53,11 +55,11 @@ def compare(scene, gpu, ref, label=""):
             "exact_rgb": float((diff == 0).mean())}
 
 
-def check_scene(scene, label=None, hdr=False):
+def check_scene(scene, label=None, hdr=False, caps=None):
     label = label or scene.name
     if hdr:
         scene.exposure = 0.0
-    gpu = render_gpu(scene, hdr=hdr)
+    gpu = render_gpu(scene, hdr=hdr, caps=caps)
     ref = pyoracle.render(scene, with_shadows_out=bool(scene.cascades))
     rep = compare(scene, gpu, ref, label)
     print(f"[parity] {label}: {rep}", flush=True)

@@ -4,8 +4,10 @@ import numpy as np
 import pytest
 
 from axi_visualizer_amd import scenes
+from axi_visualizer_amd._native import AxrError
 from axi_visualizer_amd.render import Renderer
 from oracle import pyoracle
+from tests.parity import check_scene, compare
 
 pytestmark = pytest.mark.gpu
 
@@ -70,5 +72,40 @@ def test_frame_twice_gives_identical_pixels():
         r.frame()
         b = r.draw.getRGBA8()
         assert np.array_equal(a, b)
+    finally:
+        r.close()
+
+
+# Tiny bucket capacities push most bin entries through the spill list.
+SPILL_CAPS = dict(small=8, large=2, shadow=8)
+
+
+def test_spilled_bins_match_oracle_spheres():
+    check_scene(small_spheres(), label="spill_spheres", caps=SPILL_CAPS)
+
+
+def test_spilled_bins_match_oracle_large_triangles():
+    sc = scenes.terrain_scene(W=640, H=480, grid=24, cam_dist=18.0, name="spill_closeup")
+    check_scene(sc, caps=SPILL_CAPS)
+
+
+def test_lost_entries_are_reported_and_the_redraw_is_whole():
+    """A spill list too small for the frame: the readback reports the loss
+    (AXR_ECAP), the list grows, and the next frame is complete."""
+    sc = small_spheres()
+    r = Renderer(sc)
+    try:
+        d = r.draw
+        d.setBinCapacity(small=4, large=1, shadow=4, spill=64)
+        r.frame()
+        with pytest.raises(AxrError, match="spill list"):
+            d.getRGBA8()
+        r.frame()
+        gpu = {"id": d.getIds()}
+        rr, gg, bb, dep = d.getFrame()
+        gpu.update(r=rr, g=gg, b=bb, depth=dep, stats=d.stats(),
+                   shadow=[d.getSHM(i) for i in range(len(sc.cascades))])
+        ref = pyoracle.render(sc, with_shadows_out=True)
+        compare(sc, gpu, ref, "redraw_after_loss")
     finally:
         r.close()
