@@ -77,6 +77,15 @@ enum State {
   ST_LDR16       // u16 planes hold gamma'd colour
 };
 
+// A shadowMap call recorded for the next batch of shadow passes.
+struct PendingPass {
+  int map;
+  AxrCastMask cast;
+  float bias;
+  AxrCam cam;    // placement at the time of the call
+  int from_map;  // 0 when the map was cleared before the pass
+};
+
 struct TimingRec {
   const char* name;
   hipEvent_t a, b;
@@ -121,8 +130,10 @@ struct axr_ctx {
   AxrGroupDev* d_groups = nullptr;
   AxrTriRec* d_rec = nullptr;   // [tri_cap] setup records of the last drawAll
   uint8_t* d_cls = nullptr;     // [tri_cap] 0 culled, 1 small, 2 large
-  AxrShRec* d_shrec = nullptr;  // [tri_cap] ortho setup of the last shadow pass
-  uint8_t* d_ok = nullptr;      // [tri_cap] caster kept
+  // [tri_cap] ortho setup and caster flags, one array per pass of a batch
+  AxrShRec* d_shrec[AXR_MAX_SHADOWS] = {};
+  uint8_t* d_ok[AXR_MAX_SHADOWS] = {};
+  const uint8_t* last_ok = nullptr;  // caster flags of the last shadow pass run
 
   BinStore mbins;  // screen bins: slots [0, NB) small, [NB, 2NB) large tiles
   BinStore sbins;  // shadow bins, sized for the largest map
@@ -136,6 +147,7 @@ struct axr_ctx {
   size_t light_cap = 0;
 
   HostShadow sm[AXR_MAX_SHADOWS];
+  std::vector<PendingPass> pend;  // shadow passes not yet launched (one per map)
   AxrCam view{};
   float lint[3] = {1.0f, 1.0f, 1.0f};
   float ldir[3] = {0.0f, -1.0f, 0.0f};
@@ -330,14 +342,18 @@ int reserve_tris(axr_ctx* c, size_t extra) {
   c->tri_cap = cap;
   dfree(c->d_rec);
   dfree(c->d_cls);
-  dfree(c->d_shrec);
-  dfree(c->d_ok);
-  if ((rc = dalloc(&c->d_rec, cap)) || (rc = dalloc(&c->d_cls, cap)) ||
-      (rc = dalloc(&c->d_shrec, cap)) || (rc = dalloc(&c->d_ok, cap)))
-    return rc;
+  for (int k = 0; k < AXR_MAX_SHADOWS; k++) {
+    dfree(c->d_shrec[k]);
+    dfree(c->d_ok[k]);
+  }
+  c->last_ok = nullptr;
+  if ((rc = dalloc(&c->d_rec, cap)) || (rc = dalloc(&c->d_cls, cap))) return rc;
+  for (int k = 0; k < AXR_MAX_SHADOWS; k++)
+    if ((rc = dalloc(&c->d_shrec[k], cap)) || (rc = dalloc(&c->d_ok[k], cap))) return rc;
   // cap is a power of two >= 1024: the byte arrays are whole words
   HIPCHK(launch_fill_u32((uint32_t*)c->d_cls, cap / 4, 0, c->stream));
-  HIPCHK(launch_fill_u32((uint32_t*)c->d_ok, cap / 4, 0, c->stream));
+  for (int k = 0; k < AXR_MAX_SHADOWS; k++)
+    HIPCHK(launch_fill_u32((uint32_t*)c->d_ok[k], cap / 4, 0, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->records_valid = false;
   return 0;
@@ -438,6 +454,68 @@ int check_overflow(axr_ctx* c) {
   return 0;
 }
 
+bool pass_pending(const axr_ctx* c, int i) {
+  for (const auto& p : c->pend)
+    if (p.map == i) return true;
+  return false;
+}
+
+// Launches the pending shadow passes -- with the drawAll's setup when dc
+// is given -- as one setup, one order and one shadow-depth launch.
+int run_setup(axr_ctx* c, const AxrDrawConst* dc_main) {
+  AxrPassSet ps{};
+  int nb_total = 0;
+  ps.n = (int)c->pend.size();
+  for (int k = 0; k < ps.n; k++) {
+    const PendingPass& q = c->pend[(size_t)k];
+    const HostShadow& s = c->sm[q.map];
+    AxrShadowPass& P = ps.p[k];
+    P.cam = q.cam;
+    P.dim = s.dim;
+    P.nbx = s.nbx;
+    P.bias = q.bias;
+    P.slot0 = (uint32_t)nb_total;
+    P.from_map = q.from_map;
+    P.cast = q.cast;
+    P.rec = c->d_shrec[k];
+    P.ok = c->d_ok[k];
+    P.map = s.d_map;
+    nb_total += s.nb;
+  }
+  c->pend.clear();
+  int rc;
+  if (ps.n && (rc = bins_prepare(c->sbins, nb_total, c->stream))) return rc;
+  if (dc_main && (rc = bins_prepare(c->mbins, c->NB, c->stream))) return rc;
+  AxrDrawConst dc{};
+  if (dc_main) {
+    dc = *dc_main;
+  } else {
+    dc.T = c->T;
+    dc.ngroups = (uint32_t)c->groups.size();
+  }
+  {
+    Timed tm(c, dc_main ? "setup_bin" : "shadow_setup_bin");
+    HIPCHK(launch_setup(c->d_vpos, c->d_vattr, c->d_groups, dc, dc_main ? 1 : 0, c->d_rec,
+                        c->d_cls, c->mbins.B, c->NBX, ps, c->sbins.B, c->stream));
+  }
+  {
+    Timed tm(c, dc_main ? "bin_order" : "shadow_order");
+    HIPCHK(launch_bin_order(dc_main ? &c->mbins.B : nullptr, ps.n ? &c->sbins.B : nullptr,
+                            c->d_counters, c->stream));
+  }
+  if (ps.n) {
+    Timed tm(c, "shadow_raster");
+    HIPCHK(launch_shadow_bins(c->sbins.B, ps, c->stream));
+    c->last_ok = c->d_ok[ps.n - 1];
+  }
+  return 0;
+}
+
+int flush_shadows(axr_ctx* c) {
+  if (c->pend.empty()) return 0;
+  return run_setup(c, nullptr);
+}
+
 }  // namespace
 
 extern "C" {
@@ -508,8 +586,10 @@ int axr_destroy(axr_ctx* c) {
   dfree(c->d_groups);
   dfree(c->d_rec);
   dfree(c->d_cls);
-  dfree(c->d_shrec);
-  dfree(c->d_ok);
+  for (int k = 0; k < AXR_MAX_SHADOWS; k++) {
+    dfree(c->d_shrec[k]);
+    dfree(c->d_ok[k]);
+  }
   bins_free(c->mbins);
   bins_free(c->sbins);
   dfree(c->d_depth);
@@ -558,6 +638,8 @@ int axr_add_texture_group(axr_ctx* c, const float* xyz, const float* uv, const f
       return fail(AXR_EINVAL, "texture must be N*N with N a power of two");
   }
   if ((rc = settle(c))) return rc;
+  // pending shadow passes cast the groups that existed when they were made
+  if ((rc = flush_shadows(c))) return rc;
   if ((rc = reserve_tris(c, (size_t)ntri))) return rc;
   const size_t nv = (size_t)3 * ntri;
   std::vector<float4> hp(nv), ha(2 * nv);
@@ -745,20 +827,9 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
     for (int g = 0; g < ngroups; g++) c->groups[g].flags = nf[g];
     if ((rc = upload_groups(c))) return rc;
   }
-  if (c->T > 0) {
-    if ((rc = bins_prepare(c->mbins, c->NB, c->stream))) return rc;
-    {
-      Timed tm(c, "setup_bin");
-      HIPCHK(launch_setup_bin(c->d_vpos, c->d_vattr, c->d_groups, dc, c->d_rec, c->d_cls,
-                              c->mbins.B, c->NBX, c->stream));
-    }
-    {
-      Timed tm(c, "bin_order");
-      HIPCHK(launch_bin_order(c->mbins.B, 2, c->d_counters + AXR_CTR_BIN_TOTAL,
-                              c->d_counters + AXR_CTR_BIN_SPILL, c->d_counters + AXR_CTR_LOST,
-                              c->stream));
-    }
-  }
+  // the draw's setup runs with the pending shadow passes (which also
+  // write any map this draw samples before its shading runs)
+  if (c->T > 0 && (rc = run_setup(c, &dc))) return rc;
   c->last = dc;
   c->records_valid = true;
   c->state = c->T > 0 ? ST_PENDING : ST_CLEARED;
@@ -771,6 +842,7 @@ int axr_add_shadow_map(axr_ctx* c, int i, int size, float scale, float amb) {
   if (i < 0 || i >= AXR_MAX_SHADOWS) return fail(AXR_EINVAL, "shadow map index %d", i);
   if (size < 8 || size > 16384 || (size & 1)) return fail(AXR_EINVAL, "shadow map size %d", size);
   if (bound_to_pending(c, i) && (rc = settle(c))) return rc;
+  if (pass_pending(c, i) && (rc = flush_shadows(c))) return rc;
   HostShadow& s = c->sm[i];
   HIPCHK(hipStreamSynchronize(c->stream));
   dfree(s.d_map);
@@ -784,12 +856,12 @@ int axr_add_shadow_map(axr_ctx* c, int i, int size, float scale, float amb) {
   s.scale = scale;
   s.cam.scale = scale;
   if (amb >= 0) c->amb = amb;
-  // Shadow buckets are sized for the largest map; a larger map re-fits them.
-  if (s.nb > c->sbins.nb_alloc) {
-    const uint32_t cap = fit_cap(kShadowCap, (size_t)s.nb, 1);
-    if (!c->sbins.want0 || cap < c->sbins.want0) c->sbins.want0 = cap;
-    if ((rc = bins_prepare(c->sbins, s.nb, c->stream))) return rc;
-  }
+  // Shadow buckets are sized for every map being in one batch of passes.
+  int nb_all = 0;
+  for (const auto& m : c->sm)
+    if (m.exists) nb_all += m.nb;
+  const uint32_t cap = fit_cap(kShadowCap, (size_t)nb_all, 1);
+  if (!c->sbins.want0 || cap < c->sbins.want0) c->sbins.want0 = cap;
   HIPCHK(hipStreamSynchronize(c->stream));
   return AXR_OK;
 }
@@ -813,6 +885,7 @@ int axr_clear_shadow_map(axr_ctx* c, int i) {
   if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists)
     return fail(AXR_EINVAL, "no shadow map %d", i);
   if (bound_to_pending(c, i) && (rc = settle(c))) return rc;
+  if (pass_pending(c, i) && (rc = flush_shadows(c))) return rc;
   c->sm[i].clear_pending = true;
   return AXR_OK;
 }
@@ -825,30 +898,20 @@ int axr_shadow_map(axr_ctx* c, int i, const uint8_t* cast, int ngroups, float bi
   if (ngroups != (int)c->groups.size() || !cast)
     return fail(AXR_EINVAL, "shadowMap: whichCast must cover all groups");
   if (bound_to_pending(c, i) && (rc = settle(c))) return rc;
+  // a second pass on a map starts from the first one's result
+  if (pass_pending(c, i) && (rc = flush_shadows(c))) return rc;
   AxrCastMask m{};
   for (int g = 0; g < ngroups; g++)
     if (cast[g]) m.bits[g >> 5] |= 1u << (g & 31);
   if (c->T == 0) return AXR_OK;
   HostShadow& s = c->sm[i];
-  // The previous shadow pass is complete (its map written): its bins may go.
-  if ((rc = bins_prepare(c->sbins, s.nb, c->stream))) return rc;
-  {
-    Timed tm(c, "shadow_setup_bin");
-    HIPCHK(launch_shadow_setup_bin(c->d_vpos, c->d_groups, (uint32_t)c->groups.size(), c->T, m,
-                                   s.cam, s.dim, bias, c->d_shrec, c->d_ok, c->sbins.B, s.nbx,
-                                   c->stream));
-  }
-  {
-    Timed tm(c, "shadow_order");
-    HIPCHK(launch_bin_order(c->sbins.B, 1, c->d_counters + AXR_CTR_SH_TOTAL,
-                            c->d_counters + AXR_CTR_SH_SPILL, c->d_counters + AXR_CTR_LOST,
-                            c->stream));
-  }
-  {
-    Timed tm(c, "shadow_raster");
-    HIPCHK(launch_shadow_bins(c->d_shrec, c->sbins.B, s.d_map, s.dim, s.nbx,
-                              s.clear_pending ? 0 : 1, c->stream));
-  }
+  PendingPass q;
+  q.map = i;
+  q.cast = m;
+  q.bias = bias;
+  q.cam = s.cam;
+  q.from_map = s.clear_pending ? 0 : 1;
+  c->pend.push_back(q);
   s.clear_pending = false;
   return AXR_OK;
 }
@@ -885,6 +948,7 @@ int axr_get_frame(axr_ctx* c, uint16_t* r, uint16_t* g, uint16_t* b, float* dept
   if ((rc = check_ctx(c))) return rc;
   const size_t n = (size_t)c->W * c->H;
   if (c->state == ST_UNCLEARED) return fail(AXR_ESTATE, "getFrame before any clearZBuffer");
+  if ((rc = flush_shadows(c))) return rc;
   if (c->state == ST_CLEARED || c->state == ST_PENDING) {
     if ((rc = materialize(c, SHADE_HDR, 1.0f))) return rc;
   } else if (c->state == ST_LDR8) {
@@ -903,6 +967,7 @@ int axr_get_rgba8(axr_ctx* c, uint32_t* rgba) {
   int rc;
   if ((rc = check_ctx(c))) return rc;
   if (!rgba) return fail(AXR_EINVAL, "null output");
+  if ((rc = flush_shadows(c))) return rc;
   const size_t n = (size_t)c->W * c->H;
   if (c->state == ST_LDR8) {
     HIPCHK(hipMemcpyAsync(rgba, c->d_rgba, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -930,6 +995,7 @@ int axr_get_ids(axr_ctx* c, uint32_t* ids) {
   if (c->state == ST_UNCLEARED) return fail(AXR_ESTATE, "getIds before any clearZBuffer");
   if (c->state != ST_CLEARED && !c->records_valid)
     return fail(AXR_ESTATE, "getIds: groups were added since the last drawAll");
+  if ((rc = flush_shadows(c))) return rc;
   const size_t n = (size_t)c->W * c->H;
   // The winners of the current draw (also after it has been shaded).
   if ((rc = bin_raster(c, SHADE_IDS, 1.0f))) return rc;
@@ -943,6 +1009,7 @@ int axr_get_shadow_map(axr_ctx* c, int i, float* out) {
   if ((rc = check_ctx(c))) return rc;
   if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists || !out)
     return fail(AXR_EINVAL, "no shadow map %d", i);
+  if ((rc = flush_shadows(c))) return rc;
   if ((rc = apply_shadow_clear(c, i))) return rc;
   const size_t n = (size_t)c->sm[i].dim * c->sm[i].dim;
   float* d = nullptr;
@@ -958,6 +1025,7 @@ int axr_get_shadow_map(axr_ctx* c, int i, float* out) {
 int axr_sync(axr_ctx* c) {
   int rc;
   if ((rc = check_ctx(c))) return rc;
+  if ((rc = flush_shadows(c))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return AXR_OK;
 }
@@ -966,7 +1034,8 @@ int axr_get_stats(axr_ctx* c, uint32_t out[6]) {
   int rc;
   if ((rc = check_ctx(c))) return rc;
   if (!out) return fail(AXR_EINVAL, "null output");
-  if (c->T) HIPCHK(launch_stats(c->d_cls, c->d_ok, c->T, c->d_counters + AXR_CTR_STATS, c->stream));
+  if ((rc = flush_shadows(c))) return rc;
+  if (c->T) HIPCHK(launch_stats(c->d_cls, c->last_ok, c->T, c->d_counters + AXR_CTR_STATS, c->stream));
   uint32_t h[kCounters];
   HIPCHK(hipMemcpyAsync(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

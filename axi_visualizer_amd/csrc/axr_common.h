@@ -67,9 +67,40 @@ struct AxrDrawConst {
   uint16_t clear_rgb[4];
 };
 
+// Setup output of one visible triangle (indexed by its global index).
+struct AxrTriRec {
+  float px[3], py[3], z[3];
+};
+
+// Ortho setup output of one shadow caster (int coords).
+struct AxrShRec {
+  int x[3], y[3];
+  float z[3];
+};
+
 // Groups casting into one shadow pass (Ops.py:1094-1095 whichCast).
 struct AxrCastMask {
   uint32_t bits[AXR_MAX_GROUPS / 32];
+};
+
+// One shadow pass (shadowMap(i), Ops.py:1089-1141) as the kernels run it.
+// Passes are deferred and run together: one setup launch (sharing the
+// position reads with the drawAll setup when there is one), one order
+// launch and one depth launch over the bins of every pending map.
+struct AxrShadowPass {
+  AxrCam cam;
+  int dim, nbx;
+  float bias;
+  uint32_t slot0;  // first bin slot of this map in the shadow bin store
+  int from_map;    // 0: bins start from the clear depth (map cleared before)
+  AxrCastMask cast;
+  AxrShRec* rec;
+  uint8_t* ok;
+  uint32_t* map;
+};
+struct AxrPassSet {
+  int n;
+  AxrShadowPass p[AXR_MAX_SHADOWS];
 };
 // Bit g of the mask without dynamic indexing of a by-value argument (which
 // would spill the argument to scratch).
@@ -151,17 +182,6 @@ __device__ __forceinline__ void for_bin_entries(const AxrBins& B, uint32_t s, ui
     }
   }
 }
-
-// Setup output of one visible triangle (indexed by its global index).
-struct AxrTriRec {
-  float px[3], py[3], z[3];
-};
-
-// Ortho setup output of one shadow caster (int coords).
-struct AxrShRec {
-  int x[3], y[3];
-  float z[3];
-};
 
 // -------------------------------------------------------------- helpers
 

@@ -6,18 +6,18 @@ namespace axr {
 
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
-hipError_t launch_setup_bin(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
-                            const AxrDrawConst& dc, AxrTriRec* rec, uint8_t* cls,
-                            const AxrBins& B, int NBX, hipStream_t s);
-// Counts of the pass, heavy-first order, cursor reset (nseg segments).
-hipError_t launch_bin_order(const AxrBins& B, int nseg, uint32_t* ctr_total,
-                            uint32_t* ctr_spill, uint32_t* ctr_lost, hipStream_t s);
-hipError_t launch_shadow_setup_bin(const float4* vpos, const AxrGroupDev* groups,
-                                   uint32_t ngroups, uint32_t T, const AxrCastMask& cast,
-                                   const AxrCam& cam, int dim, float bias, AxrShRec* rec,
-                                   uint8_t* ok, const AxrBins& B, int NBX, hipStream_t s);
-hipError_t launch_shadow_bins(const AxrShRec* rec, const AxrBins& B, uint32_t* map, int dim,
-                              int NBX, int from_map, hipStream_t s);
+// Setup + bin appends of the drawAll (do_main) and of the pending shadow
+// passes in ps, one launch.
+hipError_t launch_setup(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
+                        const AxrDrawConst& dc, int do_main, AxrTriRec* rec, uint8_t* cls,
+                        const AxrBins& MB, int NBX, const AxrPassSet& ps, const AxrBins& SB,
+                        hipStream_t s);
+// Counts of the pass, heavy-first order, cursor reset for the screen bins
+// and/or the shadow bins (either may be null), one launch.
+hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins,
+                            uint32_t* ctr, hipStream_t s);
+// Depth walk of every pending map's bins, one launch.
+hipError_t launch_shadow_bins(const AxrBins& B, const AxrPassSet& ps, hipStream_t s);
 hipError_t launch_stats(const uint8_t* cls, const uint8_t* ok, uint32_t T, uint32_t* out,
                         hipStream_t s);
 

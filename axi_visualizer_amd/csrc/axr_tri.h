@@ -121,7 +121,7 @@ __device__ __forceinline__ Attr load_attr(const float4* __restrict__ vpos,
 
 // ------------------------------------------------------------ a5: setup
 // Pipe/trisetup.c:28-80 (+ trisetup_cull.c:49-54).  0 reject, 1 small, 2 large.
-__device__ __forceinline__ int tri_setup(const float4* __restrict__ vpos,
+__device__ __forceinline__ int tri_setup(const float4 (&vp4)[3],
                                          const float4* __restrict__ vattr, uint32_t gi,
                                          const AxrCam& cam, int W, int H, bool cull, float px[3],
                                          float py[3], float zz[3]) {
@@ -131,7 +131,7 @@ __device__ __forceinline__ int tri_setup(const float4* __restrict__ vpos,
   float dd[3], dx[3], dy[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    float4 p = vpos[3 * (size_t)gi + k];
+    const float4 p = vp4[k];
     x[k] = sub(v3(p.x, p.y, p.z), vp);
     dd[k] = dot(x[k], SVd);
     dx[k] = dot(x[k], SVx);
@@ -550,16 +550,15 @@ __device__ __forceinline__ void shadow_bounds(const AxrShRec& r, int& x0, int& y
 }
 
 // Pipe/trisetupOrtho.c:20-59.
-__device__ __forceinline__ bool shadow_setup(const float4* __restrict__ vpos, uint32_t gi,
-                                             const AxrCam& cam, int dim, float bias,
-                                             AxrShRec& r) {
+__device__ __forceinline__ bool shadow_setup(const float4 (&vp4)[3], const AxrCam& cam,
+                                             int dim, float bias, AxrShRec& r) {
   const V3 vp = v3(cam.pos[0], cam.pos[1], cam.pos[2]);
   const V3 SVd = cam_row(cam, 0), SVx = cam_row(cam, 1), SVy = cam_row(cam, 2);
   const float half = (float)(dim / 2);
   bool yes = true;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    float4 p = vpos[3 * (size_t)gi + k];
+    const float4 p = vp4[k];
     V3 x = sub(v3(p.x, p.y, p.z), vp);
     float dd = dot(x, SVd), dx = dot(x, SVx), dy = dot(x, SVy);
     dd += bias;

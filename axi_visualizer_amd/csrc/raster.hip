@@ -1,19 +1,21 @@
 // raster.hip -- setup and binning of the gfx950 raster path, and the
 // shadow-map depth pass.
 //
-// Main pass (drawAll), two launches:
-//   k_setup_bin  triangle setup (a5) of every drawn triangle; the setup
-//                record is stored by global index; small triangles are
-//                appended to the 32x32 bins their pixel rectangle touches,
-//                large triangles to the bins of their coarse 16-px tiles
-//                (a8, Pipe/coarse.c)
-//   k_bin_order  entry counts of the pass, heavy-first bin order; resets
-//                the append cursors for the next pass
-// The per-bin depth walk and shading run later in one launch (shade.hip).
-//
-// Shadow pass (shadowMap), three launches: k_shadow_setup_bin, k_bin_order
-// and k_shadow_bins, which walks each bin's casters (a11, Shaders/drawmin.c)
-// with the bin's texels in LDS and writes them once.
+// A drawAll and the shadow passes recorded before it run as three launches:
+//   k_setup      per triangle, positions read once: the drawAll's setup (a5)
+//                with its record stored by global index, small triangles
+//                appended to the 32x32 screen bins their pixel rectangle
+//                touches and large triangles to the bins of their coarse
+//                16-px tiles (a8, Pipe/coarse.c); then the ortho setup of each
+//                pending shadow pass (a11, Pipe/trisetupOrtho.c) appended to
+//                that map's 32x32 texel bins
+//   k_bin_order  entry counts of the pass, heavy-first bin order; resets the
+//                append cursors for the next pass (one block per bin store)
+//   k_shadow_bins  each pending map's bins walked with the bin's texels in
+//                LDS (Shaders/drawmin.c) and written once
+// The screen bins' depth walk and shading run later in one launch
+// (shade.hip).  Shadow passes with no drawAll after them run the same
+// launches without the screen part.
 //
 // Appends aggregate within a wave: lanes of one wave come from one mesh
 // region and mostly share a bin, so one atomic serves all of them.  Depth
@@ -100,156 +102,234 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
-// K1: setup + bin appends.  Slot b: small triangles (global index); slot
-// NB + b: large triangles' coarse tiles, (global index << 2) | the 16-px
-// tile's quadrant in bin b.
-__global__ __launch_bounds__(kBlock) void k_setup_bin(const float4* __restrict__ vpos,
-                                                      const float4* __restrict__ vattr,
-                                                      const AxrGroupDev* __restrict__ groups,
-                                                      AxrDrawConst dc, AxrTriRec* __restrict__ rec,
-                                                      uint8_t* __restrict__ cls_out, AxrBins B,
-                                                      int NBX) {
+// K1: setup + bin appends.  Screen slot b: small triangles (global index);
+// slot NB + b: large triangles' coarse tiles, (global index << 2) | the
+// 16-px tile's quadrant in bin b.  Shadow slot slot0 + b of a pass: casters
+// (global index).
+__global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpos,
+                                                  const float4* __restrict__ vattr,
+                                                  const AxrGroupDev* __restrict__ groups,
+                                                  AxrDrawConst dc, int do_main,
+                                                  AxrTriRec* __restrict__ rec,
+                                                  uint8_t* __restrict__ cls_out, AxrBins MB,
+                                                  int NBX, AxrPassSet ps, AxrBins SB) {
   const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
-  int cls = 0;
-  AxrTriRec r;
-  if (gi < dc.T) {
-    const uint32_t f = groups[find_group(groups, (int)dc.ngroups, gi)].flags;
-    if (f & AXR_GRP_DRAW)
-      cls = tri_setup(vpos, vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
-                      r.z);
-    cls_out[gi] = (uint8_t)cls;
-    if (cls) rec[gi] = r;
+  const bool live = gi < dc.T;
+  float4 vp[3];
+  int grp = 0;
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) vp[k] = vpos[3 * (size_t)gi + k];
+    grp = find_group(groups, (int)dc.ngroups, gi);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) vp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
-  BinRect br = {0, 0, 1, 0};
-  if (cls == 1) {
-    int x0, y0, x1, y1;
-    small_bounds(r.px, r.py, x0, y0, x1, y1);
-    br = bin_rect(x0, y0, x1, y1, dc.W, dc.H);
+  if (do_main) {
+    int cls = 0;
+    AxrTriRec r;
+    if (live) {
+      const uint32_t f = groups[grp].flags;
+      if (f & AXR_GRP_DRAW)
+        cls = tri_setup(vp, vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
+                        r.z);
+      cls_out[gi] = (uint8_t)cls;
+      if (cls) rec[gi] = r;
+    }
+    BinRect br = {0, 0, 1, 0};
+    if (cls == 1) {
+      int x0, y0, x1, y1;
+      small_bounds(r.px, r.py, x0, y0, x1, y1);
+      br = bin_rect(x0, y0, x1, y1, dc.W, dc.H);
+    }
+    for (int k = 0;; k++) {
+      const bool act = k < br.n;
+      if (__ballot(act) == 0) break;
+      const uint32_t slot = act ? rect_bin(br, k, NBX) : 0u;
+      const uint32_t pos = wave_bin_add(MB.cnt, act, slot);
+      if (act) bin_store(MB, slot, pos, gi);
+    }
+    if (cls == 2) {
+      coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
+        uint32_t bin, sub;
+        tile_to_bin(t, dc.WC, NBX, bin, sub);
+        const uint32_t slot = (uint32_t)MB.nb + bin;
+        bin_store(MB, slot, atomicAdd(&MB.cnt[slot], 1u), (gi << 2) | sub);
+      });
+    }
   }
-  for (int k = 0;; k++) {
-    const bool act = k < br.n;
-    if (__ballot(act) == 0) break;
-    const uint32_t slot = act ? rect_bin(br, k, NBX) : 0u;
-    const uint32_t pos = wave_bin_add(B.cnt, act, slot);
-    if (act) bin_store(B, slot, pos, gi);
-  }
-  if (cls == 2) {
-    coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
-      uint32_t bin, sub;
-      tile_to_bin(t, dc.WC, NBX, bin, sub);
-      const uint32_t slot = (uint32_t)B.nb + bin;
-      bin_store(B, slot, atomicAdd(&B.cnt[slot], 1u), (gi << 2) | sub);
-    });
+#pragma unroll
+  for (int p = 0; p < AXR_MAX_SHADOWS; p++) {
+    if (p >= ps.n) break;
+    const AxrShadowPass& P = ps.p[p];
+    bool ok = false;
+    AxrShRec r;
+    if (live) {
+      if (cast_bit(P.cast, grp)) ok = shadow_setup(vp, P.cam, P.dim, P.bias, r);
+      P.ok[gi] = ok ? 1 : 0;
+      if (ok) P.rec[gi] = r;
+    }
+    BinRect br = {0, 0, 1, 0};
+    if (ok) {
+      int x0, y0, x1, y1;
+      shadow_bounds(r, x0, y0, x1, y1);
+      br = bin_rect(x0, y0, x1, y1, P.dim, P.dim);
+    }
+    for (int k = 0;; k++) {
+      const bool act = k < br.n;
+      if (__ballot(act) == 0) break;
+      const uint32_t slot = act ? P.slot0 + rect_bin(br, k, P.nbx) : 0u;
+      const uint32_t pos = wave_bin_add(SB.cnt, act, slot);
+      if (act) bin_store(SB, slot, pos, gi);
+    }
   }
 }
 
-constexpr int kOrderClasses = 66;
-// 0 for an empty bin, else 2*floor(log2 w) + (next bit) + 2.
-__device__ __forceinline__ int order_class(uint32_t w) {
+constexpr int kOrderClasses = 64;
+// 0 for an empty bin, else 2*floor(log2 w) + (next bit) + 2, capped at 63.
+__device__ __forceinline__ uint32_t order_class(uint32_t w) {
   if (w == 0) return 0;
-  const int l = 31 - __clz(w);
+  const int l = imin(30, 31 - __clz(w));
   const int half = l > 0 ? (int)((w >> (l - 1)) & 1u) : 0;
-  return 2 * l + half + 2;
+  return (uint32_t)(2 * l + half + 2);
 }
 
-// K2 (one block): num = cnt, cnt = 0 for the next pass; the pass's entry
-// total, spill and loss into the counter block; bins in heavy-first launch
-// order (weight = small entries + 16 x large-tile entries), so the longest
-// bins start first and short ones fill the tail.  Order within a weight
-// class is arbitrary: each bin is computed by one block, so results do not
-// depend on it.
-__global__ __launch_bounds__(1024) void k_bin_order(AxrBins B, int nseg,
-                                                    uint32_t* __restrict__ ctr_total,
-                                                    uint32_t* __restrict__ ctr_spill,
-                                                    uint32_t* __restrict__ ctr_lost) {
+// One bin store to order, with the counter words it reports into.
+struct OrderJob {
+  AxrBins B;
+  int nseg;
+  int ctr_total, ctr_spill;
+};
+
+// K2, one block per bin store: num = cnt and cnt = 0 for the next pass; the
+// pass's entry total, spill and loss into the counter block; bins in
+// heavy-first launch order (weight = small entries + 16 x large-tile
+// entries), so the longest bins start first and short ones fill the tail.
+// Order within a weight class is arbitrary: each bin is computed by one
+// block, so results do not depend on it.
+constexpr int kOrderPer = 8;               // bins per thread per round
+constexpr int kOrderLds = 1024 * kOrderPer * 2;  // bins whose class is kept in LDS
+
+__global__ __launch_bounds__(1024) void k_bin_order(OrderJob j0, OrderJob j1,
+                                                    uint32_t* __restrict__ ctr) {
   __shared__ uint32_t hist[kOrderClasses];
   __shared__ uint32_t total;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < kOrderClasses; i += 1024) hist[i] = 0;
+  __shared__ uint8_t klass[kOrderLds];
+  const bool second = blockIdx.x != 0;
+  const AxrBins B = second ? j1.B : j0.B;
+  const int nseg = second ? j1.nseg : j0.nseg;
+  const int nb = B.nb;
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (tid < kOrderClasses) hist[tid] = 0;
   if (tid == 0) total = 0;
   __syncthreads();
   uint32_t tot = 0;
-  for (int b = tid; b < B.nb; b += 1024) {
-    uint32_t w = 0;
-    for (int sg = 0; sg < nseg; sg++) {
-      const uint32_t slot = (uint32_t)(sg * B.nb + b);
-      const uint32_t c = B.cnt[slot];
-      B.num[slot] = c;
-      B.cnt[slot] = 0;
-      tot += c;
-      w += sg ? 16u * c : c;
+  // Counts are loaded a round at a time so the loads overlap.
+  for (int base = 0; base < nb; base += 1024 * kOrderPer) {
+    uint32_t c0[kOrderPer], c1[kOrderPer];
+#pragma unroll
+    for (int u = 0; u < kOrderPer; u++) {
+      const int b = base + u * 1024 + tid;
+      c0[u] = b < nb ? B.cnt[b] : 0u;
+      c1[u] = (nseg > 1 && b < nb) ? B.cnt[nb + b] : 0u;
     }
-    atomicAdd(&hist[order_class(w)], 1u);
+#pragma unroll
+    for (int u = 0; u < kOrderPer; u++) {
+      const int b = base + u * 1024 + tid;
+      const bool valid = b < nb;
+      uint32_t k = 0;
+      if (valid) {
+        B.num[b] = c0[u];
+        B.cnt[b] = 0;
+        if (nseg > 1) {
+          B.num[nb + b] = c1[u];
+          B.cnt[nb + b] = 0;
+        }
+        tot += c0[u] + c1[u];
+        k = order_class(c0[u] + 16u * c1[u]);
+        if (b < kOrderLds) klass[b] = (uint8_t)k;
+      }
+      // empty bins (most of a sparse frame) count with one atomic per wave
+      const uint64_t em = __ballot(valid && k == 0);
+      if (em && lane == __ffsll((unsigned long long)em) - 1) atomicAdd(&hist[0], (uint32_t)__popcll(em));
+      if (valid && k != 0) atomicAdd(&hist[k], 1u);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
-  if ((tid & 63) == 0) atomicAdd(&total, tot);
+  if (lane == 0) atomicAdd(&total, tot);
   __syncthreads();
-  if (tid == 0) {
-    uint32_t run = 0;
-    for (int k = kOrderClasses - 1; k >= 0; k--) {
-      const uint32_t c = hist[k];
-      hist[k] = run;
-      run += c;
+  if (tid < 64) {
+    // descending exclusive offsets: classes above this one come first
+    const uint32_t h = hist[tid];
+    uint32_t sfx = h;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_down(sfx, o);
+      if (lane + o < 64) sfx += t;
     }
-    *ctr_total = total;
+    hist[tid] = sfx - h;
+  }
+  if (tid == 0) {
+    ctr[second ? j1.ctr_total : j0.ctr_total] = total;
     const uint32_t used = B.spill_ctr[0];
     B.spill_ctr[1] = used < B.spill_cap ? used : B.spill_cap;
     B.spill_ctr[0] = 0;
-    if (used) atomicMax(ctr_spill, used);
-    if (used > B.spill_cap) atomicAdd(ctr_lost, used - B.spill_cap);
+    if (used) atomicMax(&ctr[second ? j1.ctr_spill : j0.ctr_spill], used);
+    if (used > B.spill_cap) atomicAdd(&ctr[AXR_CTR_LOST], used - B.spill_cap);
   }
   __syncthreads();
-  for (int b = tid; b < B.nb; b += 1024) {
-    uint32_t w = 0;
-    for (int sg = 0; sg < nseg; sg++) {
-      const uint32_t c = B.num[sg * B.nb + b];  // written by this thread above
-      w += sg ? 16u * c : c;
+  for (int b0 = 0; b0 < nb; b0 += 1024) {
+    const int b = b0 + tid;
+    const bool valid = b < nb;
+    uint32_t k = 0;
+    if (valid) {
+      if (b < kOrderLds) {
+        k = klass[b];
+      } else {  // written by this thread above
+        k = order_class(B.num[b] + (nseg > 1 ? 16u * B.num[nb + b] : 0u));
+      }
     }
-    B.order[atomicAdd(&hist[order_class(w)], 1u)] = (uint32_t)b;
+    const uint64_t em = __ballot(valid && k == 0);
+    uint32_t base = 0;
+    const int leader = em ? __ffsll((unsigned long long)em) - 1 : 0;
+    if (em && lane == leader) base = atomicAdd(&hist[0], (uint32_t)__popcll(em));
+    base = __shfl(base, leader);
+    if (valid) {
+      const uint32_t pos = k == 0 ? base + (uint32_t)__popcll(em & lanemask_lt())
+                                  : atomicAdd(&hist[k], 1u);
+      B.order[pos] = (uint32_t)b;
+    }
   }
 }
 
-// S1: ortho setup of every casting group + bin appends (global index).
-__global__ __launch_bounds__(kBlock) void k_shadow_setup_bin(
-    const float4* __restrict__ vpos, const AxrGroupDev* __restrict__ groups, uint32_t ngroups,
-    uint32_t T, AxrCastMask cast, AxrCam cam, int dim, float bias, AxrShRec* __restrict__ rec,
-    uint8_t* __restrict__ ok_out, AxrBins B, int NBX) {
-  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
-  bool ok = false;
-  AxrShRec r;
-  if (gi < T) {
-    if (cast_bit(cast, find_group(groups, (int)ngroups, gi)))
-      ok = shadow_setup(vpos, gi, cam, dim, bias, r);
-    ok_out[gi] = ok ? 1 : 0;
-    if (ok) rec[gi] = r;
-  }
-  BinRect br = {0, 0, 1, 0};
-  if (ok) {
-    int x0, y0, x1, y1;
-    shadow_bounds(r, x0, y0, x1, y1);
-    br = bin_rect(x0, y0, x1, y1, dim, dim);
-  }
-  for (int k = 0;; k++) {
-    const bool act = k < br.n;
-    if (__ballot(act) == 0) break;
-    const uint32_t slot = act ? rect_bin(br, k, NBX) : 0u;
-    const uint32_t pos = wave_bin_add(B.cnt, act, slot);
-    if (act) bin_store(B, slot, pos, gi);
-  }
-}
-
-// S3: one block per 32x32 texel bin, heavy bins first; minimum depth keys
-// in LDS, written once.  The bin starts from the map (from_map) or from the
-// clear depth.
-__global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(
-    const AxrShRec* __restrict__ rec, AxrBins B, uint32_t* __restrict__ map, int dim, int NBX,
-    int from_map) {
+// K3: one block per 32x32 texel bin of every pending map, heavy bins first;
+// minimum depth keys in LDS, written once.  A bin starts from its map
+// (from_map) or from the clear depth.
+__global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, AxrPassSet ps) {
   __shared__ uint32_t lk[AXR_BIN * AXR_BIN];
-  const int b = (int)B.order[blockIdx.x];
-  const uint32_t n = B.num[b];
+  const uint32_t slot = B.order[blockIdx.x];
+  // the pass owning the slot (slot0 ascends); selected field by field so the
+  // by-value pass table is never indexed dynamically
+  const AxrShRec* __restrict__ rec = ps.p[0].rec;
+  uint32_t* __restrict__ map = ps.p[0].map;
+  int dim = ps.p[0].dim, nbx = ps.p[0].nbx, from_map = ps.p[0].from_map;
+  uint32_t slot0 = 0;
+#pragma unroll
+  for (int p = 1; p < AXR_MAX_SHADOWS; p++) {
+    if (p < ps.n && slot >= ps.p[p].slot0) {
+      rec = ps.p[p].rec;
+      map = ps.p[p].map;
+      dim = ps.p[p].dim;
+      nbx = ps.p[p].nbx;
+      from_map = ps.p[p].from_map;
+      slot0 = ps.p[p].slot0;
+    }
+  }
+  const int b = (int)(slot - slot0);
+  const uint32_t n = B.num[slot];
   if (from_map && n == 0) return;
-  const int x0 = (b % NBX) << AXR_BIN_SHIFT, y0 = (b / NBX) << AXR_BIN_SHIFT;
+  const int x0 = (b % nbx) << AXR_BIN_SHIFT, y0 = (b / nbx) << AXR_BIN_SHIFT;
   const int xe = imin(x0 + AXR_BIN, dim), ye = imin(y0 + AXR_BIN, dim);
   const uint32_t clear = ordf(AXR_DEPTH_CLEAR);
   for (int i = threadIdx.x; i < AXR_BIN * AXR_BIN; i += blockDim.x) {
@@ -257,7 +337,7 @@ __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(
     lk[i] = (from_map && x < xe && y < ye) ? map[(size_t)y * dim + x] : clear;
   }
   __syncthreads();
-  for_bin_entries(B, (uint32_t)b, n, (int)threadIdx.x, (int)blockDim.x, [&](uint32_t gi) {
+  for_bin_entries(B, slot, n, (int)threadIdx.x, (int)blockDim.x, [&](uint32_t gi) {
     const AxrShRec r = rec[gi];
     shadow_walk(r, dim, x0, y0, xe, ye, [&](int ax, int cy, float tz) {
       if (tz != tz) return;
@@ -279,7 +359,7 @@ __global__ void k_stats(const uint8_t* __restrict__ cls, const uint8_t* __restri
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < T; i += gridDim.x * blockDim.x) {
     s += cls[i] == 1;
     l += cls[i] == 2;
-    c += ok[i] != 0;
+    c += ok != nullptr && ok[i] != 0;
   }
   atomicAdd(&out[0], s);
   atomicAdd(&out[1], l);
@@ -302,36 +382,31 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_setup_bin(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
-                            const AxrDrawConst& dc, AxrTriRec* rec, uint8_t* cls,
-                            const AxrBins& B, int NBX, hipStream_t s) {
-  if (dc.T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup_bin, dim3((dc.T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
-                     vattr, groups, dc, rec, cls, B, NBX);
+hipError_t launch_setup(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
+                        const AxrDrawConst& dc, int do_main, AxrTriRec* rec, uint8_t* cls,
+                        const AxrBins& MB, int NBX, const AxrPassSet& ps, const AxrBins& SB,
+                        hipStream_t s) {
+  if (dc.T == 0 || (!do_main && ps.n == 0)) return hipSuccess;
+  hipLaunchKernelGGL(k_setup, dim3((dc.T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
+                     vattr, groups, dc, do_main, rec, cls, MB, NBX, ps, SB);
   return hipGetLastError();
 }
 
-hipError_t launch_bin_order(const AxrBins& B, int nseg, uint32_t* ctr_total,
-                            uint32_t* ctr_spill, uint32_t* ctr_lost, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_order, dim3(1), dim3(1024), 0, s, B, nseg, ctr_total, ctr_spill,
-                     ctr_lost);
+hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins,
+                            uint32_t* ctr, hipStream_t s) {
+  OrderJob j[2];
+  int n = 0;
+  if (main_bins) j[n++] = OrderJob{*main_bins, 2, AXR_CTR_BIN_TOTAL, AXR_CTR_BIN_SPILL};
+  if (shadow_bins) j[n++] = OrderJob{*shadow_bins, 1, AXR_CTR_SH_TOTAL, AXR_CTR_SH_SPILL};
+  if (n == 0) return hipSuccess;
+  if (n == 1) j[1] = j[0];
+  hipLaunchKernelGGL(k_bin_order, dim3(n), dim3(1024), 0, s, j[0], j[1], ctr);
   return hipGetLastError();
 }
 
-hipError_t launch_shadow_setup_bin(const float4* vpos, const AxrGroupDev* groups,
-                                   uint32_t ngroups, uint32_t T, const AxrCastMask& cast,
-                                   const AxrCam& cam, int dim, float bias, AxrShRec* rec,
-                                   uint8_t* ok, const AxrBins& B, int NBX, hipStream_t s) {
-  if (T == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_shadow_setup_bin, dim3((T + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                     vpos, groups, ngroups, T, cast, cam, dim, bias, rec, ok, B, NBX);
-  return hipGetLastError();
-}
-
-hipError_t launch_shadow_bins(const AxrShRec* rec, const AxrBins& B, uint32_t* map, int dim,
-                              int NBX, int from_map, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow_bins, dim3(B.nb), dim3(AXR_SHADOW_THREADS), 0, s, rec, B, map, dim,
-                     NBX, from_map);
+hipError_t launch_shadow_bins(const AxrBins& B, const AxrPassSet& ps, hipStream_t s) {
+  if (ps.n == 0 || B.nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shadow_bins, dim3(B.nb), dim3(AXR_SHADOW_THREADS), 0, s, B, ps);
   return hipGetLastError();
 }
 
