@@ -28,6 +28,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the frame kernels, measured with two rocprofv3
+# counter runs (FETCH_SIZE, then WRITE_SIZE) over
+# `bench.py --steps 5 --warmup 2 --latency-frames 2 --no-cpu-baseline
+# --calibrate-copy 256`, each scaled by what the same counter reads for the
+# plain 256 MiB device copy (fetch x 1.9995, write x 1.0000).  Valid for the
+# kernels as they are; re-measure after changing one.
+MEASURED_TRAFFIC = {
+    "c2": {"shadow_raster": 105085628, "raster_shade": 133268406, "setup_bin": 138315164},
+}
+
 
 def algorithmic_bytes(sc, st):
     """Compulsory HBM bytes per frame and per kernel launch.
@@ -35,29 +45,32 @@ def algorithmic_bytes(sc, st):
     Per-unit figures: a triangle carries 132 B (3 vertices x pos 12 +
     normal 12 + uv 8 + light 12); a pixel 8 B out (RGBA8 + f32 depth); a
     texture 3 B per texel over its mip chain; a shadow cascade re-reads
-    36 B of positions per triangle and writes and reads its map (4 B/texel)."""
+    36 B of positions per triangle and writes and reads its map (4 B/texel).
+    Per kernel: a setup record is 36 B, a bin entry 4 B, a class or caster
+    flag 1 B."""
     W, H, T = sc.W, sc.H, sc.ntri
     tex = sum(3 * g.tex.shape[0] for g in sc.groups)
     frame = T * 132 + W * H * 8 + tex
     for c in sc.cascades:
         frame += T * 36 + 2 * c.dim * c.dim * 4
     visible = st["small"] + st["large"]
-    # shadow kernels: per-launch means over the cascades of one frame
-    casters = float(np.mean(st["cascade_casters"])) if sc.cascades else 0.0
     entries = st["bin_entries"]
-    sh_entries = float(np.mean(st["cascade_entries"])) if sc.cascades else 0.0
-    dmax = max((c.dim for c in sc.cascades), default=0)
+    casters = st["cascade_casters"]
+    sh_entries = st["cascade_entries"]
     per_kernel = {
-        # bin entries and the setup records they name in; RGBA8 + depth out
+        # bin entries and the setup records they name; RGBA8 + depth out
         # (8 B/px); the winning triangles' vertex data (132 B each, bounded
         # by the visible set)
         "raster_shade": entries * (4 + 36) + W * H * 8 + visible * 132,
-        # positions of every triangle (36 B) and its class byte; one 36 B
-        # setup record per visible triangle; one 4 B entry per bin it touches
-        "setup_bin": T * 37 + visible * 36 + entries * 4,
-        "shadow_setup_bin": T * 37 + casters * 36 + sh_entries * 4,
-        # caster records via their bin entries; the map written once
-        "shadow_raster": sh_entries * (36 + 4) + dmax * dmax * 4,
+        # one launch for the draw and every cascade: positions read once
+        # (36 B/tri); class byte, record and entries of the draw; caster
+        # flag, record and entries per cascade
+        "setup_bin": T * (36 + 1) + visible * 36 + entries * 4
+        + sum(T + c * 36 + e * 4 for c, e in zip(casters, sh_entries)),
+        # one launch for every cascade: caster records via their bin
+        # entries, each map written once
+        "shadow_raster": sum(e * (36 + 4) for e in sh_entries)
+        + sum(c.dim * c.dim * 4 for c in sc.cascades),
     }
     return frame, per_kernel
 
@@ -185,8 +198,9 @@ def main():
     roof = None
     if dom_bytes:
         ach = dom_bytes / (dom_us * 1e-6) / 1e9
+        traffic = MEASURED_TRAFFIC.get(args.config, {}).get(dom)
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
                 "algorithmic_bytes": int(dom_bytes), "avg_us": round(dom_us, 2)}
     frame_gbs = frame_bytes * fps / 1e9
 
