@@ -17,6 +17,19 @@ _i64 = ctypes.c_int64
 _p = ctypes.c_void_p
 _ctx = ctypes.c_void_p
 
+_u32 = ctypes.c_uint32
+_batch = ctypes.c_void_p
+
+
+class AxrJob(ctypes.Structure):
+    """axr_job of include/axr.h."""
+    _fields_ = [("scene", ctypes.c_void_p), ("pos", ctypes.c_float * 3),
+                ("vm", ctypes.c_float * 9), ("scale", ctypes.c_float),
+                ("cax", ctypes.c_float), ("cay", ctypes.c_float), ("width", ctypes.c_int),
+                ("height", ctypes.c_int), ("shadows", ctypes.c_int),
+                ("pcf_taps", ctypes.c_int), ("exposure", ctypes.c_float)]
+
+
 # name -> (restype, argtypes); every function declared in include/axr.h.
 SIGNATURES = {
     "axr_last_error": (ctypes.c_char_p, []),
@@ -47,6 +60,18 @@ SIGNATURES = {
     "axr_set_bin_capacity": (_i, [_ctx, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_uint32]),
     "axr_set_timing": (_i, [_ctx, _i]),
+    "axr_set_shaders": (_i, [_ctx, _p, _p, _i]),
+    "axr_get_bytes": (_i, [_ctx, ctypes.POINTER(ctypes.c_uint64)]),
+    "axr_batch_create": (_i, [ctypes.POINTER(_batch)]),
+    "axr_batch_destroy": (_i, [_batch]),
+    "axr_batch_configure": (_i, [_batch, _i, _u32, _u32, _i]),
+    "axr_batch_render": (_i, [_batch, ctypes.POINTER(AxrJob), _i, _i]),
+    "axr_batch_query": (_i, [_batch]),
+    "axr_batch_wait": (_i, [_batch, ctypes.POINTER(ctypes.c_float)]),
+    "axr_batch_get_rgba8": (_i, [_batch, _i, _p]),
+    "axr_batch_get_depth": (_i, [_batch, _i, _p]),
+    "axr_batch_get_ids": (_i, [_batch, _i, _p]),
+    "axr_batch_get_bytes": (_i, [_batch, ctypes.POINTER(ctypes.c_uint64)]),
     "axr_get_timing": (_i, [_ctx, _p, _i, _p, _p, _i]),
 }
 

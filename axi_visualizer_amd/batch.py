@@ -1,0 +1,183 @@
+"""Batched rendering: many views of resident scenes per launch sequence.
+
+A ``SceneContext`` keeps a Scene resident on the device (a CLDraw holding its
+groups, textures, shader binding, lights and rendered shadow maps).  A
+``Batch`` renders a list of ``Job`` s -- one camera at one resolution on one
+resident scene each -- with one setup launch, one bin-order launch and one
+persistent raster launch for all of them (libaxr's ``axr_batch_*``).  Each
+job's pixels are bit-identical to the same view drawn alone with
+``CLDraw.drawAll``.  ``autotune`` picks the bin size (8/16/32 px) and the
+bucket capacities by timing the actual batch.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import time
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from ._native import AxrError, AxrJob, check, lib, ptr
+from .ops import CLDraw
+from .render import load_scene
+from .scenes import Camera, Scene, make_camera
+
+
+class SceneContext:
+    """A Scene made resident for batches (never drawn through its own frame)."""
+
+    def __init__(self, scene: Scene):
+        self.scene = sc = scene
+        self.draw = d = CLDraw(1, len(sc.groups), 16, 16, 1)
+        load_scene(d, sc)
+        cast = [g.cast for g in sc.groups]
+        shaders = [g.flags for g in sc.groups]
+        for i, c in enumerate(sc.cascades):
+            d.clearShadowMap(i)
+            d.shadowMap(i, cast, shaders, c.bias)
+        d.setShaders(shaders, [g.skip for g in sc.groups])
+        d.sync()
+
+    @property
+    def handle(self):
+        return self.draw._ctx
+
+    def device_bytes(self) -> int:
+        return self.draw.deviceBytes()
+
+    def close(self):
+        self.draw.close()
+
+
+@dataclasses.dataclass
+class Job:
+    scene: SceneContext
+    camera: Camera
+    width: int
+    height: int
+    shadows: bool = False
+    pcf_taps: int = 5
+    exposure: float = 1.0
+
+    def to_c(self) -> AxrJob:
+        j = AxrJob()
+        j.scene = self.scene.handle
+        c = self.camera
+        j.pos[:] = [float(v) for v in np.asarray(c.pos, np.float32).reshape(3)]
+        j.vm[:] = [float(v) for v in np.asarray(c.vm, np.float32).reshape(9)]
+        j.scale, j.cax, j.cay = float(c.scale), float(c.cax), float(c.cay)
+        j.width, j.height = int(self.width), int(self.height)
+        j.shadows = 1 if self.shadows else 0
+        j.pcf_taps = int(self.pcf_taps)
+        j.exposure = float(self.exposure)
+        return j
+
+
+def job_camera(width: int, height: int, pos, target, fovx: float = 80.0) -> Camera:
+    """Camera at pos looking at target, scale/cone for this resolution."""
+    from .scenes import look_at
+
+    a, b = look_at(pos, target)
+    return make_camera(width, height, pos, a, b, fovx)
+
+
+class Batch:
+    """One batch slot: its own stream and buffers (several may be in flight)."""
+
+    def __init__(self, bin_size: int = 32, cap_small: int = 0, cap_large: int = 0,
+                 graph: bool = False):
+        self._L = lib()
+        b = ctypes.c_void_p()
+        check(self._L.axr_batch_create(ctypes.byref(b)), "axr_batch_create")
+        self._b = b
+        self.jobs: List[Job] = []
+        self.ids = False
+        self.configure(bin_size, cap_small, cap_large, graph)
+
+    def configure(self, bin_size=32, cap_small=0, cap_large=0, graph=False):
+        check(self._L.axr_batch_configure(self._b, int(bin_size), int(cap_small), int(cap_large),
+                                          1 if graph else 0), "axr_batch_configure")
+        self.bin_size = bin_size
+        self.cap_small, self.cap_large, self.graph = cap_small, cap_large, graph
+
+    def render(self, jobs: Sequence[Job], ids: bool = False):
+        """Enqueues the jobs and returns at once."""
+        self.jobs = list(jobs)
+        self.ids = ids
+        arr = (AxrJob * max(1, len(self.jobs)))(*[j.to_c() for j in self.jobs])
+        check(self._L.axr_batch_render(self._b, arr, len(self.jobs), 1 if ids else 0),
+              "axr_batch_render")
+
+    def done(self) -> bool:
+        return check(self._L.axr_batch_query(self._b), "axr_batch_query") == 1
+
+    def wait(self) -> float:
+        """Waits for the last render; returns its GPU time in ms."""
+        ms = ctypes.c_float()
+        check(self._L.axr_batch_wait(self._b, ctypes.byref(ms)), "axr_batch_wait")
+        return float(ms.value)
+
+    def _get(self, fn, j, dtype):
+        job = self.jobs[j]
+        out = np.empty((job.height, job.width), dtype)
+        check(fn(self._b, int(j), ptr(out)), "batch readback")
+        return out
+
+    def rgba8(self, j: int) -> np.ndarray:
+        """Job j's RGBA8 frame as uint32 (r | g<<8 | b<<16 | a<<24) [H, W]."""
+        return self._get(self._L.axr_batch_get_rgba8, j, np.uint32)
+
+    def depth(self, j: int) -> np.ndarray:
+        return self._get(self._L.axr_batch_get_depth, j, np.float32)
+
+    def ids_of(self, j: int) -> np.ndarray:
+        return self._get(self._L.axr_batch_get_ids, j, np.uint32)
+
+    def device_bytes(self) -> int:
+        n = ctypes.c_uint64()
+        check(self._L.axr_batch_get_bytes(self._b, ctypes.byref(n)), "axr_batch_get_bytes")
+        return int(n.value)
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._L.axr_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(jobs: Sequence[Job], sizes=(8, 16, 32), cap_scales=(1, 4), reps: int = 3,
+             graph: bool = True, batch: Optional[Batch] = None) -> dict:
+    """Times the batch under each bin size and bucket-capacity scale and
+    returns the fastest configuration with every timing (GPU ms, median of
+    reps after one warm-up).  Capacities scale the per-size defaults
+    (4096 small / 1024 large entries for a 32-px bin, by area for smaller
+    bins); a configuration that spills still renders correctly, only slower."""
+    own = batch is None
+    b = batch or Batch()
+    results = []
+    try:
+        for size in sizes:
+            area = (size / 32.0) ** 2
+            for sc in cap_scales:
+                cs = max(64, int(4096 * area * sc))
+                cl = max(64, int(1024 * area * sc))
+                b.configure(size, cs, cl, graph)
+                b.render(jobs)
+                b.wait()
+                ts = []
+                for _ in range(reps):
+                    b.render(jobs)
+                    ts.append(b.wait())
+                results.append({"bin_size": size, "cap_small": cs, "cap_large": cl,
+                                "gpu_ms": float(np.median(ts))})
+        best = min(results, key=lambda r: r["gpu_ms"])
+        return {"best": best, "all": results}
+    finally:
+        if own:
+            b.close()

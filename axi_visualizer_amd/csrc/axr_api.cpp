@@ -11,22 +11,9 @@
 // lazily: the next shadow pass starts its bins from the clear depth instead
 // of reading a cleared map.  Anything that would change the inputs of a
 // draw that has not been shaded yet shades it first.
-#include <hip/hip_runtime.h>
+#include "axr_ctx.h"
 
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <map>
-#include <string>
-#include <vector>
-
-#include "../../include/axr.h"
-#include "axr_launch.h"
-
-using namespace axr;
-
-namespace {
+namespace axri {
 
 thread_local std::string g_err;
 
@@ -40,159 +27,6 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                 \
-  do {                                                                               \
-    hipError_t e_ = (expr);                                                          \
-    if (e_ != hipSuccess) return fail(AXR_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-struct HostGroup {
-  uint32_t base = 0, ntri = 0;
-  int tex_size = 0;
-  int is_mip = 0;
-  int64_t ntexels = 0;
-  float mip_bias = 0.0f;
-  int spec_pow = 16;
-  uint32_t flags = 0;  // as last uploaded to the device table
-  ushort4* d_tex = nullptr;
-};
-
-struct HostShadow {
-  bool exists = false;
-  bool placed = false;
-  bool clear_pending = false;  // cleared since its last pass; map not yet written
-  int dim = 0;
-  int nbx = 0, nb = 0;  // 32x32 bins
-  float scale = 0.0f;
-  AxrCam cam{};
-  uint32_t* d_map = nullptr;
-};
-
-enum State {
-  ST_UNCLEARED,  // nothing drawn since the context was made
-  ST_CLEARED,    // cleared, nothing drawn
-  ST_PENDING,    // triangles binned; colour not materialised
-  ST_HDR,        // u16 planes hold HDR colour
-  ST_LDR8,       // rgba8 holds gamma'd colour
-  ST_LDR16       // u16 planes hold gamma'd colour
-};
-
-// A shadowMap call recorded for the next batch of shadow passes.
-struct PendingPass {
-  int map;
-  AxrCastMask cast;
-  float bias;
-  AxrCam cam;    // placement at the time of the call
-  int from_map;  // 0 when the map was cleared before the pass
-};
-
-struct TimingRec {
-  const char* name;
-  hipEvent_t a, b;
-};
-
-constexpr int kCounters = AXR_CTR_WORDS;
-
-// Bucket capacities (entries per bin slot).  A pass appends into fixed
-// buckets; what does not fit goes to a spill list that the raster scans for
-// the affected bins only, and a pass that spilled grows the buckets before
-// the next pass of its kind.
-constexpr uint32_t kMainCap0 = 4096;   // small triangles per screen bin
-constexpr uint32_t kMainCap1 = 1024;   // large-triangle tiles per screen bin
-constexpr uint32_t kShadowCap = 8192;  // casters per shadow bin
-constexpr uint32_t kMinCap = 256;
-constexpr size_t kBucketBudget = (size_t)256 << 20;      // bytes for the initial buckets
-constexpr size_t kBucketGrowLimit = (size_t)4 << 30;     // bytes a growth may reach
-constexpr uint32_t kSpillCap = 1u << 20;                 // spill entries
-
-// Device bin storage of one pass kind.
-struct BinStore {
-  AxrBins B{};            // pointers and capacities as kernels see them
-  int nseg = 1;
-  int nb_alloc = 0;       // bins the cursor/count/order arrays hold
-  size_t ent_alloc = 0;   // bucket entries allocated
-  uint32_t want0 = 0, want1 = 0, want_spill = 0;  // capacities for the next pass
-};
-
-}  // namespace
-
-struct axr_ctx {
-  int device = 0;
-  int W = 0, H = 0, WC = 0, HC = 0;
-  int NBX = 0, NB = 0;  // 32x32 screen bins
-  hipStream_t stream = nullptr;
-
-  std::vector<HostGroup> groups;
-  uint32_t T = 0;
-  size_t tri_cap = 0;
-  float4* d_vpos = nullptr;   // [3*tri_cap] (x,y,z,0)
-  float4* d_vattr = nullptr;  // [6*tri_cap] (nx,ny,nz,u),(v,Ix,Iy,Iz)
-  AxrGroupDev* d_groups = nullptr;
-  AxrTriRec* d_rec = nullptr;   // [tri_cap] setup records of the last drawAll
-  uint8_t* d_cls = nullptr;     // [tri_cap] 0 culled, 1 small, 2 large
-  // [tri_cap] ortho setup and caster flags, one array per pass of a batch
-  AxrShRec* d_shrec[AXR_MAX_SHADOWS] = {};
-  uint8_t* d_ok[AXR_MAX_SHADOWS] = {};
-  const uint8_t* last_ok = nullptr;  // caster flags of the last shadow pass run
-
-  BinStore mbins;  // screen bins: slots [0, NB) small, [NB, 2NB) large tiles
-  BinStore sbins;  // shadow bins, sized for the largest map
-
-  float* d_depth = nullptr;
-  uint16_t *d_r = nullptr, *d_g = nullptr, *d_b = nullptr;
-  uint32_t* d_rgba = nullptr;
-  uint32_t* d_ids = nullptr;
-  uint32_t* d_counters = nullptr;
-  float* d_light = nullptr;  // vertLight light arrays
-  size_t light_cap = 0;
-
-  HostShadow sm[AXR_MAX_SHADOWS];
-  std::vector<PendingPass> pend;  // shadow passes not yet launched (one per map)
-  AxrCam view{};
-  float lint[3] = {1.0f, 1.0f, 1.0f};
-  float ldir[3] = {0.0f, -1.0f, 0.0f};
-  float amb = 0.2f;
-  uint16_t clear_rgb[3] = {1200, 4033, 21675};  // (60,110,255)^2/3, Ops.py:941
-
-  int state = ST_UNCLEARED;
-  AxrDrawConst last{};
-  bool records_valid = false;  // setup records and bins belong to `last`
-
-  bool timing = false;
-  std::vector<TimingRec> trecs;
-  std::vector<hipEvent_t> free_events;
-  std::map<std::string, std::pair<double, int64_t>> tacc;
-};
-
-namespace {
-
-// Scoped event pair around a launch when timing is enabled.
-struct Timed {
-  axr_ctx* c;
-  const char* name;
-  hipEvent_t a = nullptr, b = nullptr;
-  hipEvent_t take() {
-    if (!c->free_events.empty()) {
-      hipEvent_t e = c->free_events.back();
-      c->free_events.pop_back();
-      return e;
-    }
-    hipEvent_t e = nullptr;
-    (void)hipEventCreate(&e);
-    return e;
-  }
-  Timed(axr_ctx* ctx, const char* n) : c(ctx), name(n) {
-    if (!c->timing) return;
-    a = take();
-    b = take();
-    (void)hipEventRecord(a, c->stream);
-  }
-  ~Timed() {
-    if (!c->timing) return;
-    (void)hipEventRecord(b, c->stream);
-    c->trecs.push_back({name, a, b});
-  }
-};
 
 void drain_timing(axr_ctx* c) {
   if (c->trecs.empty()) return;
@@ -208,20 +42,6 @@ void drain_timing(axr_ctx* c) {
     c->free_events.push_back(r.b);
   }
   c->trecs.clear();
-}
-
-template <class T>
-int dalloc(T** p, size_t n) {
-  *p = nullptr;
-  if (n == 0) n = 1;
-  HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
-  return 0;
-}
-
-template <class T>
-void dfree(T*& p) {
-  if (p) (void)hipFree((void*)p);
-  p = nullptr;
 }
 
 void bins_free(BinStore& S) {
@@ -516,7 +336,7 @@ int flush_shadows(axr_ctx* c) {
   return run_setup(c, nullptr);
 }
 
-}  // namespace
+}  // namespace axri
 
 extern "C" {
 
@@ -799,6 +619,7 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
   dc.T = c->T;
   dc.ngroups = (uint32_t)ngroups;
   dc.pcf_taps = pcf_taps;
+  dc.flag_mask = 0xFFFFFFFFu;
   bool any_shadow = false, changed = false;
   std::vector<uint32_t> nf((size_t)ngroups);
   for (int g = 0; g < ngroups; g++) {
@@ -1045,6 +866,43 @@ int axr_get_stats(axr_ctx* c, uint32_t out[6]) {
   out[3] = h[AXR_CTR_BIN_TOTAL];
   out[4] = h[AXR_CTR_LOST];
   out[5] = h[AXR_CTR_SH_TOTAL];
+  return AXR_OK;
+}
+
+int axr_set_shaders(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask,
+                    int ngroups) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (ngroups != (int)c->groups.size() || !shader_flags)
+    return fail(AXR_EINVAL, "setShaders: %d entries for %d groups", ngroups,
+                (int)c->groups.size());
+  bool changed = false;
+  std::vector<uint32_t> nf((size_t)ngroups);
+  for (int g = 0; g < ngroups; g++) {
+    const uint32_t f = shader_flags[g] & (AXR_SH_MIP | AXR_SH_SHADOW | AXR_SH_PHONG | AXR_SH_CULL);
+    if ((f & AXR_SH_MIP) && !c->groups[g].is_mip)
+      return fail(AXR_EINVAL, "group %d: mip shader on a texture without mips", g);
+    nf[g] = f | ((mask && mask[g]) ? 0u : AXR_GRP_DRAW);
+    changed |= nf[g] != c->groups[g].flags;
+  }
+  if (!changed) return AXR_OK;
+  if ((rc = settle(c))) return rc;
+  for (int g = 0; g < ngroups; g++) c->groups[g].flags = nf[g];
+  return upload_groups(c);
+}
+
+int axr_get_bytes(axr_ctx* c, uint64_t* out) {
+  if (!c || !out) return fail(AXR_EINVAL, "bytes: bad arguments");
+  uint64_t n = 0;
+  n += (uint64_t)c->tri_cap * (3 * 16 + 6 * 16 + sizeof(AxrTriRec) + 1);
+  n += (uint64_t)c->tri_cap * AXR_MAX_SHADOWS * (sizeof(AxrShRec) + 1);
+  for (const auto& g : c->groups) n += (uint64_t)g.ntexels * 8;
+  for (const auto& s : c->sm)
+    if (s.exists) n += (uint64_t)s.dim * s.dim * 4;
+  for (const BinStore* S : {&c->mbins, &c->sbins})
+    n += (uint64_t)S->ent_alloc * 4 + (uint64_t)S->nb_alloc * 4 * 5 + (uint64_t)S->B.spill_cap * 8;
+  n += (uint64_t)c->W * c->H * (4 + 2 * 3 + 4 + 4);
+  *out = n;
   return AXR_OK;
 }
 

@@ -1,0 +1,175 @@
+// axr_bin.h -- bin appends and the per-bin depth walk + shading, shared by
+// the single-frame kernels and the batched (many-job) kernels.
+#pragma once
+#include "axr_pixel.h"
+
+namespace axr {
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const unsigned lane = __lane_id();
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Adds one per active lane to ctr[bin] with one atomic per distinct bin in
+// the wave; returns the counter's previous value plus the lane's rank among
+// the lanes that share its bin.  All lanes of the wave must call it.
+__device__ __forceinline__ uint32_t wave_bin_add(uint32_t* __restrict__ ctr, bool active,
+                                                 uint32_t bin) {
+  uint64_t pending = __ballot(active);
+  uint32_t result = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t b = __shfl(bin, leader);
+    const uint64_t grp = __ballot(active && bin == b) & pending;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(&ctr[b], (uint32_t)__popcll(grp));
+    base = __shfl(base, leader);
+    if ((grp >> __lane_id()) & 1ull) result = base + (uint32_t)__popcll(grp & lanemask_lt());
+    pending &= ~grp;
+  }
+  return result;
+}
+
+// Bin rectangle (bins of 1 << shift pixels) of a pixel rectangle clipped to
+// [0,w) x [0,h).
+struct BinRect {
+  int bx0, by0, nbx, n;
+};
+__device__ __forceinline__ BinRect bin_rect(int x0, int y0, int x1, int y1, int w, int h,
+                                            int shift) {
+  BinRect r = {0, 0, 1, 0};
+  x0 = imax(x0, 0);
+  y0 = imax(y0, 0);
+  x1 = imin(x1, w - 1);
+  y1 = imin(y1, h - 1);
+  if (x0 <= x1 && y0 <= y1) {
+    r.bx0 = x0 >> shift;
+    r.by0 = y0 >> shift;
+    r.nbx = (x1 >> shift) - r.bx0 + 1;
+    r.n = r.nbx * ((y1 >> shift) - r.by0 + 1);
+  }
+  return r;
+}
+__device__ __forceinline__ uint32_t rect_bin(const BinRect& r, int k, int nbx_total) {
+  return (uint32_t)((r.by0 + k / r.nbx) * nbx_total + r.bx0 + k % r.nbx);
+}
+
+// Appends a large triangle's reference 16-px tile t to the bins of
+// 1 << SHIFT pixels that hold it: a 32-px bin gets (gi << 2) | the tile's
+// quadrant; a 16-px bin is the tile; each 8-px bin of the tile gets gi << 2.
+template <int SHIFT>
+__device__ __forceinline__ void append_tile(const AxrBins& B, uint32_t slot_base, int t, int WC,
+                                            int NBX, uint32_t gi) {
+  const int tx = t % WC, ty = t / WC;
+  if (SHIFT == 5) {
+    const uint32_t slot = slot_base + (uint32_t)((ty >> 1) * NBX + (tx >> 1));
+    const uint32_t sub = (uint32_t)(((ty & 1) << 1) | (tx & 1));
+    bin_store(B, slot, atomicAdd(&B.cnt[slot], 1u), (gi << 2) | sub);
+  } else if (SHIFT == 4) {
+    const uint32_t slot = slot_base + (uint32_t)(ty * NBX + tx);
+    bin_store(B, slot, atomicAdd(&B.cnt[slot], 1u), gi << 2);
+  } else {
+    for (int q = 0; q < 4; q++) {
+      const uint32_t slot =
+          slot_base + (uint32_t)(((ty << 1) + (q >> 1)) * NBX + (tx << 1) + (q & 1));
+      bin_store(B, slot, atomicAdd(&B.cnt[slot], 1u), gi << 2);
+    }
+  }
+}
+
+// Reference 16-px tile of a large-triangle entry v of bin (bx, by).
+template <int SHIFT>
+__device__ __forceinline__ int entry_tile(uint32_t v, int bx, int by, int WC) {
+  if (SHIFT == 5) {
+    const uint32_t sub = v & 3u;
+    return ((by << 1) + (int)(sub >> 1)) * WC + (bx << 1) + (int)(sub & 1u);
+  }
+  if (SHIFT == 4) return by * WC + bx;
+  return (by >> 1) * WC + (bx >> 1);
+}
+
+// Depth walk and shading of one bin of (1 << SHIFT)^2 pixels by one block.
+// Small triangles of slot s_small walk drawSmall's scanlines clipped to the
+// bin (a7); large-triangle tile entries of slot s_large walk the tiled pass
+// on their reference tile (a9); each fragment takes an LDS min of
+// (orderable tz << 32) | ~seq.  Covered pixels are then compacted and
+// shaded once from their winning key; empty ones get the clear colour.
+// `empty` draws only the background.  lk, list, ncov are the block's LDS.
+template <int SHIFT, int MODE>
+__device__ __forceinline__ void raster_bin(
+    unsigned long long* lk, uint16_t* list, uint32_t* ncov, const AxrTriRec* __restrict__ rec,
+    const AxrBins& B, uint32_t s_small, uint32_t s_large, int empty, int bx, int by,
+    const float4* __restrict__ vpos, const float4* __restrict__ vattr,
+    const AxrGroupDev* __restrict__ groups, const AxrDrawConst& dc, float exposure,
+    float* __restrict__ depth, uint16_t* __restrict__ ro, uint16_t* __restrict__ go,
+    uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba, uint32_t* __restrict__ ids) {
+  constexpr int BIN = 1 << SHIFT, NPX = BIN * BIN;
+  const int x0 = bx << SHIFT, y0 = by << SHIFT;
+  const int xe = imin(x0 + BIN, dc.W), ye = imin(y0 + BIN, dc.H);
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  for (int i = tid; i < NPX; i += nthr) lk[i] = key_clear();
+  if (tid == 0) *ncov = 0;
+  __syncthreads();
+  if (!empty) {
+    for_bin_entries(B, s_small, B.num[s_small], tid, nthr, [&](uint32_t gi) {
+      const AxrTriRec r = rec[gi];
+      const Geom<int> g = small_geom(r.px, r.py, r.z);
+      small_walk(g, x0, y0, xe, ye, [&](int ax, int cy, float tz, int half) {
+        if (!(tz <= AXR_DEPTH_CLEAR)) return;
+        const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
+        if (lc < (unsigned)NPX) atomicMin(&lk[lc], (unsigned long long)make_key(tz, frag_seq(gi, half)));
+      });
+    });
+    for_bin_entries(B, s_large, B.num[s_large], tid, nthr, [&](uint32_t v) {
+      const uint32_t gi = v >> 2;
+      const TileBox box = tile_box(entry_tile<SHIFT>(v, bx, by, dc.WC), dc.W);
+      const AxrTriRec r = rec[gi];
+      const Geom<float> g = tile_geom(r.px, r.py, r.z);
+      const uint32_t order = dc.T + gi;
+      tile_walk(g, box, [&](int ax, int cy, float tz, int half) {
+        if (!(tz <= AXR_DEPTH_CLEAR)) return;
+        // a tile is larger than an 8-px bin: keep the bin's part of it
+        if (SHIFT < 4 && ((unsigned)(ax - x0) >= (unsigned)BIN || (unsigned)(cy - y0) >= (unsigned)BIN))
+          return;
+        const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
+        if (lc < (unsigned)NPX) atomicMin(&lk[lc], (unsigned long long)make_key(tz, frag_seq(order, half)));
+      });
+    });
+  }
+  __syncthreads();
+  for (int i = tid; i < NPX; i += nthr) {
+    const int x = x0 + (i & (BIN - 1)), y = y0 + (i >> SHIFT);
+    if (x >= xe || y >= ye) continue;
+    const size_t p = (size_t)y * dc.W + x;
+    const uint64_t key = lk[i];
+    const uint32_t low = (uint32_t)key;
+    if (MODE == SHADE_IDS) {
+      uint32_t id = AXR_NONE;
+      if (low != 0xFFFFFFFFu) {
+        const uint32_t order = ((~low) - 1u) >> 1;
+        id = order >= dc.T ? order - dc.T : order;
+      }
+      ids[p] = id;
+      depth[p] = unordf((uint32_t)(key >> 32));
+    } else if (low == 0xFFFFFFFFu) {
+      uint16_t rgb[3] = {dc.clear_rgb[0], dc.clear_rgb[1], dc.clear_rgb[2]};
+      put_px<MODE>(p, AXR_DEPTH_CLEAR, rgb, exposure, depth, ro, go, bo, rgba);
+    } else {
+      list[atomicAdd(ncov, 1u)] = (uint16_t)i;
+    }
+  }
+  if (MODE == SHADE_IDS) return;
+  __syncthreads();
+  const uint32_t n = *ncov;
+  for (uint32_t j = tid; j < n; j += nthr) {
+    const int i = list[j];
+    const int x = x0 + (i & (BIN - 1)), y = y0 + (i >> SHIFT);
+    const uint64_t key = lk[i];
+    uint16_t rgb[3];
+    resolve_pixel(key, x, y, rec, vpos, vattr, groups, dc, rgb);
+    put_px<MODE>((size_t)y * dc.W + x, unordf((uint32_t)(key >> 32)), rgb, exposure, depth, ro,
+                 go, bo, rgba);
+  }
+}
+
+}  // namespace axr

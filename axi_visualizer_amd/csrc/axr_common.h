@@ -65,6 +65,7 @@ struct AxrDrawConst {
   int pcf_taps;
   AxrShadowDev sh[AXR_SHADOW_BIND];
   uint16_t clear_rgb[4];
+  uint32_t flag_mask;  // shader flags honoured (a batch job may drop shadows)
 };
 
 // Setup output of one visible triangle (indexed by its global index).
@@ -145,8 +146,27 @@ struct AxrBins {
   uint32_t* order;      // [nb] heavy-first launch order
   uint2* spill;         // [spill_cap] (slot, entry)
   uint32_t* spill_ctr;  // [2]: append cursor, entries of the current pass
+  uint32_t* head;       // work-queue head of a persistent raster (or null)
   uint32_t cap0, cap1, spill_cap;
   int nb;
+};
+
+// One job of a batch -- a resident scene seen by one camera at one
+// resolution -- as the batched kernels read it from the device job table.
+struct AxrJobDev {
+  AxrDrawConst dc;  // camera, lights, size, bound shadow maps, flag mask
+  const float4* vpos;
+  const float4* vattr;
+  const AxrGroupDev* groups;
+  AxrTriRec* rec;  // [dc.T] setup records of this job
+  uint8_t* cls;    // [dc.T]
+  float* depth;    // [W*H] outputs
+  uint32_t* rgba;
+  uint32_t* ids;
+  float exposure;
+  uint32_t slot0;  // first screen-bin slot of this job
+  uint32_t blk0;   // first setup block of this job
+  int nbx;         // bins per row
 };
 
 AXR_HD size_t bin_base(const AxrBins& B, uint32_t s) {

@@ -34,6 +34,13 @@ hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, u
                                size_t n, hipStream_t s);
 hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipStream_t s);
 
+// Batched jobs (batch.hip).  bin_shift 3/4/5 = 8/16/32-px bins.
+hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, int njobs, uint32_t blocks,
+                              const AxrBins& B, hipStream_t s);
+hipError_t launch_batch_raster(int bin_shift, int ids, const AxrJobDev* jobs, int njobs,
+                               const AxrBins& B, int grid, hipStream_t s);
+int batch_raster_blocks_per_cu(int bin_shift, int ids);
+
 struct AxrLights {
   int ndir, npoint, nspot;
   float amb;

@@ -1,0 +1,172 @@
+// batch.hip -- many frames (jobs) in one launch sequence.
+//
+// A job is a resident scene seen by one camera at one resolution.  A batch
+// of jobs renders with three launches for all of them:
+//   k_batch_setup  one thread per (job, triangle): setup (a5) with the job's
+//                  camera, record and class in the job's arrays, appends to
+//                  the job's screen bins (slots slot0.. of one shared store;
+//                  large triangles' coarse tiles in the second segment)
+//   k_bin_order    the shared store's counts and heavy-first order across
+//                  every job's bins (raster.hip); it also resets the queue
+//   k_batch_raster a persistent kernel: each block takes the next bin from
+//                  that order (an atomic queue head) until none is left, and
+//                  walks and shades it exactly as the single-frame raster does
+// Bins are 8, 16 or 32 pixels on a side -- a scheduling unit only: small
+// triangles walk clipped to any bin, large ones keep the reference's 16-px
+// tiles.  The result of a job is bit-identical whatever the bin size and
+// whatever else is in the batch.
+#include "axr_bin.h"
+
+using namespace axr;
+
+namespace {
+
+constexpr int kSetupBlock = 256;
+
+// Job owning setup block blk (jobs ascend in blk0).
+__device__ __forceinline__ int job_of_block(const AxrJobDev* __restrict__ jobs, int n,
+                                            uint32_t blk) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].blk0 <= blk)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Job owning bin slot s (jobs ascend in slot0).
+__device__ __forceinline__ int job_of_slot(const AxrJobDev* __restrict__ jobs, int n, uint32_t s) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].slot0 <= s)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int SHIFT>
+__global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __restrict__ jobs,
+                                                             int njobs, AxrBins B) {
+  const AxrJobDev& J = jobs[job_of_block(jobs, njobs, blockIdx.x)];
+  const AxrDrawConst& dc = J.dc;
+  const uint32_t gi = (blockIdx.x - J.blk0) * kSetupBlock + threadIdx.x;
+  int cls = 0;
+  AxrTriRec r;
+  if (gi < dc.T) {
+    float4 vp[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) vp[k] = J.vpos[3 * (size_t)gi + k];
+    const uint32_t f = J.groups[find_group(J.groups, (int)dc.ngroups, gi)].flags;
+    if (f & AXR_GRP_DRAW)
+      cls = tri_setup(vp, J.vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
+                      r.z);
+    J.cls[gi] = (uint8_t)cls;
+    if (cls) J.rec[gi] = r;
+  }
+  BinRect br = {0, 0, 1, 0};
+  if (cls == 1) {
+    int x0, y0, x1, y1;
+    small_bounds(r.px, r.py, x0, y0, x1, y1);
+    br = bin_rect(x0, y0, x1, y1, dc.W, dc.H, SHIFT);
+  }
+  for (int k = 0;; k++) {
+    const bool act = k < br.n;
+    if (__ballot(act) == 0) break;
+    const uint32_t slot = act ? J.slot0 + rect_bin(br, k, J.nbx) : 0u;
+    const uint32_t pos = wave_bin_add(B.cnt, act, slot);
+    if (act) bin_store(B, slot, pos, gi);
+  }
+  if (cls == 2) {
+    coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
+      append_tile<SHIFT>(B, (uint32_t)B.nb + J.slot0, t, dc.WC, J.nbx, gi);
+    });
+  }
+}
+
+template <int SHIFT, int MODE>
+__global__ __launch_bounds__(1 << (2 * SHIFT) > 512 ? 512 : 1 << (2 * SHIFT)) void k_batch_raster(
+    const AxrJobDev* __restrict__ jobs, int njobs, AxrBins B) {
+  constexpr int NPX = 1 << (2 * SHIFT);
+  __shared__ unsigned long long lk[NPX];
+  __shared__ uint16_t list[NPX];
+  __shared__ uint32_t ncov, item;
+  for (;;) {
+    if (threadIdx.x == 0) item = atomicAdd(B.head, 1u);
+    __syncthreads();
+    const uint32_t it = item;
+    if (it >= (uint32_t)B.nb) break;
+    const uint32_t slot = B.order[it];
+    const AxrJobDev& J = jobs[job_of_slot(jobs, njobs, slot)];
+    const int b = (int)(slot - J.slot0);
+    raster_bin<SHIFT, MODE>(lk, list, &ncov, J.rec, B, slot, (uint32_t)B.nb + slot, 0,
+                            b % J.nbx, b / J.nbx, J.vpos, J.vattr, J.groups, J.dc, J.exposure,
+                            J.depth, nullptr, nullptr, nullptr, J.rgba, J.ids);
+    __syncthreads();  // the block's LDS is reused by the next bin
+  }
+}
+
+template <int SHIFT, int MODE>
+hipError_t raster_launch(const AxrJobDev* jobs, int njobs, const AxrBins& B, int grid,
+                         hipStream_t s) {
+  constexpr int threads = (1 << (2 * SHIFT)) > 512 ? 512 : (1 << (2 * SHIFT));
+  hipLaunchKernelGGL((k_batch_raster<SHIFT, MODE>), dim3(grid), dim3(threads), 0, s, jobs, njobs,
+                     B);
+  return hipGetLastError();
+}
+
+template <int SHIFT, int MODE>
+int raster_occupancy() {
+  constexpr int threads = (1 << (2 * SHIFT)) > 512 ? 512 : (1 << (2 * SHIFT));
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_batch_raster<SHIFT, MODE>, threads,
+                                                   0) != hipSuccess)
+    return 1;
+  return blocks > 0 ? blocks : 1;
+}
+
+}  // namespace
+
+namespace axr {
+
+hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, int njobs, uint32_t blocks,
+                              const AxrBins& B, hipStream_t s) {
+  if (njobs == 0 || blocks == 0) return hipSuccess;
+  switch (bin_shift) {
+    case 3:
+      hipLaunchKernelGGL(k_batch_setup<3>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, njobs, B);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_batch_setup<4>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, njobs, B);
+      break;
+    default:
+      hipLaunchKernelGGL(k_batch_setup<5>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, njobs, B);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_raster(int bin_shift, int ids, const AxrJobDev* jobs, int njobs,
+                               const AxrBins& B, int grid, hipStream_t s) {
+  if (njobs == 0 || B.nb == 0) return hipSuccess;
+  if (bin_shift == 3)
+    return ids ? raster_launch<3, SHADE_IDS>(jobs, njobs, B, grid, s)
+               : raster_launch<3, SHADE_LDR8>(jobs, njobs, B, grid, s);
+  if (bin_shift == 4)
+    return ids ? raster_launch<4, SHADE_IDS>(jobs, njobs, B, grid, s)
+               : raster_launch<4, SHADE_LDR8>(jobs, njobs, B, grid, s);
+  return ids ? raster_launch<5, SHADE_IDS>(jobs, njobs, B, grid, s)
+             : raster_launch<5, SHADE_LDR8>(jobs, njobs, B, grid, s);
+}
+
+int batch_raster_blocks_per_cu(int bin_shift, int ids) {
+  if (bin_shift == 3) return ids ? raster_occupancy<3, SHADE_IDS>() : raster_occupancy<3, SHADE_LDR8>();
+  if (bin_shift == 4) return ids ? raster_occupancy<4, SHADE_IDS>() : raster_occupancy<4, SHADE_LDR8>();
+  return ids ? raster_occupancy<5, SHADE_IDS>() : raster_occupancy<5, SHADE_LDR8>();
+}
+
+}  // namespace axr

@@ -1,0 +1,431 @@
+// batch_api.cpp -- C ABI of batched rendering (declared in include/axr.h).
+//
+// A batch renders jobs -- a resident scene (an axr_ctx holding geometry,
+// textures, lights and shadow maps) seen by one camera at one resolution --
+// with three launches on the batch's own stream (batch.hip): setup of every
+// (job, triangle), one order pass over every job's bins, and one persistent
+// raster that drains the heavy-first bin queue.  Jobs may mix scenes and
+// resolutions.  The launch sequence can be captured once into a hipGraph
+// and replayed while the layout (job count, sizes, scenes, bin size) stays
+// the same; cameras are re-uploaded before every replay.
+//
+// Several batches may be in flight at once (each has its own stream and
+// buffers); that is how the render server overlaps batches on the device.
+#include "axr_ctx.h"
+
+namespace {
+
+constexpr uint32_t kBatchSpill = 1u << 20;
+
+struct JobInfo {
+  axr_ctx* scene;
+  int W, H;
+  size_t out_off;  // pixel offset of this job in the output pools
+};
+
+// Default bucket capacity per bin for a bin of 2^shift pixels on a side:
+// the single-frame capacities scaled by bin area.
+uint32_t default_cap(int shift, bool large) {
+  const uint32_t base = large ? 1024u : 4096u;  // for 32-px bins
+  const int scale = 2 * (5 - shift);
+  uint32_t c = base >> scale;
+  return c < 64u ? 64u : c;
+}
+
+}  // namespace
+
+struct axr_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int bin_shift = 5;
+  int occupancy[3][2] = {};  // raster blocks per CU by (bin shift - 3, ids)
+  BinStore bins;
+  uint32_t* d_head = nullptr;
+  uint32_t* d_counters = nullptr;
+  AxrTriRec* d_rec = nullptr;
+  uint8_t* d_cls = nullptr;
+  size_t rec_cap = 0;
+  float* d_depth = nullptr;
+  uint32_t* d_rgba = nullptr;
+  uint32_t* d_ids = nullptr;
+  size_t px_cap = 0;
+  AxrJobDev* d_jobs = nullptr;
+  AxrJobDev* h_jobs = nullptr;  // pinned staging of the job table
+  int jobs_cap = 0;
+  std::vector<JobInfo> info;
+  int njobs = 0;
+  uint32_t setup_blocks = 0;
+  int nb_total = 0;
+  int ids = 0;
+  hipEvent_t ev_start = nullptr, ev_done = nullptr;
+  hipEvent_t ev_copied = nullptr;  // the job table has left the staging buffer
+  bool rendered = false;
+  // graph replay
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> graph_key;
+};
+
+namespace {
+
+void free_graph(axr_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  b->graph_key.clear();
+}
+
+int grow_pools(axr_batch* b, size_t tris, size_t px, int njobs) {
+  int rc;
+  if (tris > b->rec_cap) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    dfree(b->d_rec);
+    dfree(b->d_cls);
+    if ((rc = dalloc(&b->d_rec, tris)) || (rc = dalloc(&b->d_cls, tris))) return rc;
+    b->rec_cap = tris;
+    free_graph(b);
+  }
+  if (px > b->px_cap) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    dfree(b->d_depth);
+    dfree(b->d_rgba);
+    dfree(b->d_ids);
+    if ((rc = dalloc(&b->d_depth, px)) || (rc = dalloc(&b->d_rgba, px)) ||
+        (rc = dalloc(&b->d_ids, px)))
+      return rc;
+    b->px_cap = px;
+    free_graph(b);
+  }
+  if (njobs > b->jobs_cap) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    dfree(b->d_jobs);
+    if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+    b->h_jobs = nullptr;
+    if ((rc = dalloc(&b->d_jobs, (size_t)njobs))) return rc;
+    HIPCHK(hipHostMalloc((void**)&b->h_jobs, sizeof(AxrJobDev) * (size_t)njobs));
+    b->jobs_cap = njobs;
+    free_graph(b);
+  }
+  return 0;
+}
+
+// Device job descriptor of job j (scene state as it is now).
+AxrJobDev make_job(const axr_batch* b, const axr_job& j, const JobInfo& in, uint32_t slot0,
+                   uint32_t blk0, int nbx, size_t rec_off) {
+  axr_ctx* sc = j.scene;
+  AxrJobDev d{};
+  AxrDrawConst& dc = d.dc;
+  for (int k = 0; k < 3; k++) dc.view.pos[k] = j.pos[k];
+  for (int k = 0; k < 9; k++) dc.view.vm[k] = j.vm[k];
+  dc.view.scale = j.scale;
+  dc.view.cax = j.cax;
+  dc.view.cay = j.cay;
+  for (int k = 0; k < 3; k++) {
+    dc.lint[k] = sc->lint[k];
+    dc.ldir[k] = sc->ldir[k];
+    dc.clear_rgb[k] = sc->clear_rgb[k];
+  }
+  dc.amb = sc->amb;
+  dc.W = in.W;
+  dc.H = in.H;
+  dc.WC = in.W / 16;
+  dc.HC = in.H / 16;
+  dc.T = sc->T;
+  dc.ngroups = (uint32_t)sc->groups.size();
+  dc.pcf_taps = j.pcf_taps;
+  dc.flag_mask = j.shadows ? 0xFFFFFFFFu : ~AXR_SH_SHADOW;
+  dc.nshadow = 0;
+  if (j.shadows) {
+    for (int i = 0; i < AXR_MAX_SHADOWS && dc.nshadow < AXR_SHADOW_BIND; i++) {
+      const HostShadow& s = sc->sm[i];
+      if (!s.exists || !s.placed) continue;
+      AxrShadowDev& sd = dc.sh[dc.nshadow++];
+      sd.map = s.d_map;
+      sd.dim = s.dim;
+      sd.wS = s.dim / 2;
+      sd.cam = s.cam;
+    }
+  }
+  d.vpos = sc->d_vpos;
+  d.vattr = sc->d_vattr;
+  d.groups = sc->d_groups;
+  d.rec = b->d_rec + rec_off;
+  d.cls = b->d_cls + rec_off;
+  d.depth = b->d_depth + in.out_off;
+  d.rgba = b->d_rgba + in.out_off;
+  d.ids = b->d_ids + in.out_off;
+  d.exposure = j.exposure;
+  d.slot0 = slot0;
+  d.blk0 = blk0;
+  d.nbx = nbx;
+  return d;
+}
+
+int enqueue(axr_batch* b) {
+  AxrBins B = b->bins.B;
+  B.head = b->d_head;
+  int& occ = b->occupancy[b->bin_shift - 3][b->ids];
+  if (!occ) occ = batch_raster_blocks_per_cu(b->bin_shift, b->ids);
+  const int grid_max = b->cu_count * occ;
+  const int grid = b->nb_total < grid_max ? b->nb_total : grid_max;
+  HIPCHK(launch_batch_setup(b->bin_shift, b->d_jobs, b->njobs, b->setup_blocks, B, b->stream));
+  HIPCHK(launch_bin_order(&B, nullptr, b->d_counters, b->stream));
+  HIPCHK(launch_batch_raster(b->bin_shift, b->ids, b->d_jobs, b->njobs, B, grid, b->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int axr_batch_create(axr_batch** out) {
+  if (!out) return fail(AXR_EINVAL, "null out pointer");
+  *out = nullptr;
+  axr_batch* b = new axr_batch();
+  auto bail = [&](int r) {
+    axr_batch_destroy(b);
+    return r;
+  };
+  if (hipGetDevice(&b->device) != hipSuccess) return bail(fail(AXR_EHIP, "no device"));
+  if (hipDeviceGetAttribute(&b->cu_count, hipDeviceAttributeMultiprocessorCount, b->device) !=
+      hipSuccess)
+    return bail(fail(AXR_EHIP, "device attribute query failed"));
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev_start) != hipSuccess || hipEventCreate(&b->ev_done) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_copied, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(AXR_EHIP, "stream/event creation failed"));
+  int rc;
+  if ((rc = dalloc(&b->d_head, 1)) || (rc = dalloc(&b->d_counters, AXR_CTR_WORDS))) return bail(rc);
+  if (launch_fill_u32(b->d_counters, AXR_CTR_WORDS, 0, b->stream) != hipSuccess ||
+      launch_fill_u32(b->d_head, 1, 0, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return bail(fail(AXR_EHIP, "initial fill failed"));
+  b->bins.nseg = 2;
+  b->bins.want_spill = kBatchSpill;
+  b->bins.want0 = default_cap(b->bin_shift, false);
+  b->bins.want1 = default_cap(b->bin_shift, true);
+  *out = b;
+  return AXR_OK;
+}
+
+int axr_batch_destroy(axr_batch* b) {
+  if (!b) return AXR_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  free_graph(b);
+  bins_free(b->bins);
+  dfree(b->d_head);
+  dfree(b->d_counters);
+  dfree(b->d_rec);
+  dfree(b->d_cls);
+  dfree(b->d_depth);
+  dfree(b->d_rgba);
+  dfree(b->d_ids);
+  dfree(b->d_jobs);
+  if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+  if (b->ev_start) (void)hipEventDestroy(b->ev_start);
+  if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+  if (b->ev_copied) (void)hipEventDestroy(b->ev_copied);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return AXR_OK;
+}
+
+int axr_batch_configure(axr_batch* b, int bin_size, uint32_t cap_small, uint32_t cap_large,
+                        int use_graph) {
+  if (!b) return fail(AXR_EINVAL, "null batch");
+  int shift;
+  if (bin_size == 8)
+    shift = 3;
+  else if (bin_size == 16)
+    shift = 4;
+  else if (bin_size == 32)
+    shift = 5;
+  else
+    return fail(AXR_EINVAL, "bin size must be 8, 16 or 32 (got %d)", bin_size);
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->bin_shift = shift;
+  b->bins.want0 = cap_small ? cap_small : default_cap(shift, false);
+  b->bins.want1 = cap_large ? cap_large : default_cap(shift, true);
+  b->use_graph = use_graph != 0;
+  free_graph(b);
+  return AXR_OK;
+}
+
+int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int want_ids) {
+  if (!b || (!jobs && njobs)) return fail(AXR_EINVAL, "batch render: bad arguments");
+  if (njobs < 0) return fail(AXR_EINVAL, "negative job count");
+  HIPCHK(hipSetDevice(b->device));
+  for (int i = 0; i < njobs; i++) {
+    const axr_job& j = jobs[i];
+    if (!j.scene) return fail(AXR_EINVAL, "job %d: null scene", i);
+    if (j.width <= 0 || j.height <= 0 || (j.width % 16) != 0)
+      return fail(AXR_EINVAL, "job %d: width must be a positive multiple of 16", i);
+    if (j.pcf_taps != 5 && j.pcf_taps != 9) return fail(AXR_EINVAL, "job %d: pcf_taps 5 or 9", i);
+    if (j.scene->device != b->device) return fail(AXR_EINVAL, "job %d: scene on another device", i);
+  }
+  int rc;
+  // Scenes: run pending shadow passes and clears, then order the batch
+  // stream after the scene's stream.
+  std::vector<axr_ctx*> scenes;
+  for (int i = 0; i < njobs; i++) {
+    axr_ctx* sc = jobs[i].scene;
+    bool seen = false;
+    for (auto* s : scenes) seen |= s == sc;
+    if (seen) continue;
+    scenes.push_back(sc);
+    if ((rc = flush_shadows(sc))) return rc;
+    for (int m = 0; m < AXR_MAX_SHADOWS; m++)
+      if (sc->sm[m].exists && sc->sm[m].placed && (rc = apply_shadow_clear(sc, m))) return rc;
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, sc->stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, ev, 0));
+    HIPCHK(hipEventDestroy(ev));
+  }
+  // Layout: bins, setup blocks, record and pixel offsets, in job order.
+  const int bin = 1 << b->bin_shift;
+  b->info.assign((size_t)njobs, JobInfo{});
+  size_t tris = 0, px = 0;
+  uint32_t slots = 0, blocks = 0;
+  std::vector<uint32_t> slot0((size_t)njobs), blk0((size_t)njobs);
+  std::vector<int> nbx((size_t)njobs);
+  std::vector<size_t> rec_off((size_t)njobs);
+  for (int i = 0; i < njobs; i++) {
+    const axr_job& j = jobs[i];
+    JobInfo& in = b->info[(size_t)i];
+    in.scene = j.scene;
+    in.W = j.width;
+    in.H = j.height;
+    in.out_off = px;
+    px += (size_t)j.width * j.height;
+    nbx[(size_t)i] = (j.width + bin - 1) / bin;
+    slot0[(size_t)i] = slots;
+    slots += (uint32_t)(nbx[(size_t)i] * ((j.height + bin - 1) / bin));
+    blk0[(size_t)i] = blocks;
+    blocks += (j.scene->T + 255u) / 256u;
+    rec_off[(size_t)i] = tris;
+    tris += j.scene->T;
+  }
+  b->njobs = njobs;
+  b->setup_blocks = blocks;
+  b->nb_total = (int)slots;
+  b->ids = want_ids ? 1 : 0;
+  if ((rc = grow_pools(b, tris ? tris : 1, px ? px : 1, njobs ? njobs : 1))) return rc;
+  const AxrBins before = b->bins.B;
+  if ((rc = bins_prepare(b->bins, slots ? (int)slots : 1, b->stream))) return rc;
+  b->bins.B.nb = (int)slots;
+  if (before.ent != b->bins.B.ent || before.cnt != b->bins.B.cnt ||
+      before.spill != b->bins.B.spill)
+    free_graph(b);
+  // the previous render's table copy must be done before the staging is rewritten
+  if (b->rendered) HIPCHK(hipEventSynchronize(b->ev_copied));
+  for (int i = 0; i < njobs; i++)
+    b->h_jobs[i] = make_job(b, jobs[i], b->info[(size_t)i], slot0[(size_t)i], blk0[(size_t)i],
+                            nbx[(size_t)i], rec_off[(size_t)i]);
+  b->rendered = true;
+  HIPCHK(hipEventRecord(b->ev_start, b->stream));
+  if (njobs) {
+    HIPCHK(hipMemcpyAsync(b->d_jobs, b->h_jobs, sizeof(AxrJobDev) * (size_t)njobs,
+                          hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipEventRecord(b->ev_copied, b->stream));
+    if (b->use_graph) {
+      // The graph holds the three launches; its arguments depend on this key.
+      std::vector<uint64_t> key = {(uint64_t)njobs, blocks, slots, (uint64_t)b->bin_shift,
+                                   (uint64_t)b->ids, b->bins.B.cap0, b->bins.B.cap1,
+                                   (uint64_t)(uintptr_t)b->bins.B.ent,
+                                   (uint64_t)(uintptr_t)b->d_jobs};
+      if (!b->gexec || key != b->graph_key) {
+        free_graph(b);
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue(b);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        if (e != hipSuccess) return fail(AXR_EHIP, "graph capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(AXR_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+        b->graph_key = key;
+      }
+      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    } else if ((rc = enqueue(b))) {
+      return rc;
+    }
+  }
+  HIPCHK(hipEventRecord(b->ev_done, b->stream));
+  return AXR_OK;
+}
+
+int axr_batch_query(axr_batch* b) {
+  if (!b) return fail(AXR_EINVAL, "null batch");
+  const hipError_t e = hipEventQuery(b->ev_done);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(AXR_EHIP, "batch query: %s", hipGetErrorString(e));
+}
+
+int axr_batch_wait(axr_batch* b, float* gpu_ms) {
+  if (!b) return fail(AXR_EINVAL, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipEventSynchronize(b->ev_done));
+  if (gpu_ms) {
+    *gpu_ms = 0.0f;
+    if (b->rendered) HIPCHK(hipEventElapsedTime(gpu_ms, b->ev_start, b->ev_done));
+  }
+  uint32_t h[AXR_CTR_WORDS];
+  HIPCHK(hipMemcpy(h, b->d_counters, sizeof h, hipMemcpyDeviceToHost));
+  if (h[AXR_CTR_BIN_SPILL] || h[AXR_CTR_LOST]) {
+    bins_grow(b->bins, h[AXR_CTR_BIN_SPILL], h[AXR_CTR_LOST]);
+    const uint32_t zero[AXR_CTR_WORDS] = {};
+    HIPCHK(hipMemcpy(b->d_counters, zero, sizeof zero, hipMemcpyHostToDevice));
+    if (h[AXR_CTR_LOST])
+      return fail(AXR_ECAP, "%u bin entries did not fit the spill list; it has been grown, "
+                            "render the batch again",
+                  h[AXR_CTR_LOST]);
+  }
+  return AXR_OK;
+}
+
+static int batch_out(axr_batch* b, int job, const void* src, size_t elem, void* dst) {
+  if (!b || !dst) return fail(AXR_EINVAL, "batch readback: bad arguments");
+  if (job < 0 || job >= b->njobs) return fail(AXR_EINVAL, "no job %d in the batch", job);
+  int rc;
+  if ((rc = axr_batch_wait(b, nullptr))) return rc;
+  const JobInfo& in = b->info[(size_t)job];
+  const size_t n = (size_t)in.W * in.H;
+  HIPCHK(hipMemcpy(dst, (const char*)src + in.out_off * elem, n * elem, hipMemcpyDeviceToHost));
+  return AXR_OK;
+}
+
+int axr_batch_get_rgba8(axr_batch* b, int job, uint32_t* out) {
+  if (b && b->ids) return fail(AXR_ESTATE, "the batch was rendered for ids");
+  return batch_out(b, job, b ? b->d_rgba : nullptr, 4, out);
+}
+
+int axr_batch_get_depth(axr_batch* b, int job, float* out) {
+  return batch_out(b, job, b ? b->d_depth : nullptr, 4, out);
+}
+
+int axr_batch_get_ids(axr_batch* b, int job, uint32_t* out) {
+  if (b && !b->ids) return fail(AXR_ESTATE, "the batch was rendered for colour");
+  return batch_out(b, job, b ? b->d_ids : nullptr, 4, out);
+}
+
+int axr_batch_get_bytes(axr_batch* b, uint64_t* out) {
+  if (!b || !out) return fail(AXR_EINVAL, "batch bytes: bad arguments");
+  uint64_t n = 0;
+  n += (uint64_t)b->rec_cap * (sizeof(AxrTriRec) + 1);
+  n += (uint64_t)b->px_cap * 12;
+  n += (uint64_t)b->bins.ent_alloc * 4 + (uint64_t)b->bins.nb_alloc * 4 * 5 +
+       (uint64_t)b->bins.B.spill_cap * 8;
+  n += (uint64_t)b->jobs_cap * sizeof(AxrJobDev);
+  *out = n;
+  return AXR_OK;
+}
+
+}  // extern "C"

@@ -20,81 +20,13 @@
 // Appends aggregate within a wave: lanes of one wave come from one mesh
 // region and mostly share a bin, so one atomic serves all of them.  Depth
 // minima live in LDS; no global atomics touch pixels or texels.
-#include "axr_launch.h"
-#include "axr_tri.h"
+#include "axr_bin.h"
 
 using namespace axr;
 
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ __forceinline__ int find_group(const AxrGroupDev* __restrict__ groups, int ng,
-                                          uint32_t gi) {
-  int lo = 0, hi = ng - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (groups[mid].base <= gi)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ uint64_t lanemask_lt() {
-  const unsigned lane = __lane_id();
-  return lane ? (~0ull >> (64 - lane)) : 0ull;
-}
-
-// Adds one per active lane to ctr[bin] with one atomic per distinct bin in
-// the wave; returns the counter's previous value plus the lane's rank among
-// the lanes that share its bin.  All lanes of the wave must call it.
-__device__ __forceinline__ uint32_t wave_bin_add(uint32_t* __restrict__ ctr, bool active,
-                                                 uint32_t bin) {
-  uint64_t pending = __ballot(active);
-  uint32_t result = 0;
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t b = __shfl(bin, leader);
-    const uint64_t grp = __ballot(active && bin == b) & pending;
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(&ctr[b], (uint32_t)__popcll(grp));
-    base = __shfl(base, leader);
-    if ((grp >> __lane_id()) & 1ull) result = base + (uint32_t)__popcll(grp & lanemask_lt());
-    pending &= ~grp;
-  }
-  return result;
-}
-
-// Bin rectangle of a pixel rectangle clipped to [0,w) x [0,h).
-struct BinRect {
-  int bx0, by0, nbx, n;
-};
-__device__ __forceinline__ BinRect bin_rect(int x0, int y0, int x1, int y1, int w, int h) {
-  BinRect r = {0, 0, 1, 0};
-  x0 = imax(x0, 0);
-  y0 = imax(y0, 0);
-  x1 = imin(x1, w - 1);
-  y1 = imin(y1, h - 1);
-  if (x0 <= x1 && y0 <= y1) {
-    r.bx0 = x0 >> AXR_BIN_SHIFT;
-    r.by0 = y0 >> AXR_BIN_SHIFT;
-    r.nbx = (x1 >> AXR_BIN_SHIFT) - r.bx0 + 1;
-    r.n = r.nbx * ((y1 >> AXR_BIN_SHIFT) - r.by0 + 1);
-  }
-  return r;
-}
-__device__ __forceinline__ uint32_t rect_bin(const BinRect& r, int k, int nbx_total) {
-  return (uint32_t)((r.by0 + k / r.nbx) * nbx_total + r.bx0 + k % r.nbx);
-}
-
-// 32-px bin and quadrant of a reference 16-px tile index.
-__device__ __forceinline__ void tile_to_bin(int t, int WC, int NBX, uint32_t& bin, uint32_t& sub) {
-  const int tx = t % WC, ty = t / WC;
-  bin = (uint32_t)((ty >> 1) * NBX + (tx >> 1));
-  sub = (uint32_t)(((ty & 1) << 1) | (tx & 1));
-}
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -140,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
     if (cls == 1) {
       int x0, y0, x1, y1;
       small_bounds(r.px, r.py, x0, y0, x1, y1);
-      br = bin_rect(x0, y0, x1, y1, dc.W, dc.H);
+      br = bin_rect(x0, y0, x1, y1, dc.W, dc.H, AXR_BIN_SHIFT);
     }
     for (int k = 0;; k++) {
       const bool act = k < br.n;
@@ -151,10 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
     }
     if (cls == 2) {
       coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
-        uint32_t bin, sub;
-        tile_to_bin(t, dc.WC, NBX, bin, sub);
-        const uint32_t slot = (uint32_t)MB.nb + bin;
-        bin_store(MB, slot, atomicAdd(&MB.cnt[slot], 1u), (gi << 2) | sub);
+        append_tile<AXR_BIN_SHIFT>(MB, (uint32_t)MB.nb, t, dc.WC, NBX, gi);
       });
     }
   }
@@ -173,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
     if (ok) {
       int x0, y0, x1, y1;
       shadow_bounds(r, x0, y0, x1, y1);
-      br = bin_rect(x0, y0, x1, y1, P.dim, P.dim);
+      br = bin_rect(x0, y0, x1, y1, P.dim, P.dim, AXR_BIN_SHIFT);
     }
     for (int k = 0;; k++) {
       const bool act = k < br.n;
@@ -275,6 +204,7 @@ __global__ __launch_bounds__(1024) void k_bin_order(OrderJob j0, OrderJob j1,
     const uint32_t used = B.spill_ctr[0];
     B.spill_ctr[1] = used < B.spill_cap ? used : B.spill_cap;
     B.spill_ctr[0] = 0;
+    if (B.head) *B.head = 0;
     if (used) atomicMax(&ctr[second ? j1.ctr_spill : j0.ctr_spill], used);
     if (used > B.spill_cap) atomicAdd(&ctr[AXR_CTR_LOST], used - B.spill_cap);
   }

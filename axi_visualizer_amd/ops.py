@@ -282,6 +282,19 @@ class CLDraw:
                 "bin_entries": int(s[3]), "bin_overflow": int(s[4]),
                 "shadow_bin_entries": int(s[5])}
 
+    def setShaders(self, shaders, mask=None):
+        """drawAll's shader binding without drawing: prepares this context
+        as a resident scene for batches."""
+        ng = len(self.gSize)
+        f = np.array([shader_flags(shaders[tn]) for tn in range(ng)], np.uint32)
+        m = np.array([1 if (mask is not None and mask[tn]) else 0 for tn in range(ng)], np.uint8)
+        self._chk(self._L.axr_set_shaders(self._ctx, ptr(f), ptr(m), ng), "setShaders")
+
+    def deviceBytes(self) -> int:
+        n = ctypes.c_uint64()
+        self._chk(self._L.axr_get_bytes(self._ctx, ctypes.byref(n)), "deviceBytes")
+        return int(n.value)
+
     def setBinCapacity(self, small=0, large=0, shadow=0, spill=0):
         """Bucket capacities (entries per bin) and spill-list size; 0 keeps."""
         self._chk(self._L.axr_set_bin_capacity(self._ctx, int(small), int(large), int(shadow),

@@ -14,24 +14,31 @@ from .ops import CLDraw
 from .scenes import Scene
 
 
+def load_scene(d: CLDraw, sc: Scene):
+    """Uploads a Scene into a CLDraw the way the reference engine's start-up
+    does: groups with their textures, materials, the primary light, vertex
+    light, shadow maps placed."""
+    d.setScaleCull(sc.view.scale, sc.view.cax, sc.view.cay)
+    for g in sc.groups:
+        planes = [g.tex[:, c] if g.mip else g.tex[:, c].reshape(g.tex_side, g.tex_side)
+                  for c in range(3)]
+        tn = d.addTextureGroup(g.xyz, g.uv, g.vn, *planes,
+                               mip=g.tex_side if g.mip else False)
+        d.setGroupMaterial(tn, g.mip_bias, g.spec_pow)
+    d.ambLight = np.float32(sc.amb)
+    d.setPrimaryLight(sc.light_int[None], sc.light_dir[None])
+    # one directional light: vertL leaves I at the ambient floor
+    d.vertLight([True] * len(sc.groups), sc.light_int[None], sc.light_dir[None])
+    for i, c in enumerate(sc.cascades):
+        d.addShadowMap(i, c.dim, c.cam.scale, sc.amb)
+        d.placeShadowMap(i, c.cam.pos, c.cam.vm, sc.amb)
+
+
 class Renderer:
     def __init__(self, scene: Scene):
         self.scene = sc = scene
         self.draw = d = CLDraw(1, len(sc.groups), sc.W, sc.H, 1)
-        d.setScaleCull(sc.view.scale, sc.view.cax, sc.view.cay)
-        for g in sc.groups:
-            planes = [g.tex[:, c] if g.mip else g.tex[:, c].reshape(g.tex_side, g.tex_side)
-                      for c in range(3)]
-            tn = d.addTextureGroup(g.xyz, g.uv, g.vn, *planes,
-                                   mip=g.tex_side if g.mip else False)
-            d.setGroupMaterial(tn, g.mip_bias, g.spec_pow)
-        d.ambLight = np.float32(sc.amb)
-        d.setPrimaryLight(sc.light_int[None], sc.light_dir[None])
-        # one directional light: vertL leaves I at the ambient floor
-        d.vertLight([True] * len(sc.groups), sc.light_int[None], sc.light_dir[None])
-        for i, c in enumerate(sc.cascades):
-            d.addShadowMap(i, c.dim, c.cam.scale, sc.amb)
-            d.placeShadowMap(i, c.cam.pos, c.cam.vm, sc.amb)
+        load_scene(d, sc)
         self.shaders = [g.flags for g in sc.groups]
         self.mask = [g.skip for g in sc.groups]
         self.cast = [g.cast for g in sc.groups]

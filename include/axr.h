@@ -142,6 +142,56 @@ int axr_set_timing(axr_ctx* ctx, int enable);
 int axr_get_timing(axr_ctx* ctx, char* names, int name_len, double* total_ms, int64_t* count,
                    int n);
 
+/* Shader binding of drawAll (flags per group, mask = groups not drawn)
+ * without drawing: prepares a context used as a resident scene by batches. */
+int axr_set_shaders(axr_ctx* ctx, const uint32_t* shader_flags, const uint8_t* mask,
+                    int ngroups);
+
+/* Device bytes a context holds (geometry, textures, records, bins, frame). */
+int axr_get_bytes(axr_ctx* ctx, uint64_t* out);
+
+/* ------------------------------------------------------------ batches
+ * No reference counterpart.  A batch renders many jobs -- a context used as
+ * a resident scene, seen by one camera at one resolution -- with one setup
+ * launch, one order launch and one persistent raster launch for all of
+ * them, on the batch's own stream.  Each job's pixels are bit-identical to
+ * the same view drawn alone through axr_draw_all.  Batches may be in flight
+ * together; a scene must not be changed while a batch that uses it runs. */
+typedef struct axr_batch axr_batch;
+
+typedef struct axr_job {
+  axr_ctx* scene;         /* groups, shader flags, lights, shadow maps */
+  float pos[3];           /* camera position (setPos) */
+  float vm[9];            /* view basis rows (setVM) */
+  float scale, cax, cay;  /* setScaleCull for this resolution */
+  int width, height;      /* width a positive multiple of 16 */
+  int shadows;            /* sample the scene's first two placed shadow maps */
+  int pcf_taps;           /* 5 (reference cross) or 9 (3x3) */
+  float exposure;         /* gamma exposure of the RGBA8 output */
+} axr_job;
+
+int axr_batch_create(axr_batch** out);
+int axr_batch_destroy(axr_batch* b);
+/* Bin size 8, 16 or 32 pixels (a scheduling unit; results do not depend on
+ * it), bucket capacities per bin (0: default for the size), and whether
+ * the launch sequence is captured into a hipGraph and replayed while the
+ * batch layout is unchanged. */
+int axr_batch_configure(axr_batch* b, int bin_size, uint32_t cap_small, uint32_t cap_large,
+                        int use_graph);
+/* Enqueues the jobs; returns without waiting.  want_ids: winner ids +
+ * depth instead of RGBA8 + depth. */
+int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int want_ids);
+/* 1 when the last render has finished, 0 while it runs. */
+int axr_batch_query(axr_batch* b);
+/* Waits for the last render; GPU time of it in *gpu_ms (may be NULL).
+ * AXR_ECAP when bin entries were lost (render again: storage has grown). */
+int axr_batch_wait(axr_batch* b, float* gpu_ms);
+int axr_batch_get_rgba8(axr_batch* b, int job, uint32_t* out);
+int axr_batch_get_depth(axr_batch* b, int job, float* out);
+int axr_batch_get_ids(axr_batch* b, int job, uint32_t* out);
+/* Device bytes the batch holds (records, bins, outputs). */
+int axr_batch_get_bytes(axr_batch* b, uint64_t* out);
+
 #ifdef __cplusplus
 }
 #endif
