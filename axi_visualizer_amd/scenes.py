@@ -265,24 +265,41 @@ def terrain_mesh(heights: np.ndarray, spacing: float, uvspread: float = 2.0):
 
 def terrain_scene(W=640, H=480, grid=128, extent=100.0, seed=1, tex_side=256,
                   mip=True, cam_dist=80.0, cam_yaw=pi / 4, cam_pitch=pi / 6,
-                  flags=SH_MIP | SH_PHONG, name="c1_terrain") -> Scene:
+                  flags=SH_MIP | SH_PHONG, name="c1_terrain", n_tex=1,
+                  shadow_dim=0) -> Scene:
     """Heightmap terrain lit by one directional light (config 1 at defaults:
-    128x128 grid = 32,258 triangles, 640x480, 30 degree pitch)."""
+    128x128 grid = 32,258 triangles, 640x480, 30 degree pitch).  n_tex > 1
+    splits the triangles into that many groups of contiguous rows, each with
+    its own texture; shadow_dim > 0 adds two shadow cascades and the shadow
+    shader flag."""
     t = np.arange(grid) / (grid - 1)
     xs, zs = np.meshgrid(t, t, indexing="ij")
     h = value_noise(seed, xs, zs)
     spacing = extent / (grid - 1)
     xyz, vn, uv = terrain_mesh(h, spacing)
-    tex, side = texture_for(noise_texture(seed, 77, tex_side), mip)
-    g = Group(xyz=xyz, vn=vn, uv=uv, tex=tex, tex_side=side, mip=mip,
-              flags=flags if mip else flags & ~SH_MIP)
+    if shadow_dim:
+        flags |= SH_SHADOW
+    T = xyz.shape[0] // 3
+    cuts = [T * k // n_tex for k in range(n_tex + 1)]
+    groups = []
+    for k in range(n_tex):
+        a, b = 3 * cuts[k], 3 * cuts[k + 1]
+        tex, side = texture_for(noise_texture(seed, 77 + k, tex_side), mip)
+        groups.append(Group(xyz=np.ascontiguousarray(xyz[a:b]), vn=np.ascontiguousarray(vn[a:b]),
+                            uv=np.ascontiguousarray(uv[a:b]), tex=tex, tex_side=side, mip=mip,
+                            flags=flags if mip else flags & ~SH_MIP))
     centre = np.array([extent / 2, float(h.mean()), extent / 2])
     pos = centre - cam_dist * view_vec(cam_yaw, cam_pitch)
     la, lb = 0.5, 0.9
-    return Scene(name=name, W=W, H=H, groups=[g],
+    cascades = []
+    if shadow_dim:
+        cascades = [make_cascade(shadow_dim, centre, extent * 0.6, la, lb),
+                    make_cascade(shadow_dim, centre, extent * 1.5, la, lb)]
+    return Scene(name=name, W=W, H=H, groups=groups,
                  view=make_camera(W, H, pos, cam_yaw, cam_pitch),
                  light_int=np.array([0.9, 0.85, 0.8], np.float32),
-                 light_dir=view_vec(la, lb).astype(np.float32), amb=0.2)
+                 light_dir=view_vec(la, lb).astype(np.float32), amb=0.2, cascades=cascades,
+                 pcf_taps=9 if shadow_dim else 5)
 
 
 # -------------------------------------------------------------- spheres
@@ -441,3 +458,40 @@ def random_tris_scene(W=256, H=192, ntri=400, seed=3, size=30.0, depth=(5.0, 40.
                  view=make_camera(W, H, [0.0, 0.0, 0.0], 0.0, 0.0, fovx=90.0),
                  light_int=np.array([0.9, 0.85, 0.8], np.float32),
                  light_dir=view_vec(0.4, 0.7).astype(np.float32), amb=0.2)
+
+
+# -------------------------------------------------------------- serving mix
+def serving_scenes(n: int = 16, seed: int = 1, tex_side: int = 1024, shadow_dim: int = 2048,
+                   tri_range=(50_000, 1_000_000)):
+    """The serving workload's resident scenes: n scenes of 50k-1M triangles,
+    alternately terrain grids and icosphere instances, each with 1-4 seeded
+    mipmapped textures and two shadow cascades.  Returns [(scene, kind)]."""
+    u = rand_uniform(seed, 5001, 2 * n).reshape(n, 2)
+    out = []
+    for i in range(n):
+        target = int(tri_range[0] + u[i, 0] * (tri_range[1] - tri_range[0]))
+        ntex = 1 + min(3, int(u[i, 1] * 4))
+        if i % 2 == 0:
+            grid = max(8, int(round(sqrt(target / 2.0))) + 1)
+            sc = terrain_scene(grid=grid, tex_side=tex_side, seed=seed + i, n_tex=ntex,
+                               shadow_dim=shadow_dim, name=f"serve{i}_terrain")
+            out.append((sc, "terrain"))
+        else:
+            sc = spheres_scene(n_inst=max(1, target // 500), n_tex=ntex, tex_side=tex_side,
+                               seed=seed + i, shadow_dim=shadow_dim, name=f"serve{i}_spheres")
+            out.append((sc, "spheres"))
+    return out
+
+
+def serving_camera(kind: str, w: int, h: int, u) -> Camera:
+    """A request's camera for a scene of the given kind; u: 4 uniforms."""
+    if kind == "terrain":
+        centre = np.array([50.0, 0.0, 50.0])
+        yaw = 2 * pi * u[0]
+        pitch = 0.35 + 0.4 * u[1]
+        pos = centre - (40.0 + 80.0 * u[2]) * view_vec(yaw, pitch)
+        return make_camera(w, h, pos, yaw, pitch)
+    yaw = 2 * pi * u[0]
+    pitch = -0.2 + 0.7 * u[1]
+    pos = -(150.0 + 200.0 * u[2]) * view_vec(yaw, pitch)
+    return make_camera(w, h, pos, yaw, pitch)

@@ -1,0 +1,220 @@
+"""Render server: resident scenes, HBM-budget admission, continuous batching.
+
+Scenes are made resident once (``SceneContext``: geometry, textures, shader
+binding, lights and their shadow maps stay in HBM).  Requests -- a scene, a
+camera, a resolution, shadows on or off -- are queued as they arrive.  The
+serving loop keeps several batch slots (each its own stream and buffers, so
+batches overlap on the device); whenever a slot is free it forms a batch
+from the queue, oldest first, admitting a request only while the device
+bytes of the resident scenes, the batches in flight and the new batch stay
+within the HBM budget, and launches it without waiting.  Finished slots are
+polled, their requests completed and their memory released, so requests
+that arrive while others render are taken as slots and memory free up.
+
+A request's pixels do not depend on what else is in its batch: every job is
+bit-identical to the same view rendered alone.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from .batch import Batch, Job, SceneContext, settle
+from .scenes import Camera
+
+# Default bucket capacities of a 32-px bin (entries) used for admission estimates.
+_CAP_SMALL, _CAP_LARGE = 4096, 1024
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    scene: int
+    camera: Camera
+    width: int
+    height: int
+    shadows: bool = False
+    pcf_taps: int = 9
+    arrival: float = 0.0       # seconds on the server clock
+    start: float = 0.0         # launched
+    done: float = 0.0          # completion observed
+    batch_size: int = 0
+    rgba: Optional[np.ndarray] = None
+
+    @property
+    def latency_ms(self) -> float:
+        return (self.done - self.arrival) * 1e3
+
+
+def request_bytes(ntri: int, w: int, h: int, bin_size: int = 32) -> int:
+    """Device bytes a request holds while in flight: its setup records
+    (37 B per scene triangle), outputs (12 B per pixel) and bin buckets."""
+    nb = ((w + bin_size - 1) // bin_size) * ((h + bin_size - 1) // bin_size)
+    return ntri * 37 + w * h * 12 + nb * (_CAP_SMALL + _CAP_LARGE) * 4 + nb * 20
+
+
+class _Slot:
+    def __init__(self, bin_size: int, graph: bool):
+        self.batch = Batch(bin_size=bin_size, graph=graph)
+        self.reqs: List[Request] = []
+        self.bytes = 0
+
+    @property
+    def busy(self) -> bool:
+        return bool(self.reqs)
+
+
+class RenderServer:
+    """Continuous-batching render loop over resident scenes."""
+
+    def __init__(self, scenes: Sequence, hbm_budget: float = 200e9, slots: int = 3,
+                 max_batch: int = 64, window_ms: float = 2.0, bin_size: int = 32,
+                 readback: bool = False, graph: bool = False):
+        self.contexts: List[SceneContext] = [SceneContext(s) for s in scenes]
+        self.ntri = [c.scene.ntri for c in self.contexts]
+        self.resident = sum(c.device_bytes() for c in self.contexts)
+        self.budget = float(hbm_budget)
+        if self.resident > self.budget:
+            raise ValueError(f"resident scenes need {self.resident} B, budget {self.budget:.0f}")
+        self.slots = [_Slot(bin_size, graph) for _ in range(slots)]
+        self.max_batch = max_batch
+        self.window = window_ms * 1e-3
+        self.bin_size = bin_size
+        self.readback = readback
+        self.queue: collections.deque = collections.deque()
+        self.completed: List[Request] = []
+        self.rejected: List[Request] = []
+        self.max_in_flight = 0     # bytes, for the admission check
+        self.max_busy = 0          # batches on the device at once
+        self.t0 = time.perf_counter()
+
+    # -- clock
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def in_flight_bytes(self) -> int:
+        return sum(s.bytes for s in self.slots)
+
+    def submit(self, req: Request):
+        need = request_bytes(self.ntri[req.scene], req.width, req.height, self.bin_size)
+        if self.resident + need > self.budget:
+            self.rejected.append(req)   # can never fit
+            return
+        self.queue.append(req)
+
+    # -- loop pieces
+    def _poll(self):
+        for s in self.slots:
+            if s.busy and s.batch.done():
+                self._finish(s)
+
+    def _finish(self, s: _Slot):
+        try:
+            s.batch.wait()
+        except Exception:
+            # bin storage grew after a loss: render the same jobs again
+            settle(s.batch, s.batch.jobs)
+        t = self.now()
+        for k, r in enumerate(s.reqs):
+            if self.readback:
+                r.rgba = s.batch.rgba8(k)
+            r.done = t
+            self.completed.append(r)
+        s.reqs = []
+        s.bytes = 0
+
+    def _launch(self, s: _Slot) -> bool:
+        """Forms a batch from the queue head under the budget; False if none."""
+        if not self.queue:
+            return False
+        oldest = self.queue[0].arrival
+        if len(self.queue) < self.max_batch and self.now() - oldest < self.window and \
+                any(x.busy for x in self.slots):
+            return False  # let the batch fill while the device is busy
+        free = self.budget - self.resident - self.in_flight_bytes()
+        take, used = [], 0
+        while self.queue and len(take) < self.max_batch:
+            r = self.queue[0]
+            need = request_bytes(self.ntri[r.scene], r.width, r.height, self.bin_size)
+            if used + need > free:
+                break
+            take.append(self.queue.popleft())
+            used += need
+        if not take:
+            return False
+        jobs = [Job(self.contexts[r.scene], r.camera, r.width, r.height, shadows=r.shadows,
+                    pcf_taps=r.pcf_taps) for r in take]
+        t = self.now()
+        for r in take:
+            r.start = t
+            r.batch_size = len(take)
+        s.batch.render(jobs)
+        s.reqs, s.bytes = take, used
+        self.max_in_flight = max(self.max_in_flight, self.in_flight_bytes())
+        self.max_busy = max(self.max_busy, sum(1 for x in self.slots if x.busy))
+        return True
+
+    def step(self):
+        self._poll()
+        for s in self.slots:
+            if not s.busy:
+                self._launch(s)
+
+    def drain(self):
+        while self.queue or any(s.busy for s in self.slots):
+            self.step()
+
+    def run(self, requests: Sequence[Request]):
+        """Serves requests whose .arrival are seconds from now; returns when
+        every request is complete."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        i = 0
+        self.t0 = time.perf_counter()
+        while i < len(pending) or self.queue or any(s.busy for s in self.slots):
+            t = self.now()
+            while i < len(pending) and pending[i].arrival <= t:
+                self.submit(pending[i])
+                i += 1
+            self.step()
+        return self.report()
+
+    def report(self) -> Dict:
+        lat = np.array([r.latency_ms for r in self.completed]) if self.completed else np.zeros(1)
+        span = max((r.done for r in self.completed), default=0.0) - \
+            min((r.arrival for r in self.completed), default=0.0)
+        sizes = [r.batch_size for r in self.completed]
+        return {"completed": len(self.completed), "rejected": len(self.rejected),
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_batch": float(np.mean(sizes)) if sizes else 0.0,
+                "fps": len(self.completed) / span if span > 0 else 0.0,
+                "resident_bytes": int(self.resident),
+                "max_in_flight_bytes": int(self.max_in_flight),
+                "max_concurrent_batches": int(self.max_busy)}
+
+    def close(self):
+        for s in self.slots:
+            s.batch.close()
+        for c in self.contexts:
+            c.close()
+
+
+def poisson_requests(n_scenes: int, rate: float, duration: float, sizes, seed: int = 1,
+                     shadow_frac: float = 0.5, camera_for=None) -> List[Request]:
+    """Seeded Poisson arrivals at `rate` per second for `duration` seconds;
+    each picks a scene, a resolution and a camera (camera_for(scene, w, h,
+    rng)), with shadows on a `shadow_frac` share of requests."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        s = int(rng.integers(n_scenes))
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        out.append(Request(rid=len(out), scene=s, camera=camera_for(s, w, h, rng), width=w,
+                           height=h, shadows=bool(rng.random() < shadow_frac), arrival=t))
+    return out

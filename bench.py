@@ -120,6 +120,194 @@ def build_scene(name):
     raise SystemExit(f"unknown config {name}")
 
 
+C4_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
+
+
+def c4_jobs(ctx, n_jobs=64, seed=1):
+    """64 views of the scene: seeded camera poses around and inside the
+    200^3 box, resolutions drawn uniformly from C4_SIZES."""
+    from axi_visualizer_amd.batch import Job, job_camera
+
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n_jobs):
+        w, h = C4_SIZES[int(rng.integers(len(C4_SIZES)))]
+        ang = rng.uniform(0, 2 * np.pi)
+        dist = rng.uniform(60.0, 330.0)
+        pos = np.array([dist * np.cos(ang), rng.uniform(-60.0, 120.0), dist * np.sin(ang)])
+        target = rng.uniform(-60.0, 60.0, 3)
+        jobs.append(Job(ctx, job_camera(w, h, pos, target), w, h))
+    return jobs
+
+
+def bench_c4(args):
+    """Batched multi-view: 64 jobs of a 500k-triangle scene per batch,
+    autotuned bin size and capacity, graph-replayed; against the same 64
+    views drawn one at a time through CLDraw (the plain path)."""
+    from axi_visualizer_amd import scenes
+    from axi_visualizer_amd.batch import Batch, SceneContext, autotune, settle
+    from axi_visualizer_amd.ops import CLDraw
+    from axi_visualizer_amd.render import load_scene
+
+    sc = scenes.spheres_scene(W=320, H=240, n_inst=1000, shadows=False, name="c4_spheres")
+    ctx = SceneContext(sc)
+    jobs = c4_jobs(ctx)
+    tune = autotune(jobs, sizes=(8, 16, 32), cap_scales=(1, 4), reps=3, graph=True)
+    best = tune["best"]
+    print(f"[bench] c4 autotune: {tune}", file=sys.stderr, flush=True)
+    b = Batch(best["bin_size"], best["cap_small"], best["cap_large"], graph=True)
+    settle(b, jobs)
+    for _ in range(max(2, args.warmup)):
+        b.render(jobs)
+    b.wait()
+    steps = max(1, args.steps // 20)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.render(jobs)
+    b.wait()
+    dt = time.perf_counter() - t0
+    fps = steps * len(jobs) / dt
+    lat, gpu = [], []
+    for _ in range(max(5, args.latency_frames // 10)):
+        t1 = time.perf_counter()
+        b.render(jobs)
+        gpu.append(b.wait())
+        lat.append((time.perf_counter() - t1) * 1e3)
+    # plain path: the same views one at a time, one CLDraw per resolution
+    draws = {}
+    for w, h in C4_SIZES:
+        d = CLDraw(1, len(sc.groups), w, h, 1)
+        load_scene(d, sc)
+        draws[w, h] = d
+    shaders = [g.flags for g in sc.groups]
+
+    def plain_pass():
+        for j in jobs:
+            d = draws[j.width, j.height]
+            d.setScaleCull(j.camera.scale, j.camera.cax, j.camera.cay)
+            d.setPos(j.camera.pos)
+            d.setVM(j.camera.vm)
+            d.clearZBuffer()
+            d.drawAll(shaders, shadowIds=[])
+            d.gamma(1.0)
+
+    plain_pass()
+    for d in draws.values():
+        d.sync()
+    t0 = time.perf_counter()
+    reps = max(1, steps // 2)
+    for _ in range(reps):
+        plain_pass()
+    for d in draws.values():
+        d.sync()
+    plain_fps = reps * len(jobs) / (time.perf_counter() - t0)
+    px = sum(j.width * j.height for j in jobs)
+    algo = sum(sc.ntri * 36 + j.width * j.height * 8 for j in jobs)
+    gpu_ms = float(np.median(gpu))
+    ach = algo / (gpu_ms * 1e-3) / 1e9
+    line = {
+        "metric": "frames/s (batched, 64 views per launch sequence)", "value": round(fps, 2),
+        "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generators)",
+        "config": {"workload": "batched_64_views_500k_tris_mixed_320x240_to_1080p",
+                   "jobs": len(jobs), "triangles": sc.ntri, "pixels_per_batch": px,
+                   "bin_size": best["bin_size"], "cap_small": best["cap_small"],
+                   "cap_large": best["cap_large"], "graph": True},
+        "plain_fps_same_views": round(plain_fps, 2),
+        "speedup_vs_plain": round(fps / plain_fps, 3),
+        "mpix_per_s": round(fps * px / len(jobs) / 1e6, 2),
+        "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                             "p99": round(float(np.percentile(lat, 99)), 3)},
+        "batch_gpu_ms": round(gpu_ms, 3),
+        "autotune": tune["all"],
+        "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "batch (setup + order + raster)", "algorithmic_bytes": int(algo),
+                     "avg_us": round(gpu_ms * 1e3, 2)},
+        "cpu_baseline": None,
+    }
+    b.close()
+    for d in draws.values():
+        d.close()
+    ctx.close()
+    print(json.dumps(line))
+
+
+C5_SIZES = [(848, 480), (1280, 720), (1920, 1080)]
+
+
+def bench_c5(args):
+    """Serving: 16 resident scenes (50k-1M triangles, terrain and spheres,
+    1-4 textures, two shadow cascades each), Poisson arrivals at 2,000 req/s,
+    random scene / camera / resolution (480p, 720p, 1080p), shadows on half
+    the requests, 200 GB HBM admission budget, batches formed every <= 2 ms.
+    Reports latency at the offered load, the sustained (saturated) frames/s,
+    and the same requests rendered one at a time for comparison."""
+    from axi_visualizer_amd import scenes
+    from axi_visualizer_amd.batch import Batch, Job
+    from axi_visualizer_amd.server import RenderServer, poisson_requests
+
+    mix = scenes.serving_scenes()
+    kinds = [k for _, k in mix]
+    print(f"[bench] c5: {len(mix)} scenes, {sum(s.ntri for s, _ in mix)} triangles",
+          file=sys.stderr, flush=True)
+    srv = RenderServer([s for s, _ in mix], hbm_budget=200e9, slots=3, max_batch=64,
+                       window_ms=2.0)
+
+    def cam(s, w, h, rng):
+        return scenes.serving_camera(kinds[s], w, h, rng.random(4))
+
+    # warm-up: grow every slot's buffers once
+    warm = poisson_requests(len(mix), 1e6, 200e-6, C5_SIZES, seed=99, camera_for=cam)
+    srv.run(warm)
+    srv.completed.clear()
+    offered = poisson_requests(len(mix), 2000.0, args.serve_seconds, C5_SIZES, seed=1,
+                               camera_for=cam)
+    at_load = srv.run(offered)
+    print(f"[bench] c5 at 2000 req/s: {at_load}", file=sys.stderr, flush=True)
+    srv.completed.clear()
+    burst = poisson_requests(len(mix), 1e6, 2e-3, C5_SIZES, seed=2, camera_for=cam)
+    for r in burst:
+        r.arrival = 0.0
+    sat = srv.run(burst)
+    print(f"[bench] c5 saturated: {sat}", file=sys.stderr, flush=True)
+    # the same kind of requests one at a time, each waited for
+    one = Batch()
+    sample = burst[:min(len(burst), 300)]
+    t0 = time.perf_counter()
+    for r in sample:
+        one.render([Job(srv.contexts[r.scene], r.camera, r.width, r.height, shadows=r.shadows,
+                        pcf_taps=r.pcf_taps)])
+        one.wait()
+    single_fps = len(sample) / (time.perf_counter() - t0)
+    one.close()
+    line = {
+        "metric": "frames/s (serving, sustained)", "value": round(sat["fps"], 2),
+        "unit": "frames/s", "n_gpus": 1, "steps": sat["completed"], "warmup": len(warm),
+        "ms_per_step": round(1e3 / sat["fps"], 4) if sat["fps"] else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene generators)",
+        "config": {"workload": "serving_16_scenes_poisson_2000rps_480p_720p_1080p",
+                   "scenes": len(mix), "triangles": [s.ntri for s, _ in mix],
+                   "hbm_budget_bytes": 200e9, "slots": 3, "max_batch": 64, "window_ms": 2.0},
+        "offered_rate": 2000.0,
+        "at_offered_load": {"fps": round(at_load["fps"], 2), "p50_ms": round(at_load["p50_ms"], 3),
+                            "p99_ms": round(at_load["p99_ms"], 3),
+                            "mean_batch": round(at_load["mean_batch"], 2),
+                            "completed": at_load["completed"]},
+        "saturated": {"p50_ms": round(sat["p50_ms"], 3), "p99_ms": round(sat["p99_ms"], 3),
+                      "mean_batch": round(sat["mean_batch"], 2),
+                      "max_concurrent_batches": sat["max_concurrent_batches"]},
+        "one_at_a_time_fps": round(single_fps, 2),
+        "speedup_vs_one_at_a_time": round(sat["fps"] / single_fps, 3) if single_fps else None,
+        "resident_bytes": sat["resident_bytes"],
+        "roofline": None, "cpu_baseline": None,
+    }
+    srv.close()
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +316,8 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--latency-frames", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="serving: seconds of Poisson arrivals")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -135,6 +325,11 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "c4":
+        return bench_c4(args)
+    if args.config == "c5":
+        return bench_c5(args)
 
     from axi_visualizer_amd.render import Renderer
 
