@@ -162,12 +162,15 @@ int reserve_tris(axr_ctx* c, size_t extra) {
   c->tri_cap = cap;
   dfree(c->d_rec);
   dfree(c->d_cls);
+  dfree(c->d_chunks);
   for (int k = 0; k < AXR_MAX_SHADOWS; k++) {
     dfree(c->d_shrec[k]);
     dfree(c->d_ok[k]);
   }
   c->last_ok = nullptr;
-  if ((rc = dalloc(&c->d_rec, cap)) || (rc = dalloc(&c->d_cls, cap))) return rc;
+  if ((rc = dalloc(&c->d_rec, cap)) || (rc = dalloc(&c->d_cls, cap)) ||
+      (rc = dalloc(&c->d_chunks, cap / 64)))
+    return rc;
   for (int k = 0; k < AXR_MAX_SHADOWS; k++)
     if ((rc = dalloc(&c->d_shrec[k], cap)) || (rc = dalloc(&c->d_ok[k], cap))) return rc;
   // cap is a power of two >= 1024: the byte arrays are whole words
@@ -316,7 +319,7 @@ int run_setup(axr_ctx* c, const AxrDrawConst* dc_main) {
   {
     Timed tm(c, dc_main ? "setup_bin" : "shadow_setup_bin");
     HIPCHK(launch_setup(c->d_vpos, c->d_vattr, c->d_groups, dc, dc_main ? 1 : 0, c->d_rec,
-                        c->d_cls, c->mbins.B, c->NBX, ps, c->sbins.B, c->stream));
+                        c->d_cls, c->mbins.B, c->NBX, ps, c->sbins.B, c->d_chunks, c->stream));
   }
   {
     Timed tm(c, dc_main ? "bin_order" : "shadow_order");
@@ -406,6 +409,7 @@ int axr_destroy(axr_ctx* c) {
   dfree(c->d_groups);
   dfree(c->d_rec);
   dfree(c->d_cls);
+  dfree(c->d_chunks);
   for (int k = 0; k < AXR_MAX_SHADOWS; k++) {
     dfree(c->d_shrec[k]);
     dfree(c->d_ok[k]);
@@ -486,6 +490,7 @@ int axr_add_texture_group(axr_ctx* c, const float* xyz, const float* uv, const f
   HIPCHK(hipMemcpy(hg.d_tex, ht.data(), ht.size() * sizeof(ushort4), hipMemcpyHostToDevice));
   c->groups.push_back(hg);
   c->T += (uint32_t)ntri;
+  HIPCHK(launch_chunk_bounds(c->d_vpos, c->T, c->d_chunks, c->stream));
   if ((rc = upload_groups(c))) return rc;
   if (tn_out) *tn_out = (int)c->groups.size() - 1;
   return AXR_OK;

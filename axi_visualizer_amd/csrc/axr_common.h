@@ -158,6 +158,7 @@ struct AxrJobDev {
   const float4* vpos;
   const float4* vattr;
   const AxrGroupDev* groups;
+  const float4* chunks;  // bounding sphere per 64-triangle chunk of the scene
   AxrTriRec* rec;  // [dc.T] setup records of this job
   uint8_t* cls;    // [dc.T]
   float* depth;    // [W*H] outputs

@@ -109,6 +109,7 @@ struct axr_ctx {
   size_t tri_cap = 0;
   float4* d_vpos = nullptr;   // [3*tri_cap] (x,y,z,0)
   float4* d_vattr = nullptr;  // [6*tri_cap] (nx,ny,nz,u),(v,Ix,Iy,Iz)
+  float4* d_chunks = nullptr;  // [tri_cap/64] bounding sphere per 64-triangle chunk
   AxrGroupDev* d_groups = nullptr;
   AxrTriRec* d_rec = nullptr;   // [tri_cap] setup records of the last drawAll
   uint8_t* d_cls = nullptr;     // [tri_cap] 0 culled, 1 small, 2 large

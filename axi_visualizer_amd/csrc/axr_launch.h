@@ -11,7 +11,9 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_setup(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                         const AxrDrawConst& dc, int do_main, AxrTriRec* rec, uint8_t* cls,
                         const AxrBins& MB, int NBX, const AxrPassSet& ps, const AxrBins& SB,
-                        hipStream_t s);
+                        const float4* chunks, hipStream_t s);
+// Bounding sphere per 64-triangle chunk (setup skips chunks out of view).
+hipError_t launch_chunk_bounds(const float4* vpos, uint32_t T, float4* chunks, hipStream_t s);
 // Counts of the pass, heavy-first order, cursor reset for the screen bins
 // and/or the shadow bins (either may be null), one launch.
 hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins,

@@ -58,7 +58,11 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
   const uint32_t gi = (blockIdx.x - J.blk0) * kSetupBlock + threadIdx.x;
   int cls = 0;
   AxrTriRec r;
-  if (gi < dc.T) {
+  // the wave's 64 triangles are one chunk: skip it when out of view
+  const float4 sph = J.chunks[imin((int)(gi >> 6), (int)((dc.T - 1) >> 6))];
+  if (gi < dc.T && chunk_outside_view(sph, dc.view)) {
+    J.cls[gi] = 0;
+  } else if (gi < dc.T) {
     float4 vp[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) vp[k] = J.vpos[3 * (size_t)gi + k];

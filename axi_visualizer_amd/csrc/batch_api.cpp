@@ -149,6 +149,7 @@ AxrJobDev make_job(const axr_batch* b, const axr_job& j, const JobInfo& in, uint
   d.vpos = sc->d_vpos;
   d.vattr = sc->d_vattr;
   d.groups = sc->d_groups;
+  d.chunks = sc->d_chunks;
   d.rec = b->d_rec + rec_off;
   d.cls = b->d_cls + rec_off;
   d.depth = b->d_depth + in.out_off;

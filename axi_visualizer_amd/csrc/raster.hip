@@ -34,6 +34,57 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
+// Bounding sphere of each 64-triangle chunk (one wave per chunk): the box
+// centre, and the largest vertex distance from it, widened by 1e-5 relative
+// and by 1e-6 of the coordinate magnitude so it holds every vertex.
+__global__ __launch_bounds__(256) void k_chunk_bounds(const float4* __restrict__ vpos, uint32_t T,
+                                                      float4* __restrict__ chunks) {
+  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t chunk = gi >> 6, nch = (T + 63) >> 6;
+  const bool live = gi < T;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float4 v[3];
+  bool finite = true;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    v[k] = live ? vpos[3 * (size_t)gi + k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (live) {
+      const float c[3] = {v[k].x, v[k].y, v[k].z};
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        finite = finite && isfinite(c[a]);
+        lo[a] = fminf(lo[a], c[a]);
+        hi[a] = fmaxf(hi[a], c[a]);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], o));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], o));
+    }
+  }
+  const float cx = 0.5f * (lo[0] + hi[0]), cy = 0.5f * (lo[1] + hi[1]), cz = 0.5f * (lo[2] + hi[2]);
+  float r = 0.0f;
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float dx = v[k].x - cx, dy = v[k].y - cy, dz = v[k].z - cz;
+      r = fmaxf(r, sqrtf(dx * dx + dy * dy + dz * dz));
+    }
+  }
+  const bool all_finite = __ballot(live && !finite) == 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o));
+  if (__lane_id() == 0 && chunk < nch) {
+    const float mag = fabsf(cx) + fabsf(cy) + fabsf(cz) + r;
+    chunks[chunk] = all_finite ? make_float4(cx, cy, cz, r * 1.00001f + 1e-6f * mag + 1e-30f)
+                               : make_float4(0.0f, 0.0f, 0.0f, NAN);
+  }
+}
+
 // K1: setup + bin appends.  Screen slot b: small triangles (global index);
 // slot NB + b: large triangles' coarse tiles, (global index << 2) | the
 // 16-px tile's quadrant in bin b.  Shadow slot slot0 + b of a pass: casters
@@ -44,12 +95,24 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
                                                   AxrDrawConst dc, int do_main,
                                                   AxrTriRec* __restrict__ rec,
                                                   uint8_t* __restrict__ cls_out, AxrBins MB,
-                                                  int NBX, AxrPassSet ps, AxrBins SB) {
+                                                  int NBX, AxrPassSet ps, AxrBins SB,
+                                                  const float4* __restrict__ chunks) {
   const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = gi < dc.T;
+  // The wave's 64 triangles are one chunk: skip the passes it cannot reach.
+  const float4 sph = chunks[imin((int)(gi >> 6), (int)((dc.T - 1) >> 6))];
+  const bool main_on = do_main && !chunk_outside_view(sph, dc.view);
+  bool any_on = main_on;
+  uint32_t pass_on = 0;
+#pragma unroll
+  for (int p = 0; p < AXR_MAX_SHADOWS; p++) {
+    if (p >= ps.n) break;
+    if (!chunk_outside_ortho(sph, ps.p[p].cam, ps.p[p].dim, ps.p[p].bias)) pass_on |= 1u << p;
+  }
+  any_on |= pass_on != 0;
   float4 vp[3];
   int grp = 0;
-  if (live) {
+  if (live && any_on) {
 #pragma unroll
     for (int k = 0; k < 3; k++) vp[k] = vpos[3 * (size_t)gi + k];
     grp = find_group(groups, (int)dc.ngroups, gi);
@@ -60,7 +123,8 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
   if (do_main) {
     int cls = 0;
     AxrTriRec r;
-    if (live) {
+    if (live && !main_on) cls_out[gi] = 0;
+    if (live && main_on) {
       const uint32_t f = groups[grp].flags;
       if (f & AXR_GRP_DRAW)
         cls = tri_setup(vp, vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
@@ -94,7 +158,8 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
     bool ok = false;
     AxrShRec r;
     if (live) {
-      if (cast_bit(P.cast, grp)) ok = shadow_setup(vp, P.cam, P.dim, P.bias, r);
+      if (((pass_on >> p) & 1u) && cast_bit(P.cast, grp))
+        ok = shadow_setup(vp, P.cam, P.dim, P.bias, r);
       P.ok[gi] = ok ? 1 : 0;
       if (ok) P.rec[gi] = r;
     }
@@ -315,10 +380,17 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
 hipError_t launch_setup(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                         const AxrDrawConst& dc, int do_main, AxrTriRec* rec, uint8_t* cls,
                         const AxrBins& MB, int NBX, const AxrPassSet& ps, const AxrBins& SB,
-                        hipStream_t s) {
+                        const float4* chunks, hipStream_t s) {
   if (dc.T == 0 || (!do_main && ps.n == 0)) return hipSuccess;
   hipLaunchKernelGGL(k_setup, dim3((dc.T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
-                     vattr, groups, dc, do_main, rec, cls, MB, NBX, ps, SB);
+                     vattr, groups, dc, do_main, rec, cls, MB, NBX, ps, SB, chunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_chunk_bounds(const float4* vpos, uint32_t T, float4* chunks, hipStream_t s) {
+  if (T == 0) return hipSuccess;
+  const uint32_t nch = (T + 63) / 64;
+  hipLaunchKernelGGL(k_chunk_bounds, dim3((nch + 3) / 4), dim3(256), 0, s, vpos, T, chunks);
   return hipGetLastError();
 }
 
