@@ -340,3 +340,155 @@ def test_uniform_texture_shading_closed_form():
     sc.light_dir = np.array([0, 0, -1], np.float32)  # n.L = 1
     out = po.render(sc)
     assert np.abs(out["r"][cov].astype(int) - int(1000 * 1.2)).max() <= 1
+
+
+# ------------------------------------------------ shading closed forms (a10, a12)
+def _cam128():
+    return scenes.make_camera(128, 128, [0, 0, 0], 0.0, 0.0, fovx=90.0)
+
+
+def _group(xyz, uv, tex, side, mip, flags, vn=(0.0, 0.0, 1.0), **kw):
+    n = xyz.shape[0]
+    return scenes.Group(xyz=np.asarray(xyz, np.float32),
+                        vn=np.tile(np.asarray(vn, np.float32), (n, 1)),
+                        uv=np.asarray(uv, np.float32), tex=tex, tex_side=side, mip=mip,
+                        flags=flags, **kw)
+
+
+def _scene(groups, light_dir, cascades=(), pcf=5):
+    return scenes.Scene(name="closed_form", W=128, H=128, groups=list(groups), view=_cam128(),
+                        light_int=np.ones(3, np.float32),
+                        light_dir=np.asarray(light_dir, np.float32), amb=0.2,
+                        cascades=list(cascades), pcf_taps=pcf, exposure=0.0)
+
+
+def _level_chain(n_levels):
+    """Mip chain (smallest level first) whose level k is uniform 100*(k+1)."""
+    return np.concatenate([np.full((4 ** k, 3), 100 * (k + 1), np.uint16)
+                           for k in range(n_levels)])
+
+
+@pytest.mark.parametrize("U,level", [(1.0, 5.0), (0.5, 5.5), (2.0, 4.5), (0.25, 6.0),
+                                     (2 ** 0.5, 4.75)])
+def test_trilinear_level_closed_form(U, level):
+    """av_draw_normmap_mip.c:40-57: fmip = |log2(dt)|/2 with dt the largest
+    |delta uv| / (pixel distance)^2 over the edges.  The triangle
+    (0,0,1),(0.5,0,1),(0,0.5,1) projects to legs of 32 px, so dt = U/1024:
+    U=1 -> level 5 (32 texels over 32 px), U=1/2 -> 5.5, U=2 -> 4.5,
+    U=1/4 -> 6, U=sqrt(2) -> 4.75.  Each level of the chain is uniform, so the colour is the
+    (1-f, f) blend of levels floor and floor+1, times 1.2 (n.L = 1, I = amb)."""
+    xyz = np.array([[0, 0, 1], [0.5, 0, 1], [0, 0.5, 1]], np.float32)
+    uv = np.array([[0, 0], [U, 0], [0, U]], np.float32)
+    g = _group(xyz, uv, _level_chain(8), 128, True, scenes.SH_MIP)
+    out = po.render(_scene([g], [0, 0, 1]))
+    cov = out["id"] != 0xFFFFFFFF
+    assert cov.sum() > 300
+    k = int(level)
+    f = level - k
+    want = 1.2 * ((1 - f) * 100 * (k + 1) + f * 100 * (k + 2))
+    assert np.abs(out["r"][cov].astype(float) - want).max() <= 1.0
+
+
+def test_blinn_phong_closed_form():
+    """av_draw_normmap.c:114-148 / 276-310 on a plane z = 4 with normal
+    (0,0,1), uniform texture 1000, light L = (0, 0.6, 0.8), I = amb:
+    colour = 1000*(n.L + amb) + LInt * h.n^p * 256*4 * (p+2)/(8 pi), with
+    h = normalize(normalize(pos - eye) + L); evaluated in float64 from the
+    pixel's position on the plane."""
+    z = 4.0
+    xyz = np.array([[-0.9 * z, -0.9 * z, z], [0.9 * z, -0.9 * z, z], [-0.9 * z, 0.9 * z, z],
+                    [0.9 * z, -0.9 * z, z], [0.9 * z, 0.9 * z, z], [-0.9 * z, 0.9 * z, z]],
+                   np.float32)
+    tex, side = scenes.texture_for(np.full((4, 4, 3), 1000, np.uint16), False)
+    g = _group(xyz, np.zeros((6, 2)), tex, side, False, scenes.SH_PHONG, spec_pow=16)
+    L = np.array([0.0, 0.6, 0.8])
+    out = po.render(_scene([g], L.astype(np.float32)))
+    cov = out["id"] != 0xFFFFFFFF
+    assert cov.mean() > 0.5
+    ys, xs = np.nonzero(cov)
+    # pixel (x, y) sees the plane at ((x - 64) z / 64, -(y - 64) z / 64, z)
+    p = np.stack([(xs - 64) * z / 64.0, -(ys - 64) * z / 64.0, np.full(xs.shape, z)], -1)
+    a = p / np.linalg.norm(p, axis=-1, keepdims=True)
+    h = a + L
+    h /= np.linalg.norm(h, axis=-1, keepdims=True)
+    theta = np.maximum(0.0, h[:, 2])
+    spec = theta ** 16 * 256 * 4 * 18 / (8 * 3.1416)
+    want = spec + 1000 * (0.8 + 0.2)
+    got = out["r"][ys, xs].astype(float)
+    assert (spec > 100).any()
+    assert np.abs(got - want).max() <= 2 + 2e-3 * want.max()
+
+
+@pytest.mark.parametrize("pcf", [5, 9])
+def test_pcf_shadow_closed_form(pcf):
+    """av_alpha_max.c:53-84 / the 3x3 variant: a receiver plane z = 10
+    (normal (0,0,1), uniform 1000) under light travelling +z, an off-centre
+    caster rectangle x in [1,4], y in [-3,0] at z = 6 that is not drawn.
+    Where every tap lands in the caster's shadow, light = amb, so the colour
+    is 1000*(amb*(1 + amb) + (1 - amb)*amb) = 400; away from it 1200.  A
+    flipped map axis would move the shadow to the mirrored rectangle."""
+    def quad(x0, x1, y0, y1, z):
+        return np.array([[x0, y0, z], [x1, y0, z], [x0, y1, z],
+                         [x1, y0, z], [x1, y1, z], [x0, y1, z]], np.float32)
+    tex, side = scenes.texture_for(np.full((4, 4, 3), 1000, np.uint16), False)
+    recv = _group(quad(-8, 8, -8, 8, 10.0), np.zeros((6, 2)), tex, side, False,
+                  scenes.SH_SHADOW, cast=False)
+    caster = _group(quad(1, 4, -3, 0, 6.0), np.zeros((6, 2)), tex, side, False, 0,
+                    cast=True, skip=True)
+    casc = scenes.make_cascade(256, [0, 0, 8], 20.0, 0.0, 0.0)
+    out = po.render(_scene([recv, caster], [0, 0, 1], [casc], pcf))
+    ys, xs = np.nonzero(out["id"] != 0xFFFFFFFF)
+    wx, wy = (xs - 64) * 10 / 64.0, -(ys - 64) * 10 / 64.0
+    m = 3 / 12.8  # tap reach (2 texels) plus a texel of rounding
+    inside = (wx > 1 + m) & (wx < 4 - m) & (wy > -3 + m) & (wy < 0 - m)
+    outside = (wx < 1 - m) | (wx > 4 + m) | (wy < -3 - m) | (wy > 0 + m)
+    r = out["r"][ys, xs].astype(int)
+    assert inside.sum() > 50 and outside.sum() > 1000
+    assert np.abs(r[inside] - 400).max() <= 1
+    assert np.abs(r[outside] - 1200).max() <= 1
+
+
+@pytest.mark.parametrize("pcf", [5, 9])
+def test_pcf_taps_against_the_map(pcf):
+    """The PCF lookup read against the map the pass produced: for every lit
+    receiver pixel, the taps (the cross at +-2 texels, or the 3x3 grid at
+    +-2) counted in float64 from the pixel's position give
+    light = (n*amb + K - n) / K and colour = 1000*(light*1.2 + (1-light)*0.2).
+    Pixels within rounding of a texel edge or of the stored depth are left
+    out.  The caster is a thin diagonal bar, so cross and grid differ."""
+    tex, side = scenes.texture_for(np.full((4, 4, 3), 1000, np.uint16), False)
+    def quad(x0, x1, y0, y1, z):
+        return np.array([[x0, y0, z], [x1, y0, z], [x0, y1, z],
+                         [x1, y0, z], [x1, y1, z], [x0, y1, z]], np.float32)
+    recv = _group(quad(-8, 8, -8, 8, 9.7), np.zeros((6, 2)), tex, side, False,
+                  scenes.SH_SHADOW, cast=False)
+    bar = np.array([[-3, -2.6, 6], [2, 2.4, 6], [-3, -2.2, 6]], np.float32)
+    caster = _group(bar, np.zeros((3, 2)), tex, side, False, 0, cast=True, skip=True)
+    casc = scenes.make_cascade(256, [0, 0, 8], 20.0, 0.0, 0.0)
+    sc = _scene([recv, caster], [0, 0, 1], [casc], pcf)
+    out = po.render(sc, with_shadows_out=True)
+    smap = out["shadow"][0].astype(np.float64)
+    cam = casc.cam
+    F, R, U = (cam.vm.astype(np.float64)[i] for i in range(3))
+    ys, xs = np.nonzero(out["id"] == 0)
+    z = 9.7  # texel size and pixel size incommensurate
+    p = np.stack([(xs - 64) * z / 64.0, -(ys - 64) * z / 64.0, np.full(xs.shape, z)], -1)
+    pos = p - cam.pos.astype(np.float64)
+    depth = pos @ F
+    fx, fy = (pos @ R) * cam.scale, (pos @ U) * -cam.scale
+    sx, sy = np.trunc(fx).astype(int) + 128, np.trunc(fy).astype(int) + 128
+    offs = [(0, 0), (0, 2), (0, -2), (2, 0), (-2, 0)] if pcf == 5 else \
+        [(oy, ox) for oy in (-2, 0, 2) for ox in (-2, 0, 2)]
+    n = np.zeros(xs.shape, int)
+    near = np.zeros(xs.shape, bool)
+    for oy, ox in offs:
+        sd = smap[sy + oy, sx + ox]
+        n += sd < depth
+        near |= np.abs(sd - depth) < 1e-3 * depth
+    edge = (np.abs(fx - np.round(fx)) < 1e-3) | (np.abs(fy - np.round(fy)) < 1e-3)
+    keep = ~near & ~edge
+    light = (n * 0.2 + pcf - n) / pcf
+    want = 1000 * (light * 1.2 + (1 - light) * 0.2)
+    got = out["r"][ys, xs].astype(float)
+    assert keep.sum() > 5000 and len(set(n[keep])) >= 3
+    assert np.abs(got[keep] - want[keep]).max() <= 1.0
