@@ -448,14 +448,19 @@ int axr_add_texture_group(axr_ctx* c, const float* xyz, const float* uv, const f
     return fail(AXR_ECAP, "at most %d groups per context", AXR_MAX_GROUPS);
   if ((uint64_t)c->T + (uint64_t)ntri >= (1ull << 30))
     return fail(AXR_ECAP, "at most 2^30 triangles per context");
-  // Texture shape: N*N texels, or the full smallest-first chain of N*N.
+  // Texture shape: N*N texels, the full smallest-first chain of N*N, or
+  // its N*N base level (chain built here).
+  if (is_mip < 0 || is_mip > 2) return fail(AXR_EINVAL, "is_mip must be 0, 1 or 2");
+  int64_t chain = ntexels;
   if (is_mip) {
     // Trilinear reads levels mip and mip+1 <= texSize, so a chain needs >= 2 levels.
     if (tex_size < 1 || tex_size > 14) return fail(AXR_EINVAL, "mip texSize out of range");
-    const int64_t want = (((int64_t)1 << (2 * (tex_size + 1))) - 1) / 3;
+    chain = (((int64_t)1 << (2 * (tex_size + 1))) - 1) / 3;
+    const int64_t want = is_mip == 1 ? chain : ((int64_t)1 << (2 * tex_size));
     if (ntexels != want)
-      return fail(AXR_EINVAL, "mip chain of 2^%d needs %lld texels, got %lld", tex_size,
-                  (long long)want, (long long)ntexels);
+      return fail(AXR_EINVAL, "%s of 2^%d needs %lld texels, got %lld",
+                  is_mip == 1 ? "mip chain" : "base level", tex_size, (long long)want,
+                  (long long)ntexels);
   } else {
     if (tex_size <= 0 || (tex_size & (tex_size - 1)) ||
         (int64_t)tex_size * tex_size != ntexels)
@@ -485,9 +490,15 @@ int axr_add_texture_group(axr_ctx* c, const float* xyz, const float* uv, const f
   hg.ntri = (uint32_t)ntri;
   hg.tex_size = tex_size;
   hg.is_mip = is_mip ? 1 : 0;
-  hg.ntexels = ntexels;
-  if ((rc = dalloc(&hg.d_tex, (size_t)ntexels))) return rc;
-  HIPCHK(hipMemcpy(hg.d_tex, ht.data(), ht.size() * sizeof(ushort4), hipMemcpyHostToDevice));
+  hg.ntexels = chain;
+  if ((rc = dalloc(&hg.d_tex, (size_t)chain))) return rc;
+  // the given texels are the chain, or its last (largest) level
+  HIPCHK(hipMemcpy(hg.d_tex + (chain - ntexels), ht.data(), ht.size() * sizeof(ushort4),
+                   hipMemcpyHostToDevice));
+  if (is_mip == 2) {
+    HIPCHK(launch_create_mips(hg.d_tex, tex_size + 1, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   c->groups.push_back(hg);
   c->T += (uint32_t)ntri;
   HIPCHK(launch_chunk_bounds(c->d_vpos, c->T, c->d_chunks, c->stream));
@@ -871,6 +882,23 @@ int axr_get_stats(axr_ctx* c, uint32_t out[6]) {
   out[3] = h[AXR_CTR_BIN_TOTAL];
   out[4] = h[AXR_CTR_LOST];
   out[5] = h[AXR_CTR_SH_TOTAL];
+  return AXR_OK;
+}
+
+int axr_get_texels(axr_ctx* c, int tn, uint16_t* r, uint16_t* g, uint16_t* b) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (tn < 0 || tn >= (int)c->groups.size() || !r || !g || !b)
+    return fail(AXR_EINVAL, "getTexels: bad arguments");
+  const HostGroup& hg = c->groups[(size_t)tn];
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<ushort4> h((size_t)hg.ntexels);
+  HIPCHK(hipMemcpy(h.data(), hg.d_tex, h.size() * sizeof(ushort4), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < h.size(); i++) {
+    r[i] = h[i].x;
+    g[i] = h[i].y;
+    b[i] = h[i].z;
+  }
   return AXR_OK;
 }
 

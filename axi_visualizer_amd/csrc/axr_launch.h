@@ -34,6 +34,9 @@ hipError_t launch_gamma_planes(uint16_t* r, uint16_t* g, uint16_t* b, size_t n, 
                                hipStream_t s);
 hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, uint16_t* b,
                                size_t n, hipStream_t s);
+// Smallest-first mip chain of `levels` levels whose last (largest) level
+// is filled: builds the others on the device.
+hipError_t launch_create_mips(ushort4* chain, int levels, hipStream_t s);
 hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipStream_t s);
 
 // Batched jobs (batch.hip).  bin_shift 3/4/5 = 8/16/32-px bins.

@@ -108,6 +108,26 @@ __global__ void k_vertlight(const float4* __restrict__ vpos, float4* __restrict_
   vattr[2 * (size_t)v + 1] = a1;
 }
 
+// Utils.createMips (Utils.py:79-93) one level at a time: level k (side
+// n = 2^k, at offset (4^k - 1)/3 of the smallest-first chain) from level
+// k + 1 as the 2x2 sums of each channel shifted right by 2.
+__global__ void k_mip_level(ushort4* __restrict__ chain, int k) {
+  const int n = 1 << k;
+  const size_t dst0 = (((size_t)1 << (2 * k)) - 1) / 3;
+  const size_t src0 = dst0 + (size_t)n * n;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int x = idx % n, y = idx / n, m = 2 * n;
+  const ushort4 a = chain[src0 + (size_t)(2 * y) * m + 2 * x];
+  const ushort4 b = chain[src0 + (size_t)(2 * y) * m + 2 * x + 1];
+  const ushort4 c = chain[src0 + (size_t)(2 * y + 1) * m + 2 * x];
+  const ushort4 d = chain[src0 + (size_t)(2 * y + 1) * m + 2 * x + 1];
+  chain[dst0 + (size_t)y * n + x] =
+      make_ushort4((uint16_t)(((uint32_t)a.x + b.x + c.x + d.x) >> 2),
+                   (uint16_t)(((uint32_t)a.y + b.y + c.y + d.y) >> 2),
+                   (uint16_t)(((uint32_t)a.z + b.z + c.z + d.z) >> 2), 0);
+}
+
 inline unsigned grid_for(size_t n, unsigned block, unsigned cap) {
   size_t g = (n + block - 1) / block;
   if (g < 1) g = 1;
@@ -154,6 +174,14 @@ hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, u
 
 hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_decode_shadow, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, map, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_create_mips(ushort4* chain, int levels, hipStream_t s) {
+  for (int k = levels - 2; k >= 0; k--) {
+    const int n = 1 << k;
+    hipLaunchKernelGGL(k_mip_level, dim3((n * n + 255) / 256), dim3(256), 0, s, chain, k);
+  }
   return hipGetLastError();
 }
 

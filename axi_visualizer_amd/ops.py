@@ -71,6 +71,7 @@ class CLDraw:
         self.gSize = []
         self.texSize = []
         self._isMip = []
+        self._texels = []
         self.SHADOWMAP = {}
         self.ambLight = np.float32(0.2)
         self._flags_buf = None
@@ -105,9 +106,11 @@ class CLDraw:
         outside the raster hot path, so the texture is not used."""
         self.skyTexSize = np.int32(size)
 
-    def addTextureGroup(self, xyz, uv, vn, r, g, b, mip=False):
+    def addTextureGroup(self, xyz, uv, vn, r, g, b, mip=False, build_mips=False):
         """Ops.py:331-378.  xyz, vn: (3T, 3); uv: (T, 3, 2) or (3T, 2);
-        r, g, b: N x N planes, or mip chains when mip=N."""
+        r, g, b: N x N planes, or mip chains when mip=N.  build_mips (with
+        mip=N): r, g, b are the N x N base level and the chain is built on
+        the device (Utils.createMips' 2x2 integer box filter)."""
         p = _f32(xyz, (-1, 3))
         n = _f32(vn, (-1, 3))
         t = _f32(uv, (-1, 2))
@@ -123,9 +126,11 @@ class CLDraw:
         tn = ctypes.c_int()
         self._chk(self._L.axr_add_texture_group(
             self._ctx, ptr(p), ptr(t), ptr(n), p.shape[0] // 3, ptr(rr), ptr(gg), ptr(bb),
-            rr.shape[0], tex_size, 1 if mip else 0, ctypes.byref(tn)), "addTextureGroup")
+            rr.shape[0], tex_size, (2 if build_mips else 1) if mip else 0, ctypes.byref(tn)),
+            "addTextureGroup")
         self.gSize.append(np.int32(p.shape[0]))
         self.texSize.append(np.int32(tex_size))
+        self._texels.append(((4 ** (tex_size + 1)) - 1) // 3 if mip else rr.shape[0])
         self._isMip.append(bool(mip))
         return tn.value
 
@@ -289,6 +294,14 @@ class CLDraw:
         f = np.array([shader_flags(shaders[tn]) for tn in range(ng)], np.uint32)
         m = np.array([1 if (mask is not None and mask[tn]) else 0 for tn in range(ng)], np.uint8)
         self._chk(self._L.axr_set_shaders(self._ctx, ptr(f), ptr(m), ng), "setShaders")
+
+    def getTexels(self, tn):
+        """Group tn's texels as stored on the device: (r, g, b) u16 planes."""
+        n = int(self._texels[tn])
+        out = [np.empty(n, np.uint16) for _ in range(3)]
+        self._chk(self._L.axr_get_texels(self._ctx, int(tn), *[ptr(o) for o in out]),
+                  "getTexels")
+        return tuple(out)
 
     def deviceBytes(self) -> int:
         n = ctypes.c_uint64()

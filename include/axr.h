@@ -53,8 +53,10 @@ int axr_set_scale_cull(axr_ctx* ctx, float s, float cax, float cay);
 
 /* CLDraw.addTextureGroup(xyz, uv, vn, r, g, b, mip), Ops.py:331-378.
  * xyz, vn: [3*ntri][3] f32; uv: [3*ntri][2] f32; r, g, b: ntexels u16
- * planes (N*N, or the smallest-first mip chain of Utils.createMips);
- * tex_size is the reference's texSize: N, or log2(N) when is_mip.
+ * planes; tex_size is the reference's texSize: N, or log2(N) for mips.
+ * is_mip 0: N*N planes; 1: the smallest-first mip chain of
+ * Utils.createMips (Utils.py:79-93); 2: the N*N base level only, the chain
+ * built on the device with the same 2x2 integer box filter (ntexels N*N).
  * Returns the group index (tn) in *tn_out. */
 int axr_add_texture_group(axr_ctx* ctx, const float* xyz, const float* uv, const float* vn,
                           int ntri, const uint16_t* r, const uint16_t* g, const uint16_t* b,
@@ -141,6 +143,10 @@ int axr_set_bin_capacity(axr_ctx* ctx, uint32_t small, uint32_t large, uint32_t 
 int axr_set_timing(axr_ctx* ctx, int enable);
 int axr_get_timing(axr_ctx* ctx, char* names, int name_len, double* total_ms, int64_t* count,
                    int n);
+
+/* Texels of group tn as stored on the device (r, g, b planes of the N*N
+ * texture or of its whole smallest-first mip chain). */
+int axr_get_texels(axr_ctx* ctx, int tn, uint16_t* r, uint16_t* g, uint16_t* b);
 
 /* Shader binding of drawAll (flags per group, mask = groups not drawn)
  * without drawing: prepares a context used as a resident scene by batches. */

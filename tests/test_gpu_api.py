@@ -109,3 +109,22 @@ def test_lost_entries_are_reported_and_the_redraw_is_whole():
         compare(sc, gpu, ref, "redraw_after_loss")
     finally:
         r.close()
+
+
+@pytest.mark.parametrize("side", [2, 64, 1024])
+def test_device_mip_chain_matches_create_mips(side):
+    """addTextureGroup(..., mip=N, build_mips=True): the chain built on the
+    device equals Utils.createMips (the oracle's orc_create_mips) bit-exactly."""
+    from axi_visualizer_amd.ops import CLDraw
+
+    base = scenes.noise_texture(5, 9, side)
+    d = CLDraw(1, 1, 64, 64, 1)
+    try:
+        xyz = np.array([[0, 0, 1], [1, 0, 1], [0, 1, 1]], np.float32)
+        tn = d.addTextureGroup(xyz, np.zeros((3, 2), np.float32), xyz, base[..., 0],
+                               base[..., 1], base[..., 2], mip=side, build_mips=True)
+        got = d.getTexels(tn)
+        for c in range(3):
+            assert np.array_equal(got[c], pyoracle.create_mips(base[..., c])), f"channel {c}"
+    finally:
+        d.close()
