@@ -51,6 +51,8 @@ SIGNATURES = {
     "axr_clear_shadow_map": (_i, [_ctx, _i]),
     "axr_shadow_map": (_i, [_ctx, _i, _p, _i, _f]),
     "axr_gamma": (_i, [_ctx, _f]),
+    "axr_blur": (_i, [_ctx]),
+    "axr_bloom": (_i, [_ctx, _i, _f, _f, _f]),
     "axr_get_frame": (_i, [_ctx, _p, _p, _p, _p]),
     "axr_get_rgba8": (_i, [_ctx, _p]),
     "axr_get_ids": (_i, [_ctx, _p]),

@@ -424,6 +424,7 @@ int axr_destroy(axr_ctx* c) {
   dfree(c->d_ids);
   dfree(c->d_counters);
   dfree(c->d_light);
+  if (c->d_post) (void)hipFree(c->d_post);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return AXR_OK;
@@ -778,6 +779,63 @@ int axr_gamma(axr_ctx* c, float ex) {
     default:
       return fail(AXR_ESTATE, "gamma before any clearZBuffer");
   }
+}
+
+// The planes a post pass works on: HDR colour, or gamma'd colour after
+// gamma() (the reference's post kernels read whatever RO/GO/BO hold).
+static int post_planes(axr_ctx* c, const char* what) {
+  int rc;
+  if (c->state == ST_UNCLEARED) return fail(AXR_ESTATE, "%s before any clearZBuffer", what);
+  if ((rc = flush_shadows(c))) return rc;
+  if (c->state == ST_CLEARED || c->state == ST_PENDING) return materialize(c, SHADE_HDR, 1.0f);
+  if (c->state == ST_LDR8) {
+    HIPCHK(launch_expand_rgba8(c->d_rgba, c->d_r, c->d_g, c->d_b, (size_t)c->W * c->H,
+                               c->stream));
+    c->state = ST_LDR16;
+  }
+  return 0;
+}
+
+static int post_scratch(axr_ctx* c, size_t bytes) {
+  if (c->post_bytes >= bytes) return 0;
+  if (c->d_post) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(c->d_post);
+    c->d_post = nullptr;
+    c->post_bytes = 0;
+  }
+  HIPCHK(hipMalloc(&c->d_post, bytes));
+  c->post_bytes = bytes;
+  return 0;
+}
+
+int axr_blur(axr_ctx* c) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (c->W < 4 || c->H < 8 || (c->W & 1) || (c->H & 1))
+    return fail(AXR_EINVAL, "blur needs an even frame of at least 4x8, got %dx%d", c->W, c->H);
+  if ((rc = post_planes(c, "blur"))) return rc;
+  if ((rc = post_scratch(c, 6 * sizeof(uint16_t) * (size_t)(c->W / 2) * (c->H / 2)))) return rc;
+  Timed tm(c, "blur");
+  HIPCHK(launch_blur(c->d_r, c->d_g, c->d_b, c->W, c->H, (uint16_t*)c->d_post, c->stream));
+  return AXR_OK;
+}
+
+int axr_bloom(axr_ctx* c, int levels, float threshold, float white, float strength) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (levels < 1 || levels > 12)
+    return fail(AXR_EINVAL, "bloom: levels must be 1..12, got %d", levels);
+  if ((c->W % (1 << levels)) || (c->H % (1 << levels)))
+    return fail(AXR_EINVAL, "bloom: %dx%d is not a multiple of 2^%d", c->W, c->H, levels);
+  if (!(white > 0.0f) || !(strength >= 0.0f) || threshold != threshold)
+    return fail(AXR_EINVAL, "bloom: white must be > 0 and strength >= 0");
+  if ((rc = post_planes(c, "bloom"))) return rc;
+  if ((rc = post_scratch(c, 2 * bloom_halves(c->W, c->H, levels)))) return rc;
+  Timed tm(c, "bloom");
+  HIPCHK(launch_bloom(c->d_r, c->d_g, c->d_b, c->W, c->H, levels, threshold, white, strength,
+                      c->d_post, c->stream));
+  return AXR_OK;
 }
 
 int axr_get_frame(axr_ctx* c, uint16_t* r, uint16_t* g, uint16_t* b, float* depth) {

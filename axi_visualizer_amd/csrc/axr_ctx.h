@@ -126,6 +126,8 @@ struct axr_ctx {
   uint32_t* d_rgba = nullptr;
   uint32_t* d_ids = nullptr;
   uint32_t* d_counters = nullptr;
+  void* d_post = nullptr;     // blur / bloom intermediates
+  size_t post_bytes = 0;
   float* d_light = nullptr;  // vertLight light arrays
   size_t light_cap = 0;
 

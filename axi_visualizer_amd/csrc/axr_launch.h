@@ -37,6 +37,13 @@ hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, u
 // Smallest-first mip chain of `levels` levels whose last (largest) level
 // is filled: builds the others on the device.
 hipError_t launch_create_mips(ushort4* chain, int levels, hipStream_t s);
+// Post passes on the u16 planes (post.hip).  blur: tmp holds
+// 6 * (W/2) * (H/2) u16.  bloom: tmp holds bloom_halves(W, H, levels) fp16.
+hipError_t launch_blur(uint16_t* r, uint16_t* g, uint16_t* b, int W, int H, uint16_t* tmp,
+                       hipStream_t s);
+size_t bloom_halves(int W, int H, int levels);
+hipError_t launch_bloom(uint16_t* r, uint16_t* g, uint16_t* b, int W, int H, int levels,
+                        float threshold, float white, float strength, void* tmp, hipStream_t s);
 hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipStream_t s);
 
 // Batched jobs (batch.hip).  bin_shift 3/4/5 = 8/16/32-px bins.

@@ -209,6 +209,19 @@ class CLDraw:
         """Ops.py:947-957 -> Shaders/Post/gamma.c."""
         self._chk(self._L.axr_gamma(self._ctx, float(ex)), "gamma")
 
+    def blur(self):
+        """Ops.py:1009-1020 -> Shaders/Post/blur.c, blur2.c: brightness-
+        weighted half-resolution copy blurred 19 taps each way and added
+        back to the frame."""
+        self._chk(self._L.axr_blur(self._ctx), "blur")
+
+    def bloom(self, levels=5, threshold=1.0, white=9835.0, strength=1.0):
+        """Multi-level fp16 bloom on the frame: max(frame / white -
+        threshold, 0) downsampled `levels` times, upsampled back and added
+        with weight white * strength.  W and H must divide by 2**levels."""
+        self._chk(self._L.axr_bloom(self._ctx, int(levels), float(threshold), float(white),
+                                    float(strength)), "bloom")
+
     def getFrame(self):
         """Ops.py:1224-1230: (R, G, B) u16 and depth f32, each (H, W)."""
         H, W = int(self.H), int(self.W)

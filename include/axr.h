@@ -106,6 +106,17 @@ int axr_shadow_map(axr_ctx* ctx, int i, const uint8_t* cast, int ngroups, float 
 /* CLDraw.gamma(ex), Ops.py:947-957 -> Shaders/Post/gamma.c. */
 int axr_gamma(axr_ctx* ctx, float ex);
 
+/* CLDraw.blur(), Ops.py:1009-1020 -> Shaders/Post/blur.c (blurH) and
+ * blur2.c (blurV): brightness-weighted half-resolution copy, 19-tap
+ * horizontal and vertical blur, bilinear upsample added to the frame's
+ * planes.  Frame must be even and at least 4x8. */
+int axr_blur(axr_ctx* ctx);
+/* Multi-level bloom of the frame's planes (the full-frame pipeline's post
+ * pass; no single CLDraw method): `levels` fp16 half-resolution levels of
+ * max(frame / white - threshold, 0), upsampled back and added with
+ * weight white * strength.  W and H must be multiples of 2^levels. */
+int axr_bloom(axr_ctx* ctx, int levels, float threshold, float white, float strength);
+
 /* CLDraw.getFrame(), Ops.py:1224-1230: R, G, B u16 planes and f32 depth,
  * each [h][w].  Any pointer may be NULL. */
 int axr_get_frame(axr_ctx* ctx, uint16_t* r, uint16_t* g, uint16_t* b, float* depth);

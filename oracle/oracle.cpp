@@ -987,6 +987,97 @@ int orc_gamma(uint16_t* r, uint16_t* g, uint16_t* b, int64_t n, float ex) {
   return 0;
 }
 
+// ------------------------------------------------------- a13+: blur
+// CLDraw.blur (Ops.py:1009-1020): Shaders/Post/blur.c blurH, then
+// Shaders/Post/blur2.c blurV, as serial phases (the kernels' global
+// barriers do not order work-groups).  Decrees where the text is undefined:
+// literals are float; float->ushort stores saturate toward zero; the
+// half-resolution buffers start at zero each call (rows the kernels never
+// write stay zero); the vertical blur skips taps outside the half-res rows
+// (blur2.c:47-56 bounds them by the width, reading past the buffer).
+namespace {
+inline uint16_t sat_rtz(float v) { return f2u16(v); }
+const float kTap[10] = {0.10125f, 0.09815f, 0.08941f, 0.07654f, 0.06157f,
+                        0.04654f, 0.03306f, 0.02207f, 0.01384f, 0.00816f};
+}  // namespace
+
+int orc_blur(uint16_t* R, uint16_t* G, uint16_t* B, int W, int H) {
+  if (W < 4 || H < 8 || (W & 1) || (H & 1)) return -1;
+  const int wF = W / 2, hF = H / 2;
+  const size_t n = (size_t)wF * hF;
+  std::vector<uint16_t> D(3 * n, 0), T(3 * n, 0), U(3 * n, 0);
+  uint16_t* P[3] = {R, G, B};
+  // blurH phase 1 (blur.c:23-36): brightness of a 2-px wide sample, as indexed
+  for (int cy = 0; cy < hF - 2; cy++)
+    for (int cx = 0; cx < wF; cx++) {
+      const int y1 = imin(wF * (hF - 1) * 4, wF * cy * 4 + 1);
+      const int x1 = imin(wF * 2 - 1, cx * 2 + 1);
+      float c[3];
+      for (int k = 0; k < 3; k++) {
+        const uint16_t* S = P[k];
+        c[k] = (float)(S[wF * cy * 4 + cx * 2] / 4 + S[y1 + cx * 2] / 4 + S[wF * cy * 4 + x1] / 4 +
+                       S[y1 + x1] / 4);
+      }
+      const float lum = (0.3626f * c[0] + 0.5152f * c[1]) + 0.1222f * c[2];
+      for (int k = 0; k < 3; k++) D[k * n + (size_t)wF * cy + cx] = sat_rtz(c[k] * lum / 256 / 256);
+    }
+  // blurH phase 2 (blur.c:40-64): 19 taps along x, /0.9
+  for (int cy = 0; cy < hF - 1; cy++)
+    for (int cx = 0; cx < wF; cx++)
+      for (int k = 0; k < 3; k++) {
+        const uint16_t* S = D.data() + k * n + (size_t)wF * cy;
+        float a = 0.0f;
+        for (int o = -9; o <= 9; o++) {
+          const int ao = o < 0 ? -o : o;
+          if (o < 0 && cx <= -o - 1) continue;
+          if (o > 0 && cx >= wF - o) continue;
+          a = a + kTap[ao] * (float)S[cx + o];
+        }
+        a /= 0.9f;
+        T[k * n + (size_t)wF * cy + cx] = sat_rtz(a);
+      }
+  // blurV phase 1 (blur2.c:21-60): 19 taps along y, /0.9
+  for (int cy = 0; cy < hF - 1; cy++)
+    for (int cx = 0; cx < wF; cx++)
+      for (int k = 0; k < 3; k++) {
+        const uint16_t* S = T.data() + k * n;
+        float a = 0.0f;
+        for (int o = -9; o <= 9; o++) {
+          const int ao = o < 0 ? -o : o;
+          if (o < 0 && cy <= -o - 1) continue;
+          if (o > 0 && (cy >= wF - o || cy + o >= hF)) continue;
+          a = a + kTap[ao] * (float)S[(size_t)wF * (cy + o) + cx];
+        }
+        a /= 0.9f;
+        U[k * n + (size_t)wF * cy + cx] = sat_rtz(a);
+      }
+  // blurV phase 2 (blur2.c:62-88): bilinear upsample added to the frame
+  const int lenT = wF;
+  for (int cy = 0; cy < H; cy++)
+    for (int cx = 0; cx < W; cx++) {
+      float texr1 = (float)cx / 2.f;
+      const int tex1 = (int)texr1;
+      texr1 -= (float)tex1;
+      float texr2 = (float)cy / 2.f;
+      const int tex2 = (int)texr2;
+      texr2 -= (float)tex2;
+      const int t00 = tex1 + lenT * tex2;
+      const int t10 = imin(tex1 + 1, lenT - 1) + lenT * tex2;
+      const int t01 = tex1 + lenT * imin(tex2 + 1, H / 2 - 2);
+      const int t11 = imin(tex1 + 1, lenT - 1) + lenT * imin(tex2 + 1, H / 2 - 2);
+      const float texi1 = 1 - texr1, texi2 = 1 - texr2;
+      for (int k = 0; k < 3; k++) {
+        const uint16_t* S = U.data() + k * n;
+        const float up = ((texi1 * texi2 * (float)S[t00] + texr1 * texi2 * (float)S[t10]) +
+                          texi1 * texr2 * (float)S[t01]) +
+                         texr1 * texr2 * (float)S[t11];
+        uint16_t& px = P[k][(size_t)W * cy + cx];
+        px = sat_rtz((float)px + up);
+      }
+    }
+  return 0;
+}
+
 int orc_create_mips(const uint16_t* plane, int N, uint16_t* out) {
   // Utils.py:79-93: 2x2 box sum >> 2 per level; levels concatenated
   // smallest first.

@@ -17,6 +17,7 @@
  *   Shaders_src/av_alpha_max.c:53-84 (PCF)        -> pcf (internal)
  *   Pipe/trisetupOrtho.c, Shaders/drawmin.c       -> shadow pass
  *   Shaders/clearzb.c, Shaders/Post/gamma.c       -> clear, orc_gamma
+ *   Shaders/Post/blur.c, Post/blur2.c, Ops.py:1009 -> orc_blur
  *   Pipe/vert.c:38-88                              -> orc_vertlight
  *   Utils.py:79-93 (createMips)                    -> orc_create_mips
  *   Ops0.py:471-648, Ops.py:1089-1141 (pass order) -> orc_render
@@ -126,6 +127,9 @@ int orc_fragments(int mode, const float* P6, const float* Z3, int W, int H,
                   int cap);
 
 int orc_gamma(uint16_t* r, uint16_t* g, uint16_t* b, int64_t n, float ex);
+/* CLDraw.blur: Post/blur.c blurH then Post/blur2.c blurV on u16 planes of
+ * a W x H frame (serial phases; decrees in oracle.cpp). */
+int orc_blur(uint16_t* r, uint16_t* g, uint16_t* b, int W, int H);
 int orc_create_mips(const uint16_t* plane, int N, uint16_t* out);
 int orc_vertlight(const float* xyz, const float* vn, int nvert,
                   const float* dirI, const float* dirD, int ndir,

@@ -75,9 +75,10 @@ def lib() -> ctypes.CDLL:
         L.orc_fragments.argtypes = [i, p, p, i, i, i, p, p, p, p, i]
         L.orc_gamma.argtypes = [p, p, p, ctypes.c_int64, f]
         L.orc_create_mips.argtypes = [p, i, p]
+        L.orc_blur.argtypes = [p, p, p, i, i]
         L.orc_vertlight.argtypes = [p, p, i, p, p, i, p, p, i, p, p, p, i, f, p]
         for fn in ("orc_render", "orc_trisetup", "orc_shadow_setup", "orc_coarse_tri",
-                   "orc_fragments", "orc_gamma", "orc_create_mips", "orc_vertlight"):
+                   "orc_fragments", "orc_gamma", "orc_create_mips", "orc_vertlight", "orc_blur"):
             getattr(L, fn).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -221,6 +222,66 @@ def gamma(r, g, b, ex):
     r, g, b = (_c(x, np.uint16).copy() for x in (r, g, b))
     lib().orc_gamma(r.ctypes.data, g.ctypes.data, b.ctypes.data, r.size, float(ex))
     return r, g, b
+
+
+def blur(r, g, b):
+    """CLDraw.blur on (H, W) u16 planes; returns new planes."""
+    r, g, b = (_c(x, np.uint16).copy() for x in (r, g, b))
+    H, W = r.shape
+    if lib().orc_blur(r.ctypes.data, g.ctypes.data, b.ctypes.data, W, H) < 0:
+        raise ValueError("blur: W >= 4, H >= 8, both even")
+    return r, g, b
+
+
+def _bilerp_up(src, h, w):
+    """Bilinear upsample of a (h/2, w/2) float32 level to (h, w): sample at
+    ((x + 0.5) / 2 - 0.5, (y + 0.5) / 2 - 0.5), edges clamped, weights in
+    float32, terms added left to right."""
+    sh, sw = src.shape
+    fx = (np.arange(w, dtype=np.float32) + np.float32(0.5)) / np.float32(2) - np.float32(0.5)
+    fy = (np.arange(h, dtype=np.float32) + np.float32(0.5)) / np.float32(2) - np.float32(0.5)
+    fx = np.maximum(fx, np.float32(0))
+    fy = np.maximum(fy, np.float32(0))
+    x0 = np.minimum(fx.astype(np.int64), sw - 1)
+    y0 = np.minimum(fy.astype(np.int64), sh - 1)
+    ax = fx - x0.astype(np.float32)
+    ay = fy - y0.astype(np.float32)
+    x1 = np.minimum(x0 + 1, sw - 1)
+    y1 = np.minimum(y0 + 1, sh - 1)
+    bx, by = np.float32(1) - ax, np.float32(1) - ay
+    s00, s10 = src[y0][:, x0], src[y0][:, x1]
+    s01, s11 = src[y1][:, x0], src[y1][:, x1]
+    return ((by[:, None] * bx[None, :] * s00 + by[:, None] * ax[None, :] * s10)
+            + ay[:, None] * bx[None, :] * s01) + ay[:, None] * ax[None, :] * s11
+
+
+def bloom(r, g, b, levels=5, threshold=1.0, white=9835.0, strength=1.0):
+    """Multi-level bloom on an fp16 pyramid (the full-frame pipeline's
+    post pass; the reference's blur is single-level).  Per channel:
+    level 1 = max(avg2x2(frame / white) - threshold, 0); level l+1 =
+    avg2x2(level l); going up, up_l = level_l + bilinear(up_{l+1}); frame
+    += white * strength * bilinear(up_1), stored saturated toward zero.
+    Every level is stored as float16 (round to nearest even); arithmetic is
+    float32 in the order written."""
+    f32, f16 = np.float32, np.float16
+    out = []
+    for plane in (r, g, b):
+        P = _c(plane, np.uint16)
+        H, W = P.shape
+        x = P.astype(f32) / f32(white)
+        def avg(a):
+            return ((a[0::2, 0::2] + a[0::2, 1::2]) + a[1::2, 0::2] + a[1::2, 1::2]) * f32(0.25)
+        lv = [np.maximum(avg(x) - f32(threshold), f32(0)).astype(f16)]
+        for _ in range(levels - 1):
+            lv.append(avg(lv[-1].astype(f32)).astype(f16))
+        up = lv[-1]
+        for l in range(levels - 2, -1, -1):
+            h, w = lv[l].shape
+            up = (lv[l].astype(f32) + _bilerp_up(up.astype(f32), h, w)).astype(f16)
+        add = _bilerp_up(up.astype(f32), H, W) * (f32(white) * f32(strength))
+        v = P.astype(f32) + add
+        out.append(np.clip(np.trunc(np.maximum(v, f32(0))), 0, 65535).astype(np.uint16))
+    return tuple(out)
 
 
 def create_mips(plane):
