@@ -52,6 +52,9 @@ SIGNATURES = {
     "axr_shadow_map": (_i, [_ctx, _i, _p, _i, _f]),
     "axr_gamma": (_i, [_ctx, _f]),
     "axr_blur": (_i, [_ctx]),
+    "axr_set_refl_tex": (_i, [_ctx, _i, _p, _p, _p, _i]),
+    "axr_get_refl_tex": (_i, [_ctx, _i, _p, _p, _p]),
+    "axr_draw_ssr": (_i, [_ctx, _p, _p, _i, _i, _i, _f, _i]),
     "axr_bloom": (_i, [_ctx, _i, _f, _f, _f]),
     "axr_get_frame": (_i, [_ctx, _p, _p, _p, _p]),
     "axr_get_rgba8": (_i, [_ctx, _p]),

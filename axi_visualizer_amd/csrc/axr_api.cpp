@@ -193,6 +193,8 @@ int upload_groups(axr_ctx* c) {
     h[i].mip_bias = g.mip_bias;
     h[i].spec_pow = g.spec_pow;
     h[i].tex = g.d_tex;
+    h[i].refl = g.d_refl;
+    h[i].refl_dim = g.refl_dim;
   }
   // Kernels in flight read the table: let them finish before rewriting it.
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -285,6 +287,27 @@ bool pass_pending(const axr_ctx* c, int i) {
 
 // Launches the pending shadow passes -- with the drawAll's setup when dc
 // is given -- as one setup, one order and one shadow-depth launch.
+// Draw constants of the context's current view and lights.
+AxrDrawConst base_dc(const axr_ctx* c) {
+  AxrDrawConst dc{};
+  dc.view = c->view;
+  for (int k = 0; k < 3; k++) {
+    dc.lint[k] = c->lint[k];
+    dc.ldir[k] = c->ldir[k];
+    dc.clear_rgb[k] = c->clear_rgb[k];
+  }
+  dc.amb = c->amb;
+  dc.W = c->W;
+  dc.H = c->H;
+  dc.WC = c->WC;
+  dc.HC = c->HC;
+  dc.T = c->T;
+  dc.ngroups = (uint32_t)c->groups.size();
+  dc.pcf_taps = 5;
+  dc.flag_mask = 0xFFFFFFFFu;
+  return dc;
+}
+
 int run_setup(axr_ctx* c, const AxrDrawConst* dc_main) {
   AxrPassSet ps{};
   int nb_total = 0;
@@ -425,6 +448,8 @@ int axr_destroy(axr_ctx* c) {
   dfree(c->d_counters);
   dfree(c->d_light);
   if (c->d_post) (void)hipFree(c->d_post);
+  if (c->d_snap) (void)hipFree(c->d_snap);
+  for (ReflTex& t : c->refl) dfree(t.d);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return AXR_OK;
@@ -621,22 +646,8 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
   if (nshadow < 0 || nshadow > AXR_SHADOW_BIND)
     return fail(AXR_EINVAL, "drawAll: at most %d shadow maps", AXR_SHADOW_BIND);
   if (pcf_taps != 5 && pcf_taps != 9) return fail(AXR_EINVAL, "pcf_taps must be 5 or 9");
-  AxrDrawConst dc{};
-  dc.view = c->view;
-  for (int k = 0; k < 3; k++) {
-    dc.lint[k] = c->lint[k];
-    dc.ldir[k] = c->ldir[k];
-    dc.clear_rgb[k] = c->clear_rgb[k];
-  }
-  dc.amb = c->amb;
-  dc.W = c->W;
-  dc.H = c->H;
-  dc.WC = c->WC;
-  dc.HC = c->HC;
-  dc.T = c->T;
-  dc.ngroups = (uint32_t)ngroups;
+  AxrDrawConst dc = base_dc(c);
   dc.pcf_taps = pcf_taps;
-  dc.flag_mask = 0xFFFFFFFFu;
   bool any_shadow = false, changed = false;
   std::vector<uint32_t> nf((size_t)ngroups);
   for (int g = 0; g < ngroups; g++) {
@@ -671,6 +682,126 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
   c->last = dc;
   c->records_valid = true;
   c->state = c->T > 0 ? ST_PENDING : ST_CLEARED;
+  return AXR_OK;
+}
+
+int axr_set_refl_tex(axr_ctx* c, int slot, const uint16_t* r, const uint16_t* g,
+                     const uint16_t* b, int size) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (!r || !g || !b) return fail(AXR_EINVAL, "setReflTex: null plane");
+  if (size < 2 || size > 8192 || (size & 1))
+    return fail(AXR_EINVAL, "setReflTex: face size %d must be even, 2..8192", size);
+  if (slot >= (int)c->refl.size()) return fail(AXR_EINVAL, "setReflTex: no slot %d", slot);
+  if (slot < 0) {
+    if (c->refl.size() >= 64) return fail(AXR_EINVAL, "setReflTex: at most 64 cube maps");
+    c->refl.push_back(ReflTex{});
+    slot = (int)c->refl.size() - 1;
+  }
+  ReflTex& t = c->refl[(size_t)slot];
+  const size_t n = (size_t)6 * size * size;
+  std::vector<ushort4> h(n);
+  for (size_t i = 0; i < n; i++) h[i] = make_ushort4(r[i], g[i], b[i], 0);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (t.size != size) {
+    dfree(t.d);
+    if ((rc = dalloc(&t.d, n))) return rc;
+    t.size = size;
+  }
+  HIPCHK(hipMemcpy(t.d, h.data(), n * sizeof(ushort4), hipMemcpyHostToDevice));
+  return slot;
+}
+
+int axr_get_refl_tex(axr_ctx* c, int slot, uint16_t* r, uint16_t* g, uint16_t* b) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (slot < 0 || slot >= (int)c->refl.size() || !c->refl[(size_t)slot].d)
+    return fail(AXR_EINVAL, "getReflTex: no cube map in slot %d", slot);
+  const ReflTex& t = c->refl[(size_t)slot];
+  const size_t n = (size_t)6 * t.size * t.size;
+  std::vector<ushort4> h(n);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(h.data(), t.d, n * sizeof(ushort4), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; i++) {
+    if (r) r[i] = h[i].x;
+    if (g) g[i] = h[i].y;
+    if (b) b[i] = h[i].z;
+  }
+  return t.size;
+}
+
+int axr_draw_ssr(axr_ctx* c, const int* groups, const int* refl, int n, int steps, int step,
+                 float absorb, int fresnel) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (n < 1 || !groups || !refl) return fail(AXR_EINVAL, "drawSSR: no groups");
+  if (steps < 1 || steps > (1 << 20) || step < 1 || step > steps)
+    return fail(AXR_EINVAL, "drawSSR: steps %d / step %d", steps, step);
+  if (!(absorb == absorb)) return fail(AXR_EINVAL, "drawSSR: absorb is NaN");
+  const int ng = (int)c->groups.size();
+  std::vector<int> slot_of((size_t)ng, -1);
+  for (int k = 0; k < n; k++) {
+    const int g = groups[k], s = refl[k];
+    if (g < 0 || g >= ng) return fail(AXR_EINVAL, "drawSSR: no group %d", g);
+    if (c->groups[(size_t)g].is_mip)
+      return fail(AXR_EINVAL, "drawSSR: group %d: the water texture must not be a mip chain", g);
+    if (s < 0 || s >= (int)c->refl.size() || !c->refl[(size_t)s].d)
+      return fail(AXR_EINVAL, "drawSSR: group %d: no cube map in slot %d", g, s);
+    slot_of[(size_t)g] = s;
+  }
+  if (c->state == ST_UNCLEARED) return fail(AXR_ESTATE, "drawSSR before any clearZBuffer");
+  if (c->state == ST_LDR8 || c->state == ST_LDR16)
+    return fail(AXR_ESTATE, "drawSSR after gamma: the pass works on the HDR frame");
+  if ((rc = flush_shadows(c))) return rc;
+  if (c->state == ST_CLEARED || c->state == ST_PENDING) {
+    if ((rc = materialize(c, SHADE_HDR, 1.0f))) return rc;
+  }
+  const size_t np = (size_t)c->W * c->H;
+  const size_t snap = np * (sizeof(float) + 3 * sizeof(uint16_t));
+  if (c->snap_bytes < snap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->d_snap) (void)hipFree(c->d_snap);
+    c->d_snap = nullptr;
+    c->snap_bytes = 0;
+    HIPCHK(hipMalloc(&c->d_snap, snap));
+    c->snap_bytes = snap;
+  }
+  AxrSsrConst S{};
+  S.F0 = (const float*)c->d_snap;
+  S.R0 = (const uint16_t*)((char*)c->d_snap + np * sizeof(float));
+  S.G0 = S.R0 + np;
+  S.B0 = S.G0 + np;
+  S.steps = steps;
+  S.step = step;
+  S.absorb = absorb;
+  S.fresnel = fresnel ? 1 : 0;
+  {
+    Timed tm(c, "ssr_copy");
+    HIPCHK(hipMemcpyAsync((void*)S.F0, c->d_depth, np * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void*)S.R0, c->d_r, np * 2, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void*)S.G0, c->d_g, np * 2, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync((void*)S.B0, c->d_b, np * 2, hipMemcpyDeviceToDevice, c->stream));
+  }
+  // only the water groups take part in this draw's setup
+  for (int g = 0; g < ng; g++) {
+    HostGroup& G = c->groups[(size_t)g];
+    const int s = slot_of[(size_t)g];
+    G.flags = s >= 0 ? AXR_GRP_DRAW : 0u;
+    G.d_refl = s >= 0 ? c->refl[(size_t)s].d : nullptr;
+    G.refl_dim = s >= 0 ? c->refl[(size_t)s].size / 2 : 0;
+  }
+  if ((rc = upload_groups(c))) return rc;
+  AxrDrawConst dc = base_dc(c);
+  if (c->T > 0) {
+    if ((rc = run_setup(c, &dc))) return rc;
+    Timed tm(c, "ssr_raster");
+    AxrBins B = c->mbins.B;
+    B.nb = c->NB;
+    HIPCHK(launch_ssr(c->d_rec, B, c->d_vpos, c->d_vattr, c->d_groups, dc, S, c->NBX, c->d_depth,
+                      c->d_r, c->d_g, c->d_b, c->stream));
+  }
+  c->state = ST_HDR;
+  c->records_valid = false;
   return AXR_OK;
 }
 

@@ -88,6 +88,49 @@ __device__ __forceinline__ int entry_tile(uint32_t v, int bx, int by, int WC) {
   return (by >> 1) * WC + (bx >> 1);
 }
 
+// Every fragment of the bin's entries takes an LDS min of its depth key:
+// small triangles of slot s_small walk drawSmall's scanlines clipped to the
+// bin (a7), large-triangle tile entries of slot s_large walk the tiled pass
+// on their reference tile (a9).  FIRST: the key's low word is seq, so the
+// first fragment in draw order wins a depth tie (a strict test); otherwise
+// ~seq, the last one wins (the reference's F >= tz).
+template <int SHIFT, bool FIRST>
+__device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec* __restrict__ rec,
+                                         const AxrBins& B, uint32_t s_small, uint32_t s_large,
+                                         int bx, int by, const AxrDrawConst& dc) {
+  constexpr int BIN = 1 << SHIFT, NPX = BIN * BIN;
+  const int x0 = bx << SHIFT, y0 = by << SHIFT;
+  const int xe = imin(x0 + BIN, dc.W), ye = imin(y0 + BIN, dc.H);
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  auto key = [](float tz, uint32_t seq) -> unsigned long long {
+    return FIRST ? make_key_first(tz, seq) : make_key(tz, seq);
+  };
+  for_bin_entries(B, s_small, B.num[s_small], tid, nthr, [&](uint32_t gi) {
+    const AxrTriRec r = rec[gi];
+    const Geom<int> g = small_geom(r.px, r.py, r.z);
+    small_walk(g, x0, y0, xe, ye, [&](int ax, int cy, float tz, int half) {
+      if (!(tz <= AXR_DEPTH_CLEAR)) return;
+      const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
+      if (lc < (unsigned)NPX) atomicMin(&lk[lc], key(tz, frag_seq(gi, half)));
+    });
+  });
+  for_bin_entries(B, s_large, B.num[s_large], tid, nthr, [&](uint32_t v) {
+    const uint32_t gi = v >> 2;
+    const TileBox box = tile_box(entry_tile<SHIFT>(v, bx, by, dc.WC), dc.W);
+    const AxrTriRec r = rec[gi];
+    const Geom<float> g = tile_geom(r.px, r.py, r.z);
+    const uint32_t order = dc.T + gi;
+    tile_walk(g, box, [&](int ax, int cy, float tz, int half) {
+      if (!(tz <= AXR_DEPTH_CLEAR)) return;
+      // a tile is larger than an 8-px bin: keep the bin's part of it
+      if (SHIFT < 4 && ((unsigned)(ax - x0) >= (unsigned)BIN || (unsigned)(cy - y0) >= (unsigned)BIN))
+        return;
+      const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
+      if (lc < (unsigned)NPX) atomicMin(&lk[lc], key(tz, frag_seq(order, half)));
+    });
+  });
+}
+
 // Depth walk and shading of one bin of (1 << SHIFT)^2 pixels by one block.
 // Small triangles of slot s_small walk drawSmall's scanlines clipped to the
 // bin (a7); large-triangle tile entries of slot s_large walk the tiled pass
@@ -110,32 +153,7 @@ __device__ __forceinline__ void raster_bin(
   for (int i = tid; i < NPX; i += nthr) lk[i] = key_clear();
   if (tid == 0) *ncov = 0;
   __syncthreads();
-  if (!empty) {
-    for_bin_entries(B, s_small, B.num[s_small], tid, nthr, [&](uint32_t gi) {
-      const AxrTriRec r = rec[gi];
-      const Geom<int> g = small_geom(r.px, r.py, r.z);
-      small_walk(g, x0, y0, xe, ye, [&](int ax, int cy, float tz, int half) {
-        if (!(tz <= AXR_DEPTH_CLEAR)) return;
-        const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
-        if (lc < (unsigned)NPX) atomicMin(&lk[lc], (unsigned long long)make_key(tz, frag_seq(gi, half)));
-      });
-    });
-    for_bin_entries(B, s_large, B.num[s_large], tid, nthr, [&](uint32_t v) {
-      const uint32_t gi = v >> 2;
-      const TileBox box = tile_box(entry_tile<SHIFT>(v, bx, by, dc.WC), dc.W);
-      const AxrTriRec r = rec[gi];
-      const Geom<float> g = tile_geom(r.px, r.py, r.z);
-      const uint32_t order = dc.T + gi;
-      tile_walk(g, box, [&](int ax, int cy, float tz, int half) {
-        if (!(tz <= AXR_DEPTH_CLEAR)) return;
-        // a tile is larger than an 8-px bin: keep the bin's part of it
-        if (SHIFT < 4 && ((unsigned)(ax - x0) >= (unsigned)BIN || (unsigned)(cy - y0) >= (unsigned)BIN))
-          return;
-        const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
-        if (lc < (unsigned)NPX) atomicMin(&lk[lc], (unsigned long long)make_key(tz, frag_seq(order, half)));
-      });
-    });
-  }
+  if (!empty) walk_bin<SHIFT, false>(lk, rec, B, s_small, s_large, bx, by, dc);
   __syncthreads();
   for (int i = tid; i < NPX; i += nthr) {
     const int x = x0 + (i & (BIN - 1)), y = y0 + (i >> SHIFT);

@@ -43,6 +43,8 @@ struct AxrGroupDev {
   float mip_bias;
   int spec_pow;
   const ushort4* tex;  // texels (R,G,B,0) u16; mip chain smallest first
+  const ushort4* refl;  // SSR groups: reflection cube strip (R,G,B,0), 2*refl_dim x 12*refl_dim
+  int refl_dim;
 };
 
 struct AxrShadowDev {
@@ -250,4 +252,54 @@ AXR_HD uint64_t make_key(float tz, uint32_t seq) {
   float d = (tz == 0.0f) ? 0.0f : tz;
   return ((uint64_t)ordf(d) << 32) | (uint64_t)(~seq);
 }
+// Key of a strict depth test (the first fragment wins a tie).
+AXR_HD uint64_t make_key_first(float tz, uint32_t seq) {
+  float d = (tz == 0.0f) ? 0.0f : tz;
+  return ((uint64_t)ordf(d) << 32) | seq;
+}
 AXR_HD uint32_t frag_seq(uint32_t order, int half) { return 2u * order + (uint32_t)half + 1u; }
+
+// exp(x): Cody-Waite reduction x = k ln2 + r (ln 2 split into a 16-bit head
+// and a tail, so k * head is exact) and the degree-7 Taylor polynomial of
+// e^r in Horner form, scaled by 2^k.  0 for x <= -80 (and NaN), +inf above
+// 88.  The same operation sequence on host and device gives the same bits.
+AXR_HD float exp_cw(float x) {
+  if (!(x > -80.0f)) return 0.0f;
+  if (x > 88.0f) return __builtin_inff();
+  const float k = rintf(x * 1.44269504f);
+  const float r = (x - k * 0.693145751953125f) - k * 1.428606765330187e-6f;
+  float p = 0.0001984127f;
+  p = p * r + 0.0013888889f;
+  p = p * r + 0.008333334f;
+  p = p * r + 0.041666668f;
+  p = p * r + 0.16666667f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  return ldexpf(p, (int)k);
+}
+
+// log2(x): x = m 2^e with m in [sqrt(1/2), sqrt(2)); log2 x = e + ln(m) / ln 2
+// with ln m = 2 atanh(s), s = (m - 1) / (m + 1), by its odd series through
+// s^9 in Horner form.  Exact at powers of two; -inf at 0, NaN below 0 or at
+// NaN, +inf at +inf.  The same operation sequence on host and device gives
+// the same bits (the vendor log2 of host and device libraries differ).
+AXR_HD float log2_cw(float x) {
+  if (x != x || x < 0.0f) return __builtin_nanf("");
+  if (x == 0.0f) return -__builtin_inff();
+  if (x == __builtin_inff()) return x;
+  int e = 0;
+  float m = frexpf(x, &e);
+  if (m < 0.70710678f) {
+    m = m * 2.0f;
+    e -= 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  float q = 0.11111111f;
+  q = q * s2 + 0.14285715f;
+  q = q * s2 + 0.2f;
+  q = q * s2 + 0.33333334f;
+  q = q * s2 + 1.0f;
+  return (float)e + ((2.0f * s) * q) * 1.44269504f;
+}

@@ -35,6 +35,15 @@ struct HostGroup {
   int spec_pow = 16;
   uint32_t flags = 0;  // as last uploaded to the device table
   ushort4* d_tex = nullptr;
+  const ushort4* d_refl = nullptr;  // SSR draws: the group's reflection cube
+  int refl_dim = 0;
+};
+
+// A reflection cube strip (setReflTex, Ops.py:458-467): faces of side S
+// side by side, S rows of 6S texels, stored as (R,G,B,0).
+struct ReflTex {
+  ushort4* d = nullptr;
+  int size = 0;
 };
 
 struct HostShadow {
@@ -128,6 +137,9 @@ struct axr_ctx {
   uint32_t* d_counters = nullptr;
   void* d_post = nullptr;     // blur / bloom intermediates
   size_t post_bytes = 0;
+  void* d_snap = nullptr;     // SSR: copy of the opaque frame (depth, R, G, B)
+  size_t snap_bytes = 0;
+  std::vector<ReflTex> refl;
   float* d_light = nullptr;  // vertLight light arrays
   size_t light_cap = 0;
 

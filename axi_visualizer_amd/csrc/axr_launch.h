@@ -37,6 +37,22 @@ hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, u
 // Smallest-first mip chain of `levels` levels whose last (largest) level
 // is filled: builds the others on the device.
 hipError_t launch_create_mips(ushort4* chain, int levels, hipStream_t s);
+// SSR water pass (ssr.hip): F0/R0/G0/B0 are a copy of the opaque frame.
+struct AxrSsrConst {
+  const float* F0;
+  const uint16_t *R0, *G0, *B0;
+  int steps, step;  // REFL_LENGTH, REFL_STEP
+  float absorb;     // ABSORB
+  int fresnel;      // 0: fr = 1 (the metal variant)
+};
+hipError_t launch_ssr(const AxrTriRec* rec, const AxrBins& B, const float4* vpos,
+                      const float4* vattr, const AxrGroupDev* groups, const AxrDrawConst& dc,
+                      const AxrSsrConst& S, int NBX, float* depth, uint16_t* r, uint16_t* g,
+                      uint16_t* b, hipStream_t s);
+// Six faces' planes [6][S][S] into the cube strip [S][6S] of (R,G,B,0).
+hipError_t launch_pack_cube(const uint16_t* r, const uint16_t* g, const uint16_t* b, int S,
+                            ushort4* strip, hipStream_t s);
+
 // Post passes on the u16 planes (post.hip).  blur: tmp holds
 // 6 * (W/2) * (H/2) u16.  bloom: tmp holds bloom_halves(W, H, levels) fp16.
 hipError_t launch_blur(uint16_t* r, uint16_t* g, uint16_t* b, int W, int H, uint16_t* tmp,

@@ -29,14 +29,22 @@ __device__ __forceinline__ Attr pick3(const Attr& a0, const Attr& a1, const Attr
   return r;
 }
 
-// Rebuilds the winning fragment at (x, y) and shades it.
-__device__ __forceinline__ void resolve_pixel(uint64_t key, int x, int y,
-                                              const AxrTriRec* __restrict__ rec,
-                                              const float4* __restrict__ vpos,
-                                           const float4* __restrict__ vattr,
-                                           const AxrGroupDev* __restrict__ groups,
-                                           const AxrDrawConst& dc, uint16_t rgb[3]) {
-  const uint32_t seq = ~(uint32_t)key;
+// A rebuilt fragment: its group, mip level and the span interpolants.
+struct FragState {
+  AxrGroupDev grp;
+  MipInfo mi;
+  float t, tz;
+  Attr c1, c2;
+};
+
+// Rebuilds fragment seq (frag_seq of its triangle's order and scanline
+// half) at pixel (x, y): the winner's walk replayed up to the pixel's row.
+__device__ __forceinline__ void replay_frag(uint32_t seq, int x, int y,
+                                            const AxrTriRec* __restrict__ rec,
+                                            const float4* __restrict__ vpos,
+                                            const float4* __restrict__ vattr,
+                                            const AxrGroupDev* __restrict__ groups,
+                                            const AxrDrawConst& dc, FragState& f) {
   const uint32_t order = (seq - 1u) >> 1;
   const int half = (int)((seq - 1u) & 1u);
   const bool large = order >= dc.T;
@@ -123,9 +131,24 @@ __device__ __forceinline__ void resolve_pixel(uint64_t key, int x, int y,
       }
     }
   }
-  const float t = span_t(x, cx1, cx2);
-  const float tz = span_tz(t, cz1, cz2);
-  shade(dc, grp, mi, t, tz, c1, c2, rgb);
+  f.t = span_t(x, cx1, cx2);
+  f.tz = span_tz(f.t, cz1, cz2);
+  f.grp = grp;
+  f.mi = mi;
+  f.c1 = c1;
+  f.c2 = c2;
+}
+
+// Rebuilds the winning fragment of a depth key at (x, y) and shades it.
+__device__ __forceinline__ void resolve_pixel(uint64_t key, int x, int y,
+                                              const AxrTriRec* __restrict__ rec,
+                                              const float4* __restrict__ vpos,
+                                              const float4* __restrict__ vattr,
+                                              const AxrGroupDev* __restrict__ groups,
+                                              const AxrDrawConst& dc, uint16_t rgb[3]) {
+  FragState f;
+  replay_frag(~(uint32_t)key, x, y, rec, vpos, vattr, groups, dc, f);
+  shade(dc, f.grp, f.mi, f.t, f.tz, f.c1, f.c2, rgb);
 }
 
 template <int MODE>

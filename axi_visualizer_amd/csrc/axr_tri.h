@@ -655,7 +655,7 @@ __device__ __forceinline__ MipInfo mip_setup(Attr& a1, Attr& a2, Attr& a3, const
   const float s3x = fabsf((a3.u - a2.u) / d3), s3y = fabsf((a3.v - a2.v) / d3);
   float dt = omax(s1x, omax(s1y, omax(s2x, s2y)));
   dt = omax(dt, omax(s3x, s3y));
-  float fmip = omax(0.0f, omin((float)(numMip - 1), fabsf(log2f(dt)) / 2 + bias));
+  float fmip = omax(0.0f, omin((float)(numMip - 1), fabsf(log2_cw(dt)) / 2 + bias));
   MipInfo m;
   m.mip = f2i(fmip);
   m.fmip = fmip - (float)m.mip;

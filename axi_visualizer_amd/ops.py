@@ -27,6 +27,12 @@ _UNSUPPORTED = ("alpha", "multi", "sky", "SSR", "refl", "sub", "emissive", "norm
                 "fog", "2d")
 
 
+def is_ssr(spec) -> bool:
+    """A water shader entry: a dict with an "SSR" key naming the reflection
+    cube map (shaders[tn]["SSR"], Ops.py:859-860)."""
+    return isinstance(spec, dict) and "SSR" in spec
+
+
 def shader_flags(spec) -> int:
     """Maps a reference shader string (matched by substring, as Ops.py:768-805
     does) or an int of AXR_SH_* bits to flag bits."""
@@ -77,6 +83,11 @@ class CLDraw:
         self._flags_buf = None
         self._mask_buf = None
         self._sid_buf = np.zeros(2, np.int32)
+        self.reflTexSize = {}
+        self._refl = {}
+        # REFL_LENGTH, REFL_STEP, ABSORB and the Fresnel term of the water
+        # shader (Shaders_src/av_SSR.c:6-13, 52-53)
+        self.ssrParams = dict(steps=400, step=1, absorb=0.3, fresnel=True)
 
     # -- lifetime
     def close(self):
@@ -194,9 +205,18 @@ class CLDraw:
         if self._flags_buf is None or self._flags_buf.shape[0] != ng:
             self._flags_buf = np.zeros(ng, np.uint32)
             self._mask_buf = np.zeros(ng, np.uint8)
+        water = []
         for tn in range(ng):
+            masked = mask is not None and mask[tn]
+            if is_ssr(shaders[tn]):
+                # drawn after the opaque groups, over their frame
+                self._flags_buf[tn] = 0
+                self._mask_buf[tn] = 1
+                if not masked:
+                    water.append(tn)
+                continue
             self._flags_buf[tn] = shader_flags(shaders[tn])
-            self._mask_buf[tn] = 1 if (mask is not None and mask[tn]) else 0
+            self._mask_buf[tn] = 1 if masked else 0
         sids = [s for s in (shadowIds or []) if s in self.SHADOWMAP][:2]
         ns = len(sids)
         for k, s in enumerate(sids):
@@ -204,6 +224,42 @@ class CLDraw:
         self._chk(self._L.axr_draw_all(self._ctx, self._flags_buf.ctypes.data,
                                        self._mask_buf.ctypes.data, ng,
                                        self._sid_buf.ctypes.data, ns, int(pcf)), "drawAll")
+        if water:
+            names = [shaders[tn]["SSR"] for tn in water]
+            missing = [n for n in names if n not in self._refl]
+            if missing:
+                raise KeyError(f"drawAll: no reflection texture {missing[0]!r} (setReflTex)")
+            grp = np.array(water, np.int32)
+            slot = np.array([self._refl[n] for n in names], np.int32)
+            p = self.ssrParams
+            self._chk(self._L.axr_draw_ssr(self._ctx, ptr(grp), ptr(slot), len(water),
+                                           int(p["steps"]), int(p["step"]), float(p["absorb"]),
+                                           1 if p["fresnel"] else 0), "drawAll (SSR)")
+
+    def setReflTex(self, name, r, g, b, size):
+        """Ops.py:458-467: a reflection cube strip (faces of side `size`,
+        size rows of 6*size texels, Cubemap.py setupTex2's layout) under
+        `name`; SSR groups name it in their shader entry."""
+        n = 6 * int(size) * int(size)
+        planes = [np.ascontiguousarray(np.asarray(x).reshape(-1)[:n].astype(np.uint16))
+                  for x in (r, g, b)]
+        if any(p.size != n for p in planes):
+            raise ValueError(f"setReflTex: planes need 6*size*size = {n} texels")
+        slot = self._refl.get(name, -1)
+        slot = self._chk(self._L.axr_set_refl_tex(self._ctx, slot, *[ptr(p) for p in planes],
+                                                  int(size)), "setReflTex")
+        self._refl[name] = slot
+        self.reflTexSize[name] = np.int32(int(size) // 2)
+        return len(self.reflTexSize) - 1
+
+    def getReflTex(self, name):
+        """The cube strip `name` as stored on the device: (r, g, b) planes of
+        shape (size, 6 * size)."""
+        S = 2 * int(self.reflTexSize[name])
+        out = [np.empty((S, 6 * S), np.uint16) for _ in range(3)]
+        self._chk(self._L.axr_get_refl_tex(self._ctx, self._refl[name], *[ptr(o) for o in out]),
+                  "getReflTex")
+        return tuple(out)
 
     def gamma(self, ex):
         """Ops.py:947-957 -> Shaders/Post/gamma.c."""

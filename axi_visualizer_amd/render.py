@@ -32,6 +32,16 @@ def load_scene(d: CLDraw, sc: Scene):
     for i, c in enumerate(sc.cascades):
         d.addShadowMap(i, c.dim, c.cam.scale, sc.amb)
         d.placeShadowMap(i, c.cam.pos, c.cam.vm, sc.amb)
+    for k, strip in enumerate(sc.refl):
+        d.setReflTex(k, strip[..., 0], strip[..., 1], strip[..., 2], strip.shape[0])
+    d.ssrParams = dict(steps=sc.ssr_steps, step=sc.ssr_step, absorb=sc.ssr_absorb,
+                       fresnel=sc.ssr_fresnel)
+
+
+def scene_shaders(sc: Scene):
+    """drawAll shader entries of a Scene's groups: AXR_SH_* flags, or
+    {"SSR": k} for a water group reflecting cube map k."""
+    return [{"SSR": g.ssr} if g.ssr >= 0 else g.flags for g in sc.groups]
 
 
 class Renderer:
@@ -39,7 +49,7 @@ class Renderer:
         self.scene = sc = scene
         self.draw = d = CLDraw(1, len(sc.groups), sc.W, sc.H, 1)
         load_scene(d, sc)
-        self.shaders = [g.flags for g in sc.groups]
+        self.shaders = scene_shaders(sc)
         self.mask = [g.skip for g in sc.groups]
         self.cast = [g.cast for g in sc.groups]
 

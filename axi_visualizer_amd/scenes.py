@@ -122,6 +122,7 @@ class Group:
     spec_pow: int = 16
     cast: bool = True
     skip: bool = False
+    ssr: int = -1            # >= 0: SSR water reflecting scene.refl[ssr]
 
     @property
     def ntri(self) -> int:
@@ -163,6 +164,11 @@ class Scene:
     pcf_taps: int = 5
     exposure: float = 1.0
     clear_rgb: tuple = (1200, 4033, 21675)
+    refl: List[np.ndarray] = dataclasses.field(default_factory=list)  # (S, 6S, 3) u16 strips
+    ssr_steps: int = 400     # REFL_LENGTH (av_SSR.c:7)
+    ssr_step: int = 1        # REFL_STEP
+    ssr_absorb: float = 0.3  # ABSORB
+    ssr_fresnel: bool = True
 
     @property
     def ntri(self) -> int:
@@ -421,6 +427,72 @@ def spheres_scene(W=1920, H=1080, n_inst=2000, n_tex=8, tex_side=1024, seed=1,
                  light_int=np.array([0.9, 0.85, 0.8], np.float32),
                  light_dir=view_vec(la, lb).astype(np.float32), amb=0.2,
                  cascades=cascades, pcf_taps=pcf_taps)
+
+
+# -------------------------------------------------------------- water
+def water_plane(x0: float, x1: float, z0: float, z1: float, y: float, n: int = 8,
+                uv_repeat: float = 4.0):
+    """A flat n x n quad grid at height y with inward (-y) normals, split as
+    terrain_mesh splits its cells."""
+    h = np.full((n + 1, n + 1), float(y))
+    xyz, vn, uv = terrain_mesh(h, 1.0, uvspread=(n / uv_repeat))
+    xyz = xyz.astype(np.float64)
+    xyz[:, 0] = x0 + xyz[:, 0] * ((x1 - x0) / n)
+    xyz[:, 2] = z0 + xyz[:, 2] * ((z1 - z0) / n)
+    return xyz.astype(np.float32), vn, uv
+
+
+def cube_strip(seed: int, side: int) -> np.ndarray:
+    """A seeded reflection cube strip: side rows of 6 * side u16 texels (a
+    smooth sky gradient per face plus noise), as setReflTex receives it."""
+    y = np.arange(side)[:, None] / max(side - 1, 1)
+    base = np.empty((side, 6 * side, 3))
+    for f in range(6):
+        for c in range(3):
+            base[:, f * side:(f + 1) * side, c] = 2000.0 + 9000.0 * (0.3 + 0.7 * y) * (
+                0.5 + 0.5 * ((f + c) % 3) / 2.0)
+    n = rand_uniform(seed, 4100, side * 6 * side * 3).reshape(side, 6 * side, 3)
+    return np.clip(base * (0.85 + 0.3 * n), 0, 65535).astype(np.uint16)
+
+
+def water_scene(W=320, H=240, grid=48, extent=60.0, seed=1, n_inst=24, refl_side=32,
+                shadow_dim=0, water_level=-1.0, ssr_steps=400, name="water") -> Scene:
+    """Terrain with a few spheres and an SSR water plane at water_level over
+    the whole terrain (the valleys below it are under water), reflecting a
+    seeded cube strip."""
+    t = np.arange(grid) / (grid - 1)
+    xs, zs = np.meshgrid(t, t, indexing="ij")
+    h = value_noise(seed, xs, zs)
+    spacing = extent / (grid - 1)
+    xyz, vn, uv = terrain_mesh(h, spacing)
+    flags = SH_MIP | SH_PHONG | (SH_SHADOW if shadow_dim else 0)
+    tex, side = texture_for(noise_texture(seed, 77, 64), True)
+    groups = [Group(xyz=xyz, vn=vn, uv=uv, tex=tex, tex_side=side, mip=True, flags=flags)]
+    if n_inst:
+        sx, sv, su = sphere_instances(n_inst, seed, box=extent * 0.8, smin=1.0, smax=3.0)
+        sx = sx.reshape(-1, 3) + np.array([extent / 2, 8.0, extent / 2])
+        stex, sside = texture_for(noise_texture(seed, 300, 64), True)
+        groups.append(Group(xyz=sx.astype(np.float32), vn=sv.reshape(-1, 3).astype(np.float32),
+                            uv=np.ascontiguousarray(su).reshape(-1, 2).astype(np.float32),
+                            tex=stex, tex_side=sside, mip=True, flags=flags))
+    wx, wn, wu = water_plane(0.0, extent, 0.0, extent, water_level, n=8)
+    wtex, wside = texture_for(noise_texture(seed, 90, 64), False)
+    groups.append(Group(xyz=wx, vn=wn, uv=wu, tex=(wtex // 4 + 500).astype(np.uint16),
+                        tex_side=wside, mip=False, flags=0, cast=False, ssr=0))
+    centre = np.array([extent / 2, float(h.mean()), extent / 2])
+    yaw, pitch = pi / 4, pi / 7
+    pos = centre - extent * 0.9 * view_vec(yaw, pitch)
+    la, lb = 0.5, 0.9
+    cascades = []
+    if shadow_dim:
+        cascades = [make_cascade(shadow_dim, centre, extent * 0.8, la, lb),
+                    make_cascade(shadow_dim, centre, extent * 1.6, la, lb)]
+    return Scene(name=name, W=W, H=H, groups=groups,
+                 view=make_camera(W, H, pos, yaw, pitch),
+                 light_int=np.array([0.9, 0.85, 0.8], np.float32),
+                 light_dir=view_vec(la, lb).astype(np.float32), amb=0.2, cascades=cascades,
+                 pcf_taps=9 if shadow_dim else 5, refl=[cube_strip(seed, refl_side)],
+                 ssr_steps=ssr_steps)
 
 
 def random_tris_scene(W=256, H=192, ntri=400, seed=3, size=30.0, depth=(5.0, 40.0),

@@ -106,6 +106,23 @@ int axr_shadow_map(axr_ctx* ctx, int i, const uint8_t* cast, int ngroups, float 
 /* CLDraw.gamma(ex), Ops.py:947-957 -> Shaders/Post/gamma.c. */
 int axr_gamma(axr_ctx* ctx, float ex);
 
+/* CLDraw.setReflTex(name, r, g, b, size), Ops.py:458-467: a reflection
+ * cube strip of faces of side `size` (size rows of 6*size texels, as
+ * Cubemap.py's setupTex2 lays them out), u16 planes.  slot < 0 adds a new
+ * cube map; returns its slot (>= 0) or an error code. */
+int axr_set_refl_tex(axr_ctx* ctx, int slot, const uint16_t* r, const uint16_t* g,
+                     const uint16_t* b, int size);
+/* The cube strip in `slot` as stored on the device; returns the face size. */
+int axr_get_refl_tex(axr_ctx* ctx, int slot, uint16_t* r, uint16_t* g, uint16_t* b);
+/* The SSR water draw of drawAll (Ops.py:788, 859-869, 910-924 dispatch
+ * groups whose shader holds "SSR" to Shaders_src/av_SSR.c):
+ * groups[k] drawn with reflection cube refl[k] over the frame drawn so far
+ * (the opaque drawAll).  steps = REFL_LENGTH, step = REFL_STEP, absorb =
+ * ABSORB, fresnel = 0 for the metal variant's fr = 1.  The frame stays
+ * HDR; getIds is not available after it. */
+int axr_draw_ssr(axr_ctx* ctx, const int* groups, const int* refl, int n, int steps, int step,
+                 float absorb, int fresnel);
+
 /* CLDraw.blur(), Ops.py:1009-1020 -> Shaders/Post/blur.c (blurH) and
  * blur2.c (blurV): brightness-weighted half-resolution copy, 19-tap
  * horizontal and vertical blur, bilinear upsample added to the frame's

@@ -65,6 +65,29 @@ inline float pown_(float x, int n) {
   }
   return r;
 }
+// log2(x): x = m 2^e with m in [sqrt(1/2), sqrt(2)); log2 x = e + ln(m)/ln 2,
+// ln m = 2 atanh(s), s = (m - 1)/(m + 1), by its odd series through s^9.
+// Decree for OpenCL's log2 (the vendor's): exact at powers of two, within a
+// few ulp elsewhere; -inf at 0, NaN below 0 or at NaN, +inf at +inf.
+inline float log2_cw(float x) {
+  if (x != x || x < 0.0f) return NAN;
+  if (x == 0.0f) return -INFINITY;
+  if (x == INFINITY) return x;
+  int e = 0;
+  float m = std::frexp(x, &e);
+  if (m < 0.70710678f) {
+    m = m * 2.0f;
+    e -= 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  float q = 0.11111111f;
+  q = q * s2 + 0.14285715f;
+  q = q * s2 + 0.2f;
+  q = q * s2 + 0.33333334f;
+  q = q * s2 + 1.0f;
+  return (float)e + ((2.0f * s) * q) * 1.44269504f;
+}
 inline int abs_mask(int v, int m) {
   unsigned u = v < 0 ? 0u - (unsigned)v : (unsigned)v;
   return (int)(u & (unsigned)m);
@@ -295,7 +318,7 @@ MipInfo mip_setup(Attr a[3], const float z[3], T x1, T y1, T x2, T y2, T x3, T y
   float s3y = std::fabs((a[2].v[1] - a[1].v[1]) / d3);
   float dt = omax(s1x, omax(s1y, omax(s2x, s2y)));
   dt = omax(dt, omax(s3x, s3y));
-  float fmip = omax(0.0f, omin((float)(numMip - 1), std::fabs(std::log2(dt)) / 2 + mipBias));
+  float fmip = omax(0.0f, omin((float)(numMip - 1), std::fabs(log2_cw(dt)) / 2 + mipBias));
   MipInfo m;
   m.mip = f2i(fmip);
   m.fmip = fmip - (float)m.mip;
@@ -424,6 +447,138 @@ void shade(const Ctx& cx, const OrcGroup& g, const MipInfo& mi, float t, float t
     // av_draw_base.c:44-55
     for (int c = 0; c < 3; c++)
       rgb[c] = f2u16(L[c] * (light * (dc[c] + cc[c]) + (1 - light) * cc[c]));
+  }
+}
+
+// exp(x): Cody-Waite reduction x = k ln2 + r (ln 2 split into a 16-bit head
+// and a tail) and the degree-7 Taylor polynomial of e^r in Horner form,
+// times 2^k; 0 for x <= -80 (and NaN), +inf above 88.
+float exp_cw(float x) {
+  if (!(x > -80.0f)) return 0.0f;
+  if (x > 88.0f) return INFINITY;
+  const float k = std::rint(x * 1.44269504f);
+  const float r = (x - k * 0.693145751953125f) - k * 1.428606765330187e-6f;
+  float p = 0.0001984127f;
+  p = p * r + 0.0013888889f;
+  p = p * r + 0.008333334f;
+  p = p * r + 0.041666668f;
+  p = p * r + 0.16666667f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  return std::ldexp(p, (int)k);
+}
+
+// The opaque frame as the SSR pass reads it.
+struct Snap {
+  std::vector<float> F;
+  std::vector<uint16_t> R, G, B;
+};
+
+// Shaders_src/av_SSR.c:37-148 (water: REFL_STEP, REFL_LENGTH and ABSORB
+// from the frame, REFL_DEPTH 2, REFL_VSIZE 2, REFL_DBIAS 0.5, REFL_FADE 80,
+// maxattn 0.1).  Frame reads (own pixel, march, hit) are of the snapshot.
+// Decrees: fast_normalize = normalize; exp = exp_cw; float->ushort stores
+// saturate toward zero; if no cube face matches (NaN direction) face 0 is
+// sampled at the clamped (ax, cy) the shader's rxy still holds.
+void shade_ssr(const Ctx& cx, const OrcGroup& g, const Snap& S, int ax, int cy, float t,
+               float tz, const Attr& c1, const Attr& c2, uint16_t rgb[3]) {
+  const OrcFrame& fr = *cx.fr;
+  const int wF = fr.W, hF = fr.H;
+  const int64_t p = (int64_t)wF * cy + ax;
+  const float u = 1 - t;
+  const f3 pos = {(u * c2.v[8] + t * c1.v[8]) * tz, (u * c2.v[9] + t * c1.v[9]) * tz,
+                  (u * c2.v[10] + t * c1.v[10]) * tz};
+  const f3 norm = normalize3({(u * c2.v[5] + t * c1.v[5]) * tz, (u * c2.v[6] + t * c1.v[6]) * tz,
+                              (u * c2.v[7] + t * c1.v[7]) * tz});
+  const int lenT = g.tex_size;
+  float texr1 = ((u * c2.v[0] + t * c1.v[0]) * tz) * (float)lenT;
+  int tex1 = abs_mask(f2i(texr1), lenT - 1);
+  float texr2 = ((float)lenT * (u * c2.v[1] + t * c1.v[1])) * tz;
+  int tex2 = abs_mask(f2i(texr2), lenT - 1);
+  const int tex = tex1 + lenT * tex2;
+  const f3 vp = {fr.view.pos[0], fr.view.pos[1], fr.view.pos[2]};
+  float frs = 1.0f;
+  if (fr.ssr_fresnel) {
+    frs = 1 - omax(0.0f, dot3(normalize3(sub3(pos, vp)), norm)) * 0.1f;
+    for (int k = 0; k < 6; k++) frs *= frs;
+  }
+  const float scatter = exp_cw(fr.ssr_absorb * (tz - S.F[p]));
+  const float tsr = scatter * (float)S.R[p] + (1 - scatter) * (float)g.tr[tex];
+  const float tsg = scatter * (float)S.G[p] + (1 - scatter) * (float)g.tg[tex];
+  const float tsb = scatter * (float)S.B[p] + (1 - scatter) * (float)g.tb[tex];
+
+  const f3 a = sub3(pos, vp);
+  const float nd = dot3(a, norm);
+  const f3 refl = scl3(normalize3(sub3(a, scl3(norm, 2 * nd))), 2.0f);
+  const bool test = refl.y < 0;
+  const f3 SVd = {fr.view.vm[0], fr.view.vm[1], fr.view.vm[2]};
+  const f3 SVx = {fr.view.vm[3], fr.view.vm[4], fr.view.vm[5]};
+  const f3 SVy = {fr.view.vm[6], fr.view.vm[7], fr.view.vm[8]};
+  float rxyx = (float)ax, rxyy = (float)cy;
+  const f3 ar = add3(a, refl);
+  const float rpz = dot3(ar, SVd);
+  const float rpx = dot3(ar, SVx) * -fr.view.scale / rpz + (float)(wF / 2);
+  const float rpy = dot3(ar, SVy) * fr.view.scale / rpz + (float)(hF / 2);
+  const float dt = omax(std::fabs(rpx - rxyx), std::fabs(rpy - rxyy));
+  const int vx = fr.ssr_step;
+  const float slopex = (rpx - rxyx) / dt * (float)vx;
+  const float slopey = (rpy - rxyy) / dt * (float)vx;
+  const float slopez = (1.f / rpz - 1.f / tz) / dt * (float)vx;
+  float sy = (float)cy, sx = (float)ax, sz = 1.f / tz, sd = 2.f;
+  int rn = 0, hit = 0, lx = 0, ly = 0;
+  for (rn = 0; (hit == 0) && (rn < fr.ssr_steps) && (sx >= 0) && (sx < (float)wF) && (sy >= 0) &&
+               (sy < (float)hF) && (sz > 0);
+       rn += vx) {
+    const float f = S.F[(int64_t)wF * f2i(sy) + f2i(sx)];
+    if ((f < 1.f / sz) && (f > 1.f / sz - sd)) {
+      lx = f2i(sx);
+      ly = f2i(sy);
+      hit = 1;
+    }
+    sx += slopex;
+    sy += slopey;
+    sz += slopez;
+    sd = std::fabs(1.f / sz - 1.f / (sz - slopez)) + 0.5f;
+  }
+  int face = 0;
+  const float m = omax(std::fabs(refl.x), omax(std::fabs(refl.y), std::fabs(refl.z)));
+  if (m == std::fabs(refl.x)) {
+    face = (refl.x > 0) ? 0 : 3;
+    rxyx = refl.z / refl.x;
+    rxyy = refl.y / refl.x;
+  }
+  if (m == std::fabs(refl.y)) {
+    face = (refl.y > 0) ? 1 : 4;
+    rxyx = refl.x / refl.y;
+    rxyy = refl.z / refl.y;
+  }
+  if (m == std::fabs(refl.z)) {
+    face = (refl.z > 0) ? 2 : 5;
+    rxyx = refl.y / refl.z;
+    rxyy = refl.x / refl.z;
+  }
+  const int dimR = fr.refl_size[g.ssr] / 2;
+  rxyx = omin((rxyx + 1) * (float)dimR, (float)(dimR * 2 - 1));
+  rxyy = omin((rxyy + 1) * (float)dimR, (float)(dimR * 2 - 1));
+  const int lenR = 2 * dimR;
+  const int rex1 = f2i(rxyx) + face * lenR;
+  const int rex2 = f2i(rxyy);
+  const int64_t rex = rex1 + 6 * (int64_t)lenR * rex2;
+  const float RRc = fr.refl_r[g.ssr][rex], RGc = fr.refl_g[g.ssr][rex],
+              RBc = fr.refl_b[g.ssr][rex];
+  const float fade = omin(1.f, (float)(fr.ssr_steps - rn) / (float)80);
+  if (test) frs = 0.f;
+  if (hit) {
+    const int64_t h = (int64_t)wF * ly + lx;
+    const float ox = S.R[h], oy = S.G[h], oz = S.B[h];
+    rgb[0] = f2u16((1 - frs) * tsr + frs * (fade * ox + (1 - fade) * RRc));
+    rgb[1] = f2u16((1 - frs) * tsg + frs * (fade * oy + (1 - fade) * RGc));
+    rgb[2] = f2u16((1 - frs) * tsb + frs * (fade * oz + (1 - fade) * RBc));
+  } else {
+    rgb[0] = f2u16((1 - frs) * tsr + frs * RRc);
+    rgb[1] = f2u16((1 - frs) * tsg + frs * RGc);
+    rgb[2] = f2u16((1 - frs) * tsb + frs * RBc);
   }
 }
 
@@ -880,6 +1035,152 @@ int64_t render_shadow(Ctx& cx, int c, int nthreads) {
 }  // namespace
 
 // =================================================================== C ABI
+namespace {
+
+// One drawAll pass over the groups of one kind (water = the SSR groups)
+// in the reference's order: setup, small triangles per 16-row band in
+// global order, then large triangles per group and tile.  on_frag(G, mi,
+// gi, ax, cy, t, tz, c1, c2) sees every fragment in that order (per pixel).
+template <class OnFrag>
+void draw_pass(Ctx& cx, bool water, int nthreads, int64_t st[5], OnFrag&& on_frag) {
+  const OrcGroup* groups = cx.groups;
+  const int ngroups = cx.ngroups;
+  const OrcFrame* fr = cx.fr;
+  const int W = fr->W, H = fr->H;
+  const int64_t T = cx.base[ngroups];
+  // Triangle setup for every group drawn in this pass.
+  cx.P.assign(3 * T, f2{0, 0});
+  cx.Z.assign(3 * T, 0.0f);
+  cx.cls.assign(T, 0);
+  parallel_for((int)((T + 4095) / 4096), nthreads, [&](int blk) {
+    int64_t lo = (int64_t)blk * 4096, hi = std::min<int64_t>(T, lo + 4096);
+    for (int64_t gi = lo; gi < hi; gi++) {
+      int g = (int)(std::upper_bound(cx.base.begin(), cx.base.end(), gi) - cx.base.begin()) - 1;
+      const OrcGroup& G = groups[g];
+      if (G.skip || (G.ssr >= 0) != water) continue;
+      cx.cls[gi] = (int8_t)setup_one(G.xyz, G.vn, gi - cx.base[g], fr->view, W, H,
+                                     (G.flags & ORC_CULL) ? 1 : 0, &cx.P[3 * gi], &cx.Z[3 * gi]);
+    }
+  });
+
+  // Small triangles bucketed into 16-row bands in pass order.
+  const int nb = (H + BAND - 1) / BAND;
+  std::vector<std::vector<uint32_t>> sbands(nb);
+  int64_t nsmall = 0, nlarge = 0;
+  for (int64_t gi = 0; gi < T; gi++) {
+    if (cx.cls[gi] == 1) {
+      nsmall++;
+      int y[3];
+      for (int k = 0; k < 3; k++) y[k] = f2i(cx.P[3 * gi + k].y);
+      bucket_rows(sbands, (uint32_t)gi, std::min(y[0], std::min(y[1], y[2])),
+                  std::max(y[0], std::max(y[1], y[2])), H);
+    } else if (cx.cls[gi] == 2) {
+      nlarge++;
+    }
+  }
+  st[0] = nsmall;
+  st[1] = nlarge;
+
+  // Coarse bins per group (Ops0.py:614-643: coarse then tiled, per group).
+  const int WC = W / 16, HC = H / 16, NT = WC * HC;
+  std::vector<std::vector<std::vector<uint32_t>>> bins(ngroups);
+  int64_t maxbin = 0, entries = 0;
+  for (int g = 0; g < ngroups; g++) {
+    bool any = false;
+    for (int64_t gi = cx.base[g]; gi < cx.base[g + 1]; gi++)
+      if (cx.cls[gi] == 2) { any = true; break; }
+    if (!any) continue;
+    bins[g].assign(NT, {});
+    for (int64_t gi = cx.base[g]; gi < cx.base[g + 1]; gi++) {
+      if (cx.cls[gi] != 2) continue;
+      coarse_walk(&cx.P[3 * gi], WC, HC, [&](int t) { bins[g][t].push_back((uint32_t)gi); });
+    }
+    for (auto& b : bins[g]) {
+      maxbin = std::max<int64_t>(maxbin, (int64_t)b.size());
+      entries += (int64_t)b.size();
+    }
+  }
+  st[3] = maxbin;
+  st[4] = entries;
+
+  std::atomic<int64_t> frags{0};
+  parallel_for(nb, nthreads, [&](int b) {
+    const int rlo = b * BAND, rhi = std::min(H, rlo + BAND);
+    int64_t nf = 0;
+    // -- small pass, all groups in order
+    for (uint32_t gi : sbands[b]) {
+      int g = (int)(std::upper_bound(cx.base.begin(), cx.base.end(), (int64_t)gi) -
+                    cx.base.begin()) - 1;
+      const OrcGroup& G = groups[g];
+      const int64_t li = gi - cx.base[g];
+      SmallTri s;
+      s.x1 = f2i(cx.P[3 * gi].x); s.y1 = f2i(cx.P[3 * gi].y);
+      s.x2 = f2i(cx.P[3 * gi + 1].x); s.y2 = f2i(cx.P[3 * gi + 1].y);
+      s.x3 = f2i(cx.P[3 * gi + 2].x); s.y3 = f2i(cx.P[3 * gi + 2].y);
+      s.z1 = cx.Z[3 * gi]; s.z2 = cx.Z[3 * gi + 1]; s.z3 = cx.Z[3 * gi + 2];
+      s.a1 = a_scale(raw_attr(G, 3 * li), s.z1);
+      s.a2 = a_scale(raw_attr(G, 3 * li + 1), s.z2);
+      s.a3 = a_scale(raw_attr(G, 3 * li + 2), s.z3);
+      small_sort(s);
+      small_fourth(s);
+      MipInfo mi;
+      if (G.flags & ORC_MIP) {
+        Attr a[3] = {s.a1, s.a2, s.a3};
+        float z[3] = {s.z1, s.z2, s.z3};
+        mi = mip_setup<int>(a, z, s.x1, s.y1, s.x2, s.y2, s.x3, s.y3, G.tex_size, G.mip_bias);
+        s.a1 = a[0]; s.a2 = a[1]; s.a3 = a[2];
+      }
+      raster_small(s, W, H, rlo, rhi,
+                   [&](int ax, int cy, float t, float tz, const Attr& c1, const Attr& c2, int) {
+                     nf++;
+                     on_frag(G, mi, gi, ax, cy, t, tz, c1, c2);
+                   });
+    }
+    // -- large pass, per group: tiles of this band row
+    if (b < HC) {
+      for (int g = 0; g < ngroups; g++) {
+        if (bins[g].empty()) continue;
+        const OrcGroup& G = groups[g];
+        for (int tx = 0; tx < WC; tx++) {
+          const int bx = b * WC + tx;
+          TileBox tb = tile_box(bx, W);
+          for (uint32_t gi : bins[g][bx]) {
+            const int64_t li = gi - cx.base[g];
+            TileTri s;
+            s.x1 = cx.P[3 * gi].x; s.y1 = cx.P[3 * gi].y;
+            s.x2 = cx.P[3 * gi + 1].x; s.y2 = cx.P[3 * gi + 1].y;
+            s.x3 = cx.P[3 * gi + 2].x; s.y3 = cx.P[3 * gi + 2].y;
+            s.z1 = cx.Z[3 * gi]; s.z2 = cx.Z[3 * gi + 1]; s.z3 = cx.Z[3 * gi + 2];
+            s.a1 = a_scale(raw_attr(G, 3 * li), s.z1);
+            s.a2 = a_scale(raw_attr(G, 3 * li + 1), s.z2);
+            s.a3 = a_scale(raw_attr(G, 3 * li + 2), s.z3);
+            tile_sort(s);
+            tile_fourth(s);
+            MipInfo mi;
+            if (G.flags & ORC_MIP) {
+              Attr a[3] = {s.a1, s.a2, s.a3};
+              float z[3] = {s.z1, s.z2, s.z3};
+              mi = mip_setup<float>(a, z, s.x1, s.y1, s.x2, s.y2, s.x3, s.y3, G.tex_size,
+                                    G.mip_bias);
+              s.a1 = a[0]; s.a2 = a[1]; s.a3 = a[2];
+            }
+            raster_tile(s, tb,
+                        [&](int ax, int cy, float t, float tz, const Attr& c1, const Attr& c2,
+                            int) {
+                          nf++;
+                          on_frag(G, mi, gi, ax, cy, t, tz, c1, c2);
+                        });
+          }
+        }
+      }
+    }
+    frags += nf;
+  });
+  st[2] = frags.load();
+}
+
+}  // namespace
+
 extern "C" {
 
 int orc_trisetup(const float* xyz, const float* vn, int ntri, const OrcCam* cam, int W, int H,
@@ -1078,6 +1379,12 @@ int orc_blur(uint16_t* R, uint16_t* G, uint16_t* B, int W, int H) {
   return 0;
 }
 
+int orc_math(int which, const float* x, float* y, int64_t n) {
+  if (which != 0 && which != 1) return -1;
+  for (int64_t i = 0; i < n; i++) y[i] = which == 0 ? log2_cw(x[i]) : exp_cw(x[i]);
+  return 0;
+}
+
 int orc_create_mips(const uint16_t* plane, int N, uint16_t* out) {
   // Utils.py:79-93: 2x2 box sum >> 2 per level; levels concatenated
   // smallest first.
@@ -1154,7 +1461,13 @@ int orc_render(const OrcGroup* groups, int ngroups, const OrcFrame* fr, OrcOut* 
   cx.fr = fr;
   cx.base.assign(ngroups + 1, 0);
   for (int g = 0; g < ngroups; g++) cx.base[g + 1] = cx.base[g] + groups[g].ntri;
-  const int64_t T = cx.base[ngroups];
+  for (int g = 0; g < ngroups; g++) {
+    const OrcGroup& G = groups[g];
+    if (G.ssr < 0 || G.skip) continue;
+    if (G.ssr >= 4 || !fr->refl_r[G.ssr] || !fr->refl_g[G.ssr] || !fr->refl_b[G.ssr] ||
+        fr->refl_size[G.ssr] < 2 || (G.flags & ORC_MIP) || fr->ssr_steps < 1 || fr->ssr_step < 1)
+      return -3;
+  }
   for (int i = 0; i < 8; i++) out->stats[i] = 0;  // 5, 6: shadow fragments per cascade
 
   // Shadow cascades (Ops.py:1089-1141 per map), before the main pass.
@@ -1173,153 +1486,49 @@ int orc_render(const OrcGroup* groups, int ngroups, const OrcFrame* fr, OrcOut* 
   cx.G.assign(NP, fr->clear_rgb[1]);
   cx.B.assign(NP, fr->clear_rgb[2]);
 
-  // Triangle setup for every drawn group.
-  cx.P.assign(3 * T, f2{0, 0});
-  cx.Z.assign(3 * T, 0.0f);
-  cx.cls.assign(T, 0);
-  parallel_for((int)((T + 4095) / 4096), nthreads, [&](int blk) {
-    int64_t lo = (int64_t)blk * 4096, hi = std::min<int64_t>(T, lo + 4096);
-    for (int64_t gi = lo; gi < hi; gi++) {
-      int g = (int)(std::upper_bound(cx.base.begin(), cx.base.end(), gi) - cx.base.begin()) - 1;
-      const OrcGroup& G = groups[g];
-      if (G.skip) continue;
-      cx.cls[gi] = (int8_t)setup_one(G.xyz, G.vn, gi - cx.base[g], fr->view, W, H,
-                                     (G.flags & ORC_CULL) ? 1 : 0, &cx.P[3 * gi], &cx.Z[3 * gi]);
+  int64_t st[5];
+  draw_pass(cx, false, nthreads, st, [&](const OrcGroup& G, const MipInfo& mi, uint32_t gi,
+                                          int ax, int cy, float t, float tz, const Attr& c1,
+                                          const Attr& c2) {
+    const int64_t p = (int64_t)W * cy + ax;
+    if (cx.F[p] >= tz) {
+      cx.F[p] = tz;
+      cx.id[p] = gi;
+      uint16_t rgb[3];
+      shade(cx, G, mi, t, tz, c1, c2, rgb);
+      cx.R[p] = rgb[0];
+      cx.G[p] = rgb[1];
+      cx.B[p] = rgb[2];
     }
   });
+  for (int k = 0; k < 5; k++) out->stats[k] = st[k];
 
-  // Small triangles bucketed into 16-row bands in pass order.
-  const int nb = (H + BAND - 1) / BAND;
-  std::vector<std::vector<uint32_t>> sbands(nb);
-  int64_t nsmall = 0, nlarge = 0;
-  for (int64_t gi = 0; gi < T; gi++) {
-    if (cx.cls[gi] == 1) {
-      nsmall++;
-      int y[3];
-      for (int k = 0; k < 3; k++) y[k] = f2i(cx.P[3 * gi + k].y);
-      bucket_rows(sbands, (uint32_t)gi, std::min(y[0], std::min(y[1], y[2])),
-                  std::max(y[0], std::max(y[1], y[2])), H);
-    } else if (cx.cls[gi] == 2) {
-      nlarge++;
-    }
-  }
-  out->stats[0] = nsmall;
-  out->stats[1] = nlarge;
-
-  // Coarse bins per group (Ops0.py:614-643: coarse then tiled, per group).
-  const int WC = W / 16, HC = H / 16, NT = WC * HC;
-  std::vector<std::vector<std::vector<uint32_t>>> bins(ngroups);
-  int64_t maxbin = 0, entries = 0;
-  for (int g = 0; g < ngroups; g++) {
-    bool any = false;
-    for (int64_t gi = cx.base[g]; gi < cx.base[g + 1]; gi++)
-      if (cx.cls[gi] == 2) { any = true; break; }
-    if (!any) continue;
-    bins[g].assign(NT, {});
-    for (int64_t gi = cx.base[g]; gi < cx.base[g + 1]; gi++) {
-      if (cx.cls[gi] != 2) continue;
-      coarse_walk(&cx.P[3 * gi], WC, HC, [&](int t) { bins[g][t].push_back((uint32_t)gi); });
-    }
-    for (auto& b : bins[g]) {
-      maxbin = std::max<int64_t>(maxbin, (int64_t)b.size());
-      entries += (int64_t)b.size();
-    }
-  }
-  out->stats[3] = maxbin;
-  out->stats[4] = entries;
-
-  std::atomic<int64_t> frags{0};
-  parallel_for(nb, nthreads, [&](int b) {
-    const int rlo = b * BAND, rhi = std::min(H, rlo + BAND);
-    int64_t nf = 0;
-    // -- small pass, all groups in order
-    for (uint32_t gi : sbands[b]) {
-      int g = (int)(std::upper_bound(cx.base.begin(), cx.base.end(), (int64_t)gi) -
-                    cx.base.begin()) - 1;
-      const OrcGroup& G = groups[g];
-      const int64_t li = gi - cx.base[g];
-      SmallTri s;
-      s.x1 = f2i(cx.P[3 * gi].x); s.y1 = f2i(cx.P[3 * gi].y);
-      s.x2 = f2i(cx.P[3 * gi + 1].x); s.y2 = f2i(cx.P[3 * gi + 1].y);
-      s.x3 = f2i(cx.P[3 * gi + 2].x); s.y3 = f2i(cx.P[3 * gi + 2].y);
-      s.z1 = cx.Z[3 * gi]; s.z2 = cx.Z[3 * gi + 1]; s.z3 = cx.Z[3 * gi + 2];
-      s.a1 = a_scale(raw_attr(G, 3 * li), s.z1);
-      s.a2 = a_scale(raw_attr(G, 3 * li + 1), s.z2);
-      s.a3 = a_scale(raw_attr(G, 3 * li + 2), s.z3);
-      small_sort(s);
-      small_fourth(s);
-      MipInfo mi;
-      if (G.flags & ORC_MIP) {
-        Attr a[3] = {s.a1, s.a2, s.a3};
-        float z[3] = {s.z1, s.z2, s.z3};
-        mi = mip_setup<int>(a, z, s.x1, s.y1, s.x2, s.y2, s.x3, s.y3, G.tex_size, G.mip_bias);
-        s.a1 = a[0]; s.a2 = a[1]; s.a3 = a[2];
+  // SSR water groups over the opaque frame (av_SSR.c): the shader reads a
+  // copy of the opaque frame; the strict test keeps the nearest water
+  // fragment, the first in draw order on a tie.
+  bool any_water = false;
+  for (int g = 0; g < ngroups; g++) any_water |= !groups[g].skip && groups[g].ssr >= 0;
+  if (any_water) {
+    Snap S;
+    S.F = cx.F;
+    S.R = cx.R;
+    S.G = cx.G;
+    S.B = cx.B;
+    int64_t wst[5];
+    draw_pass(cx, true, nthreads, wst, [&](const OrcGroup& G, const MipInfo&, uint32_t,
+                                           int ax, int cy, float t, float tz, const Attr& c1,
+                                           const Attr& c2) {
+      const int64_t p = (int64_t)W * cy + ax;
+      if (cx.F[p] > tz) {
+        cx.F[p] = tz;
+        uint16_t rgb[3];
+        shade_ssr(cx, G, S, ax, cy, t, tz, c1, c2, rgb);
+        cx.R[p] = rgb[0];
+        cx.G[p] = rgb[1];
+        cx.B[p] = rgb[2];
       }
-      raster_small(s, W, H, rlo, rhi,
-                   [&](int ax, int cy, float t, float tz, const Attr& c1, const Attr& c2, int) {
-                     nf++;
-                     int64_t p = (int64_t)W * cy + ax;
-                     if (cx.F[p] >= tz) {
-                       cx.F[p] = tz;
-                       cx.id[p] = gi;
-                       uint16_t rgb[3];
-                       shade(cx, G, mi, t, tz, c1, c2, rgb);
-                       cx.R[p] = rgb[0];
-                       cx.G[p] = rgb[1];
-                       cx.B[p] = rgb[2];
-                     }
-                   });
-    }
-    // -- large pass, per group: tiles of this band row
-    if (b < HC) {
-      for (int g = 0; g < ngroups; g++) {
-        if (bins[g].empty()) continue;
-        const OrcGroup& G = groups[g];
-        for (int tx = 0; tx < WC; tx++) {
-          const int bx = b * WC + tx;
-          TileBox tb = tile_box(bx, W);
-          for (uint32_t gi : bins[g][bx]) {
-            const int64_t li = gi - cx.base[g];
-            TileTri s;
-            s.x1 = cx.P[3 * gi].x; s.y1 = cx.P[3 * gi].y;
-            s.x2 = cx.P[3 * gi + 1].x; s.y2 = cx.P[3 * gi + 1].y;
-            s.x3 = cx.P[3 * gi + 2].x; s.y3 = cx.P[3 * gi + 2].y;
-            s.z1 = cx.Z[3 * gi]; s.z2 = cx.Z[3 * gi + 1]; s.z3 = cx.Z[3 * gi + 2];
-            s.a1 = a_scale(raw_attr(G, 3 * li), s.z1);
-            s.a2 = a_scale(raw_attr(G, 3 * li + 1), s.z2);
-            s.a3 = a_scale(raw_attr(G, 3 * li + 2), s.z3);
-            tile_sort(s);
-            tile_fourth(s);
-            MipInfo mi;
-            if (G.flags & ORC_MIP) {
-              Attr a[3] = {s.a1, s.a2, s.a3};
-              float z[3] = {s.z1, s.z2, s.z3};
-              mi = mip_setup<float>(a, z, s.x1, s.y1, s.x2, s.y2, s.x3, s.y3, G.tex_size,
-                                    G.mip_bias);
-              s.a1 = a[0]; s.a2 = a[1]; s.a3 = a[2];
-            }
-            raster_tile(s, tb,
-                        [&](int ax, int cy, float t, float tz, const Attr& c1, const Attr& c2,
-                            int) {
-                          nf++;
-                          int64_t p = (int64_t)W * cy + ax;
-                          if (cx.F[p] >= tz) {
-                            cx.F[p] = tz;
-                            cx.id[p] = gi;
-                            uint16_t rgb[3];
-                            shade(cx, G, mi, t, tz, c1, c2, rgb);
-                            cx.R[p] = rgb[0];
-                            cx.G[p] = rgb[1];
-                            cx.B[p] = rgb[2];
-                          }
-                        });
-          }
-        }
-      }
-    }
-    frags += nf;
-  });
-  out->stats[2] = frags.load();
+    });
+  }
 
   if (fr->exposure > 0)
     orc_gamma(cx.R.data(), cx.G.data(), cx.B.data(), NP, fr->exposure);

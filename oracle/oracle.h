@@ -63,6 +63,8 @@ typedef struct {
   int spec_pow;
   int skip; /* drawAll mask: 1 = group not drawn */
   int cast; /* casts into shadow maps */
+  int ssr;  /* >= 0: an SSR water group (av_SSR.c) reflecting cube map
+               refl_*[ssr]; drawn after the opaque groups; -1 otherwise */
 } OrcGroup;
 
 typedef struct {
@@ -90,6 +92,15 @@ typedef struct {
   float exposure; /* > 0: gamma to 8 bit after drawing */
   uint16_t clear_rgb[3];
   int nthreads; /* 0 = all cores */
+  /* reflection cube strips (setReflTex): faces of side refl_size[k], as
+   * refl_size rows of 6*refl_size texels per plane */
+  const uint16_t* refl_r[4];
+  const uint16_t* refl_g[4];
+  const uint16_t* refl_b[4];
+  int refl_size[4];
+  int ssr_steps, ssr_step; /* REFL_LENGTH, REFL_STEP */
+  float ssr_absorb;        /* ABSORB */
+  int ssr_fresnel;         /* 0: fr = 1 (metal variant) */
 } OrcFrame;
 
 typedef struct {
@@ -130,6 +141,9 @@ int orc_gamma(uint16_t* r, uint16_t* g, uint16_t* b, int64_t n, float ex);
 /* CLDraw.blur: Post/blur.c blurH then Post/blur2.c blurV on u16 planes of
  * a W x H frame (serial phases; decrees in oracle.cpp). */
 int orc_blur(uint16_t* r, uint16_t* g, uint16_t* b, int W, int H);
+/* The oracle's decreed elementary functions (which 0: log2, 1: exp), as the
+ * kernels compute them, over n floats. */
+int orc_math(int which, const float* x, float* y, int64_t n);
 int orc_create_mips(const uint16_t* plane, int N, uint16_t* out);
 int orc_vertlight(const float* xyz, const float* vn, int nvert,
                   const float* dirI, const float* dirD, int ndir,

@@ -23,7 +23,8 @@ class OrcGroup(ctypes.Structure):
                 ("uv", ctypes.c_void_p), ("li", ctypes.c_void_p), ("tr", ctypes.c_void_p),
                 ("tg", ctypes.c_void_p), ("tb", ctypes.c_void_p), ("tex_size", ctypes.c_int),
                 ("flags", ctypes.c_uint32), ("mip_bias", ctypes.c_float),
-                ("spec_pow", ctypes.c_int), ("skip", ctypes.c_int), ("cast", ctypes.c_int)]
+                ("spec_pow", ctypes.c_int), ("skip", ctypes.c_int), ("cast", ctypes.c_int),
+                ("ssr", ctypes.c_int)]
 
 
 class OrcCam(ctypes.Structure):
@@ -41,7 +42,11 @@ class OrcFrame(ctypes.Structure):
                 ("amb", ctypes.c_float), ("nshadow", ctypes.c_int),
                 ("shadow", OrcShadowCfg * 2), ("pcf_taps", ctypes.c_int),
                 ("exposure", ctypes.c_float), ("clear_rgb", ctypes.c_uint16 * 3),
-                ("nthreads", ctypes.c_int)]
+                ("nthreads", ctypes.c_int),
+                ("refl_r", ctypes.c_void_p * 4), ("refl_g", ctypes.c_void_p * 4),
+                ("refl_b", ctypes.c_void_p * 4), ("refl_size", ctypes.c_int * 4),
+                ("ssr_steps", ctypes.c_int), ("ssr_step", ctypes.c_int),
+                ("ssr_absorb", ctypes.c_float), ("ssr_fresnel", ctypes.c_int)]
 
 
 class OrcOut(ctypes.Structure):
@@ -76,9 +81,10 @@ def lib() -> ctypes.CDLL:
         L.orc_gamma.argtypes = [p, p, p, ctypes.c_int64, f]
         L.orc_create_mips.argtypes = [p, i, p]
         L.orc_blur.argtypes = [p, p, p, i, i]
+        L.orc_math.argtypes = [i, p, p, ctypes.c_int64]
         L.orc_vertlight.argtypes = [p, p, i, p, p, i, p, p, i, p, p, p, i, f, p]
         for fn in ("orc_render", "orc_trisetup", "orc_shadow_setup", "orc_coarse_tri",
-                   "orc_fragments", "orc_gamma", "orc_create_mips", "orc_vertlight", "orc_blur"):
+                   "orc_fragments", "orc_gamma", "orc_create_mips", "orc_vertlight", "orc_blur", "orc_math"):
             getattr(L, fn).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -143,7 +149,17 @@ def render(scene, nthreads=0, with_shadows_out=False):
         G.spec_pow = int(g.spec_pow)
         G.skip = 1 if g.skip else 0
         G.cast = 1 if g.cast else 0
+        G.ssr = int(getattr(g, "ssr", -1))
     fr = OrcFrame()
+    for k, strip in enumerate(getattr(scene, "refl", [])[:4]):
+        planes = [_c(strip[..., c], np.uint16) for c in range(3)]
+        keep += planes
+        fr.refl_r[k], fr.refl_g[k], fr.refl_b[k] = (x.ctypes.data for x in planes)
+        fr.refl_size[k] = strip.shape[0]
+    fr.ssr_steps = int(getattr(scene, "ssr_steps", 400))
+    fr.ssr_step = int(getattr(scene, "ssr_step", 1))
+    fr.ssr_absorb = float(np.float32(getattr(scene, "ssr_absorb", 0.3)))
+    fr.ssr_fresnel = 1 if getattr(scene, "ssr_fresnel", True) else 0
     fr.W, fr.H = scene.W, scene.H
     fr.view = make_cam(scene.view)
     fr.lint[:] = [float(x) for x in np.asarray(scene.light_int, np.float32)]
@@ -231,6 +247,22 @@ def blur(r, g, b):
     if lib().orc_blur(r.ctypes.data, g.ctypes.data, b.ctypes.data, W, H) < 0:
         raise ValueError("blur: W >= 4, H >= 8, both even")
     return r, g, b
+
+
+def log2_cw(x):
+    """The decreed log2 (mip level selection) over float32 values."""
+    x = _c(x, np.float32).reshape(-1)
+    y = np.empty_like(x)
+    lib().orc_math(0, x.ctypes.data, y.ctypes.data, x.size)
+    return y
+
+
+def exp_cw(x):
+    """The decreed exp (SSR transmittance) over float32 values."""
+    x = _c(x, np.float32).reshape(-1)
+    y = np.empty_like(x)
+    lib().orc_math(1, x.ctypes.data, y.ctypes.data, x.size)
+    return y
 
 
 def _bilerp_up(src, h, w):

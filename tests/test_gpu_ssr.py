@@ -1,0 +1,122 @@
+"""SSR water (Shaders_src/av_SSR.c via drawAll's "SSR" shader entries,
+Ops.py:788, 859-869, 910-924) and reflection cube maps (setReflTex,
+Ops.py:458-467) on the GPU against the oracle: depth bit-exact, colour
+bit-exact in HDR (the tolerance contract is reported beside it)."""
+import numpy as np
+import pytest
+
+from axi_visualizer_amd import scenes
+from axi_visualizer_amd._native import AxrError
+from axi_visualizer_amd.render import Renderer
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_frame(sc, gamma=False):
+    r = Renderer(sc)
+    try:
+        r.frame(gamma=gamma)
+        return r.draw.getFrame()
+    finally:
+        r.close()
+
+
+def check(sc, label):
+    sc.exposure = 0.0
+    rr, gg, bb, d = gpu_frame(sc)
+    ref = pyoracle.render(sc)
+    dg, dr = d.view(np.uint32), ref["depth"].view(np.uint32)
+    bad = np.argwhere(dg != dr)
+    assert bad.size == 0, f"{label}: depth differs at {len(bad)} px, first {bad[:3].tolist()}"
+    got = np.stack([rr, gg, bb], -1).astype(np.int64)
+    want = np.stack([ref["r"], ref["g"], ref["b"]], -1).astype(np.int64)
+    diff = np.abs(got - want).max(-1)
+    water = (ref["depth"] != pyoracle.render(_no_water(sc))["depth"])
+    print(f"[ssr] {label}: water px {int(water.sum())}, exact {float((diff == 0).mean()):.6f}, "
+          f"worst {int(diff.max())}", flush=True)
+    bad = np.argwhere(diff != 0)
+    assert bad.size == 0, f"{label}: colour differs at {len(bad)} px, first {bad[:3].tolist()}"
+    assert water.mean() > 0.1
+    return water
+
+
+def _no_water(sc):
+    import copy
+    s2 = copy.copy(sc)
+    s2.groups = [copy.copy(g) for g in sc.groups]
+    for g in s2.groups:
+        if g.ssr >= 0:
+            g.skip = True
+    return s2
+
+
+def test_water_matches_oracle():
+    check(scenes.water_scene(), "water")
+
+
+def test_water_with_shadows_and_64_steps():
+    check(scenes.water_scene(W=256, H=192, shadow_dim=256, ssr_steps=64), "water_shadow_64")
+
+
+def test_metal_variant_step_two():
+    sc = scenes.water_scene(W=192, H=160, n_inst=10)
+    sc.ssr_step, sc.ssr_steps, sc.ssr_absorb, sc.ssr_fresnel = 2, 800, 0.4, False
+    check(sc, "metal")
+
+
+def test_masked_water_is_not_drawn_and_refl_round_trip():
+    sc = scenes.water_scene(W=160, H=128)
+    sc.exposure = 0.0
+    r = Renderer(sc)
+    try:
+        got = r.draw.getReflTex(0)
+        for c in range(3):
+            assert np.array_equal(got[c], sc.refl[0][..., c])
+        r.mask = [False] * (len(sc.groups) - 1) + [True]
+        r.frame(gamma=False)
+        frame = r.draw.getFrame()
+        ids = r.draw.getIds()  # no SSR draw ran: the opaque ids stay available
+    finally:
+        r.close()
+    ref = pyoracle.render(_no_water(sc))
+    assert np.array_equal(frame[3].view(np.uint32), ref["depth"].view(np.uint32))
+    assert np.array_equal(ids, ref["id"])
+
+
+def test_ssr_errors():
+    sc = scenes.water_scene(W=160, H=128, n_inst=0)
+    r = Renderer(sc)
+    try:
+        d = r.draw
+        d.setPos(sc.view.pos)
+        d.setVM(sc.view.vm)
+        d.clearZBuffer()
+        d.drawAll([g.flags for g in sc.groups[:-1]] + [{"SSR": "nope"}], shadowIds=[])
+    except KeyError:
+        pass
+    else:
+        raise AssertionError("unknown reflection texture accepted")
+    finally:
+        r.close()
+    r = Renderer(sc)
+    try:
+        r.frame(gamma=True)
+        with pytest.raises(AxrError):
+            r.draw.getIds()  # the SSR draw replaced the bins
+        ng = len(sc.groups)
+        grp = np.array([ng - 1], np.int32)
+        slot = np.array([0], np.int32)
+        from axi_visualizer_amd._native import ptr
+        rc = r.draw._L.axr_draw_ssr(r.draw._ctx, ptr(grp), ptr(slot), 1, 400, 1, 0.3, 1)
+        assert rc < 0  # after gamma
+        slot[0] = 5
+        r.draw.clearZBuffer()
+        rc = r.draw._L.axr_draw_ssr(r.draw._ctx, ptr(grp), ptr(slot), 1, 400, 1, 0.3, 1)
+        assert rc < 0  # no such cube map
+        grp[0] = 0  # the terrain's texture is a mip chain
+        slot[0] = 0
+        rc = r.draw._L.axr_draw_ssr(r.draw._ctx, ptr(grp), ptr(slot), 1, 400, 1, 0.3, 1)
+        assert rc < 0
+    finally:
+        r.close()

@@ -144,3 +144,34 @@ def test_bloom_light_is_local():
     assert abs(y - 64.5) <= 3 and abs(x - 64.5) <= 3
     assert add[65, 67] > add[65, 70] > add[65, 73] > 0 == add[65, 100]
     assert not out[1].any() and not out[2].any()
+
+
+def _ulps(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float64).astype(np.float32)
+    return np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+
+
+def test_log2_decree_exact_at_powers_of_two_and_close_elsewhere():
+    k = np.arange(-140, 128)
+    x = np.ldexp(np.float32(1), k).astype(np.float32)
+    assert np.array_equal(pyoracle.log2_cw(x), k.astype(np.float32))
+    rng = np.random.default_rng(5)
+    y = (rng.random(200000) * 60 - 30).astype(np.float32)
+    x = np.exp2(y.astype(np.float64)).astype(np.float32)
+    got = pyoracle.log2_cw(x)
+    want = np.log2(x.astype(np.float64))
+    far = np.abs(want) > 0.25  # relative ulps are meaningful away from x = 1
+    assert _ulps(got[far], want[far]).max() <= 4
+    assert np.abs(got[~far] - want[~far]).max() < 1e-7
+    s = pyoracle.log2_cw(np.array([0.0, -1.0, np.inf, np.nan], np.float32))
+    assert s[0] == -np.inf and np.isnan(s[1]) and s[2] == np.inf and np.isnan(s[3])
+
+
+def test_exp_decree_close_to_exp():
+    rng = np.random.default_rng(6)
+    x = (rng.random(200000) * 160 - 79).astype(np.float32)
+    got = pyoracle.exp_cw(x)
+    assert _ulps(got, np.exp(x.astype(np.float64))).max() <= 4
+    s = pyoracle.exp_cw(np.array([0.0, -80.0, -1e4, 89.0, np.nan], np.float32))
+    assert s[0] == 1.0 and s[1] == 0.0 and s[2] == 0.0 and s[3] == np.inf and s[4] == 0.0
