@@ -15,6 +15,9 @@ _f = ctypes.c_float
 _i = ctypes.c_int
 _i64 = ctypes.c_int64
 _p = ctypes.c_void_p
+# axr_batch_render outputs (include/axr.h AXR_OUT_*)
+OUT_RGBA8, OUT_IDS, OUT_HDR = 0, 1, 2
+
 _ctx = ctypes.c_void_p
 
 _u32 = ctypes.c_uint32
@@ -77,6 +80,8 @@ SIGNATURES = {
     "axr_batch_get_rgba8": (_i, [_batch, _i, _p]),
     "axr_batch_get_depth": (_i, [_batch, _i, _p]),
     "axr_batch_get_ids": (_i, [_batch, _i, _p]),
+    "axr_batch_get_hdr": (_i, [_batch, _i, _p, _p, _p]),
+    "axr_render_cube": (_i, [_batch, _ctx, _i, _p, _i, _i, _i]),
     "axr_batch_get_bytes": (_i, [_batch, ctypes.POINTER(ctypes.c_uint64)]),
     "axr_get_timing": (_i, [_ctx, _p, _i, _p, _p, _i]),
 }

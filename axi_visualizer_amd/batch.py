@@ -18,7 +18,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from ._native import AxrError, AxrJob, check, lib, ptr
+from ._native import OUT_HDR, OUT_IDS, OUT_RGBA8, AxrError, AxrJob, check, lib, ptr
 from .ops import CLDraw
 from .render import load_scene
 from .scenes import Camera, Scene, make_camera
@@ -101,13 +101,14 @@ class Batch:
         self.bin_size = bin_size
         self.cap_small, self.cap_large, self.graph = cap_small, cap_large, graph
 
-    def render(self, jobs: Sequence[Job], ids: bool = False):
-        """Enqueues the jobs and returns at once."""
+    def render(self, jobs: Sequence[Job], ids: bool = False, hdr: bool = False):
+        """Enqueues the jobs and returns at once: RGBA8 frames, or winner ids
+        (ids=True), or HDR u16 planes (hdr=True), each with depth."""
         self.jobs = list(jobs)
         self.ids = ids
         arr = (AxrJob * max(1, len(self.jobs)))(*[j.to_c() for j in self.jobs])
-        check(self._L.axr_batch_render(self._b, arr, len(self.jobs), 1 if ids else 0),
-              "axr_batch_render")
+        out = OUT_IDS if ids else OUT_HDR if hdr else OUT_RGBA8
+        check(self._L.axr_batch_render(self._b, arr, len(self.jobs), out), "axr_batch_render")
 
     def done(self) -> bool:
         return check(self._L.axr_batch_query(self._b), "axr_batch_query") == 1
@@ -133,6 +134,31 @@ class Batch:
 
     def ids_of(self, j: int) -> np.ndarray:
         return self._get(self._L.axr_batch_get_ids, j, np.uint32)
+
+    def hdr(self, j: int):
+        """Job j's HDR planes (r, g, b) u16 [H, W] of a render with hdr=True."""
+        job = self.jobs[j]
+        out = [np.empty((job.height, job.width), np.uint16) for _ in range(3)]
+        check(self._L.axr_batch_get_hdr(self._b, int(j), *[ptr(o) for o in out]),
+              "batch readback")
+        return tuple(out)
+
+    def render_cube(self, scene, name, pos, size: int, shadows: bool = True, pcf_taps: int = 5):
+        """The cube map pass: `scene` (a CLDraw) seen from pos as six size^2
+        HDR faces, stored as its reflection texture `name` in the strip
+        layout setReflTex takes.  Enqueued; the scene's later draws wait for
+        it."""
+        d = scene.draw if hasattr(scene, "draw") else scene
+        slot = d._refl.get(name, -1)
+        p = (ctypes.c_float * 3)(*[float(v) for v in np.asarray(pos, np.float32).reshape(3)])
+        slot = check(self._L.axr_render_cube(self._b, d._ctx, slot, p, int(size),
+                                             1 if shadows else 0, int(pcf_taps)),
+                     "axr_render_cube")
+        d._refl[name] = slot
+        d.reflTexSize[name] = np.int32(int(size) // 2)
+        self.jobs = [Job(scene=None, camera=None, width=size, height=size) for _ in range(6)]
+        self.ids = False
+        return slot
 
     def device_bytes(self) -> int:
         n = ctypes.c_uint64()

@@ -166,6 +166,7 @@ struct AxrJobDev {
   float* depth;    // [W*H] outputs
   uint32_t* rgba;
   uint32_t* ids;
+  uint16_t *r, *g, *b;  // HDR planes (batches rendered for HDR)
   float exposure;
   uint32_t slot0;  // first screen-bin slot of this job
   uint32_t blk0;   // first setup block of this job

@@ -65,9 +65,10 @@ hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipSt
 // Batched jobs (batch.hip).  bin_shift 3/4/5 = 8/16/32-px bins.
 hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, int njobs, uint32_t blocks,
                               const AxrBins& B, hipStream_t s);
-hipError_t launch_batch_raster(int bin_shift, int ids, const AxrJobDev* jobs, int njobs,
+// mode: a ShadeMode (RGBA8 with gamma, winner ids, or HDR planes).
+hipError_t launch_batch_raster(int bin_shift, int mode, const AxrJobDev* jobs, int njobs,
                                const AxrBins& B, int grid, hipStream_t s);
-int batch_raster_blocks_per_cu(int bin_shift, int ids);
+int batch_raster_blocks_per_cu(int bin_shift, int mode);
 
 struct AxrLights {
   int ndir, npoint, nspot;

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(1 << (2 * SHIFT) > 512 ? 512 : 1 << (2 * SHIFT)) vo
     const int b = (int)(slot - J.slot0);
     raster_bin<SHIFT, MODE>(lk, list, &ncov, J.rec, B, slot, (uint32_t)B.nb + slot, 0,
                             b % J.nbx, b / J.nbx, J.vpos, J.vattr, J.groups, J.dc, J.exposure,
-                            J.depth, nullptr, nullptr, nullptr, J.rgba, J.ids);
+                            J.depth, J.r, J.g, J.b, J.rgba, J.ids);
     __syncthreads();  // the block's LDS is reused by the next bin
   }
 }
@@ -154,23 +154,28 @@ hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, int njobs, u
   return hipGetLastError();
 }
 
-hipError_t launch_batch_raster(int bin_shift, int ids, const AxrJobDev* jobs, int njobs,
+hipError_t launch_batch_raster(int bin_shift, int mode, const AxrJobDev* jobs, int njobs,
                                const AxrBins& B, int grid, hipStream_t s) {
   if (njobs == 0 || B.nb == 0) return hipSuccess;
-  if (bin_shift == 3)
-    return ids ? raster_launch<3, SHADE_IDS>(jobs, njobs, B, grid, s)
-               : raster_launch<3, SHADE_LDR8>(jobs, njobs, B, grid, s);
-  if (bin_shift == 4)
-    return ids ? raster_launch<4, SHADE_IDS>(jobs, njobs, B, grid, s)
-               : raster_launch<4, SHADE_LDR8>(jobs, njobs, B, grid, s);
-  return ids ? raster_launch<5, SHADE_IDS>(jobs, njobs, B, grid, s)
-             : raster_launch<5, SHADE_LDR8>(jobs, njobs, B, grid, s);
+#define AXR_RL(SH)                                                       \
+  return mode == SHADE_IDS   ? raster_launch<SH, SHADE_IDS>(jobs, njobs, B, grid, s) \
+         : mode == SHADE_HDR ? raster_launch<SH, SHADE_HDR>(jobs, njobs, B, grid, s) \
+                             : raster_launch<SH, SHADE_LDR8>(jobs, njobs, B, grid, s);
+  if (bin_shift == 3) AXR_RL(3)
+  if (bin_shift == 4) AXR_RL(4)
+  AXR_RL(5)
+#undef AXR_RL
 }
 
-int batch_raster_blocks_per_cu(int bin_shift, int ids) {
-  if (bin_shift == 3) return ids ? raster_occupancy<3, SHADE_IDS>() : raster_occupancy<3, SHADE_LDR8>();
-  if (bin_shift == 4) return ids ? raster_occupancy<4, SHADE_IDS>() : raster_occupancy<4, SHADE_LDR8>();
-  return ids ? raster_occupancy<5, SHADE_IDS>() : raster_occupancy<5, SHADE_LDR8>();
+int batch_raster_blocks_per_cu(int bin_shift, int mode) {
+#define AXR_OCC(SH)                                                 \
+  return mode == SHADE_IDS   ? raster_occupancy<SH, SHADE_IDS>()    \
+         : mode == SHADE_HDR ? raster_occupancy<SH, SHADE_HDR>()    \
+                             : raster_occupancy<SH, SHADE_LDR8>();
+  if (bin_shift == 3) AXR_OCC(3)
+  if (bin_shift == 4) AXR_OCC(4)
+  AXR_OCC(5)
+#undef AXR_OCC
 }
 
 }  // namespace axr

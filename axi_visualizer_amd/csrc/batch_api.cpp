@@ -39,7 +39,7 @@ struct axr_batch {
   hipStream_t stream = nullptr;
   int cu_count = 0;
   int bin_shift = 5;
-  int occupancy[3][2] = {};  // raster blocks per CU by (bin shift - 3, ids)
+  int occupancy[3][3] = {};  // raster blocks per CU by (bin shift - 3, shade mode)
   BinStore bins;
   uint32_t* d_head = nullptr;
   uint32_t* d_counters = nullptr;
@@ -50,6 +50,8 @@ struct axr_batch {
   uint32_t* d_rgba = nullptr;
   uint32_t* d_ids = nullptr;
   size_t px_cap = 0;
+  uint16_t* d_hdr = nullptr;  // HDR planes [3][hdr_cap], for HDR renders
+  size_t hdr_cap = 0;
   AxrJobDev* d_jobs = nullptr;
   AxrJobDev* h_jobs = nullptr;  // pinned staging of the job table
   int jobs_cap = 0;
@@ -57,7 +59,7 @@ struct axr_batch {
   int njobs = 0;
   uint32_t setup_blocks = 0;
   int nb_total = 0;
-  int ids = 0;
+  int mode = SHADE_LDR8;  // ShadeMode of the last render
   hipEvent_t ev_start = nullptr, ev_done = nullptr;
   hipEvent_t ev_copied = nullptr;  // the job table has left the staging buffer
   bool rendered = false;
@@ -75,7 +77,7 @@ void free_graph(axr_batch* b) {
   b->graph_key.clear();
 }
 
-int grow_pools(axr_batch* b, size_t tris, size_t px, int njobs) {
+int grow_pools(axr_batch* b, size_t tris, size_t px, int njobs, bool hdr) {
   int rc;
   if (tris > b->rec_cap) {
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -94,6 +96,13 @@ int grow_pools(axr_batch* b, size_t tris, size_t px, int njobs) {
         (rc = dalloc(&b->d_ids, px)))
       return rc;
     b->px_cap = px;
+    free_graph(b);
+  }
+  if (hdr && px > b->hdr_cap) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    dfree(b->d_hdr);
+    if ((rc = dalloc(&b->d_hdr, 3 * px))) return rc;
+    b->hdr_cap = px;
     free_graph(b);
   }
   if (njobs > b->jobs_cap) {
@@ -155,6 +164,11 @@ AxrJobDev make_job(const axr_batch* b, const axr_job& j, const JobInfo& in, uint
   d.depth = b->d_depth + in.out_off;
   d.rgba = b->d_rgba + in.out_off;
   d.ids = b->d_ids + in.out_off;
+  if (b->d_hdr && b->mode == SHADE_HDR) {
+    d.r = b->d_hdr + in.out_off;
+    d.g = b->d_hdr + b->hdr_cap + in.out_off;
+    d.b = b->d_hdr + 2 * b->hdr_cap + in.out_off;
+  }
   d.exposure = j.exposure;
   d.slot0 = slot0;
   d.blk0 = blk0;
@@ -165,13 +179,13 @@ AxrJobDev make_job(const axr_batch* b, const axr_job& j, const JobInfo& in, uint
 int enqueue(axr_batch* b) {
   AxrBins B = b->bins.B;
   B.head = b->d_head;
-  int& occ = b->occupancy[b->bin_shift - 3][b->ids];
-  if (!occ) occ = batch_raster_blocks_per_cu(b->bin_shift, b->ids);
+  int& occ = b->occupancy[b->bin_shift - 3][b->mode];
+  if (!occ) occ = batch_raster_blocks_per_cu(b->bin_shift, b->mode);
   const int grid_max = b->cu_count * occ;
   const int grid = b->nb_total < grid_max ? b->nb_total : grid_max;
   HIPCHK(launch_batch_setup(b->bin_shift, b->d_jobs, b->njobs, b->setup_blocks, B, b->stream));
   HIPCHK(launch_bin_order(&B, nullptr, b->d_counters, b->stream));
-  HIPCHK(launch_batch_raster(b->bin_shift, b->ids, b->d_jobs, b->njobs, B, grid, b->stream));
+  HIPCHK(launch_batch_raster(b->bin_shift, b->mode, b->d_jobs, b->njobs, B, grid, b->stream));
   return 0;
 }
 
@@ -222,6 +236,7 @@ int axr_batch_destroy(axr_batch* b) {
   dfree(b->d_depth);
   dfree(b->d_rgba);
   dfree(b->d_ids);
+  dfree(b->d_hdr);
   dfree(b->d_jobs);
   if (b->h_jobs) (void)hipHostFree(b->h_jobs);
   if (b->ev_start) (void)hipEventDestroy(b->ev_start);
@@ -254,9 +269,11 @@ int axr_batch_configure(axr_batch* b, int bin_size, uint32_t cap_small, uint32_t
   return AXR_OK;
 }
 
-int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int want_ids) {
+int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
   if (!b || (!jobs && njobs)) return fail(AXR_EINVAL, "batch render: bad arguments");
   if (njobs < 0) return fail(AXR_EINVAL, "negative job count");
+  if (output != AXR_OUT_RGBA8 && output != AXR_OUT_IDS && output != AXR_OUT_HDR)
+    return fail(AXR_EINVAL, "batch render: output %d", output);
   HIPCHK(hipSetDevice(b->device));
   for (int i = 0; i < njobs; i++) {
     const axr_job& j = jobs[i];
@@ -312,8 +329,10 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int want_ids)
   b->njobs = njobs;
   b->setup_blocks = blocks;
   b->nb_total = (int)slots;
-  b->ids = want_ids ? 1 : 0;
-  if ((rc = grow_pools(b, tris ? tris : 1, px ? px : 1, njobs ? njobs : 1))) return rc;
+  b->mode = output == AXR_OUT_IDS ? SHADE_IDS : output == AXR_OUT_HDR ? SHADE_HDR : SHADE_LDR8;
+  if ((rc = grow_pools(b, tris ? tris : 1, px ? px : 1, njobs ? njobs : 1,
+                       b->mode == SHADE_HDR)))
+    return rc;
   const AxrBins before = b->bins.B;
   if ((rc = bins_prepare(b->bins, slots ? (int)slots : 1, b->stream))) return rc;
   b->bins.B.nb = (int)slots;
@@ -334,7 +353,7 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int want_ids)
     if (b->use_graph) {
       // The graph holds the three launches; its arguments depend on this key.
       std::vector<uint64_t> key = {(uint64_t)njobs, blocks, slots, (uint64_t)b->bin_shift,
-                                   (uint64_t)b->ids, b->bins.B.cap0, b->bins.B.cap1,
+                                   (uint64_t)b->mode, b->bins.B.cap0, b->bins.B.cap1,
                                    (uint64_t)(uintptr_t)b->bins.B.ent,
                                    (uint64_t)(uintptr_t)b->d_jobs};
       if (!b->gexec || key != b->graph_key) {
@@ -404,7 +423,7 @@ static int batch_out(axr_batch* b, int job, const void* src, size_t elem, void* 
 }
 
 int axr_batch_get_rgba8(axr_batch* b, int job, uint32_t* out) {
-  if (b && b->ids) return fail(AXR_ESTATE, "the batch was rendered for ids");
+  if (b && b->mode != SHADE_LDR8) return fail(AXR_ESTATE, "the batch was not rendered for RGBA8");
   return batch_out(b, job, b ? b->d_rgba : nullptr, 4, out);
 }
 
@@ -413,15 +432,83 @@ int axr_batch_get_depth(axr_batch* b, int job, float* out) {
 }
 
 int axr_batch_get_ids(axr_batch* b, int job, uint32_t* out) {
-  if (b && !b->ids) return fail(AXR_ESTATE, "the batch was rendered for colour");
+  if (b && b->mode != SHADE_IDS) return fail(AXR_ESTATE, "the batch was not rendered for ids");
   return batch_out(b, job, b ? b->d_ids : nullptr, 4, out);
+}
+
+int axr_batch_get_hdr(axr_batch* b, int job, uint16_t* r, uint16_t* g, uint16_t* bl) {
+  if (b && b->mode != SHADE_HDR) return fail(AXR_ESTATE, "the batch was not rendered for HDR");
+  if (!b || !r || !g || !bl) return fail(AXR_EINVAL, "batch readback: bad arguments");
+  int rc;
+  if ((rc = batch_out(b, job, b->d_hdr, 2, r)) || (rc = batch_out(b, job, b->d_hdr + b->hdr_cap, 2, g)) ||
+      (rc = batch_out(b, job, b->d_hdr + 2 * b->hdr_cap, 2, bl)))
+    return rc;
+  return AXR_OK;
+}
+
+int axr_render_cube(axr_batch* b, axr_ctx* scene, int slot, const float pos[3], int size,
+                    int shadows, int pcf_taps) {
+  if (!b || !scene || !pos) return fail(AXR_EINVAL, "renderCube: bad arguments");
+  if (size < 16 || size > 8192 || (size % 16) != 0)
+    return fail(AXR_EINVAL, "renderCube: face size %d must be a multiple of 16", size);
+  int rc;
+  if ((rc = check_ctx(scene))) return rc;
+  if (slot >= (int)scene->refl.size()) return fail(AXR_EINVAL, "renderCube: no slot %d", slot);
+  // Face f looks along s e_a (s = +1 for f < 3, -1 after, a = f mod 3) with
+  // horizontal row -s e_u and vertical row s e_v, (u, v) = (z, y), (x, z),
+  // (y, x) for a = x, y, z, at a 90 degree field of view: then texel (x, y)
+  // of face f holds the direction the SSR shader's cube lookup maps there.
+  axr_job jobs[6];
+  for (int f = 0; f < 6; f++) {
+    const float sgn = f < 3 ? 1.0f : -1.0f;
+    const int a = f % 3;
+    const int u = a == 0 ? 2 : a == 1 ? 0 : 1;
+    const int v = a == 0 ? 1 : a == 1 ? 2 : 0;
+    axr_job& j = jobs[f];
+    std::memset(&j, 0, sizeof j);
+    j.scene = scene;
+    for (int k = 0; k < 3; k++) j.pos[k] = pos[k];
+    for (int k = 0; k < 9; k++) j.vm[k] = 0.0f;
+    j.vm[a] = sgn;
+    j.vm[3 + u] = -sgn;
+    j.vm[6 + v] = sgn;
+    j.scale = (float)(size / 2);
+    j.cax = j.cay = 3.0f;  // (W/2)/scale + 2
+    j.width = j.height = size;
+    j.shadows = shadows;
+    j.pcf_taps = pcf_taps;
+    j.exposure = 1.0f;
+  }
+  if ((rc = axr_batch_render(b, jobs, 6, AXR_OUT_HDR))) return rc;
+  if (slot < 0) {
+    if (scene->refl.size() >= 64) return fail(AXR_EINVAL, "renderCube: at most 64 cube maps");
+    scene->refl.push_back(ReflTex{});
+    slot = (int)scene->refl.size() - 1;
+  }
+  ReflTex& t = scene->refl[(size_t)slot];
+  if (t.size != size) {
+    HIPCHK(hipStreamSynchronize(scene->stream));  // an SSR draw may read the old strip
+    dfree(t.d);
+    if ((rc = dalloc(&t.d, (size_t)6 * size * size))) return rc;
+    t.size = size;
+  }
+  HIPCHK(launch_pack_cube(b->d_hdr, b->d_hdr + b->hdr_cap, b->d_hdr + 2 * b->hdr_cap, size, t.d,
+                          b->stream));
+  // the scene's later draws (an SSR pass) read the strip after this
+  hipEvent_t ev;
+  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(ev, b->stream));
+  HIPCHK(hipStreamWaitEvent(scene->stream, ev, 0));
+  HIPCHK(hipEventDestroy(ev));
+  HIPCHK(hipEventRecord(b->ev_done, b->stream));
+  return slot;
 }
 
 int axr_batch_get_bytes(axr_batch* b, uint64_t* out) {
   if (!b || !out) return fail(AXR_EINVAL, "batch bytes: bad arguments");
   uint64_t n = 0;
   n += (uint64_t)b->rec_cap * (sizeof(AxrTriRec) + 1);
-  n += (uint64_t)b->px_cap * 12;
+  n += (uint64_t)b->px_cap * 12 + (uint64_t)b->hdr_cap * 6;
   n += (uint64_t)b->bins.ent_alloc * 4 + (uint64_t)b->bins.nb_alloc * 4 * 5 +
        (uint64_t)b->bins.B.spill_cap * 8;
   n += (uint64_t)b->jobs_cap * sizeof(AxrJobDev);

@@ -455,6 +455,23 @@ def cube_strip(seed: int, side: int) -> np.ndarray:
     return np.clip(base * (0.85 + 0.3 * n), 0, 65535).astype(np.uint16)
 
 
+def cube_face_camera(f: int, pos, size: int) -> Camera:
+    """Camera of cube face f (0..5: +x, +y, +z, -x, -y, -z) at pos, 90 degree
+    field of view on size x size: forward s e_a, horizontal row -s e_u,
+    vertical row s e_v with (u, v) = (z, y), (x, z), (y, x) for a = x, y, z,
+    so that texel (x, y) of the face holds the direction the SSR shader's
+    cube lookup (Shaders_src/av_SSR.c:107-128) maps to it."""
+    sgn = 1.0 if f < 3 else -1.0
+    a = f % 3
+    u, v = {0: (2, 1), 1: (0, 2), 2: (1, 0)}[a]
+    vm = np.zeros((3, 3), np.float32)
+    vm[0, a] = sgn
+    vm[1, u] = -sgn
+    vm[2, v] = sgn
+    return Camera(pos=np.asarray(pos, np.float32).reshape(3), vm=vm,
+                  scale=float(np.float32(size // 2)), cax=3.0, cay=3.0)
+
+
 def water_scene(W=320, H=240, grid=48, extent=60.0, seed=1, n_inst=24, refl_side=32,
                 shadow_dim=0, water_level=-1.0, ssr_steps=400, name="water") -> Scene:
     """Terrain with a few spheres and an SSR water plane at water_level over

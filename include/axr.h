@@ -212,9 +212,13 @@ int axr_batch_destroy(axr_batch* b);
  * batch layout is unchanged. */
 int axr_batch_configure(axr_batch* b, int bin_size, uint32_t cap_small, uint32_t cap_large,
                         int use_graph);
-/* Enqueues the jobs; returns without waiting.  want_ids: winner ids +
- * depth instead of RGBA8 + depth. */
-int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int want_ids);
+/* Enqueues the jobs; returns without waiting.  output: AXR_OUT_RGBA8
+ * (gamma'd colour + depth), AXR_OUT_IDS (winner ids + depth) or
+ * AXR_OUT_HDR (u16 colour planes + depth). */
+#define AXR_OUT_RGBA8 0
+#define AXR_OUT_IDS 1
+#define AXR_OUT_HDR 2
+int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output);
 /* 1 when the last render has finished, 0 while it runs. */
 int axr_batch_query(axr_batch* b);
 /* Waits for the last render; GPU time of it in *gpu_ms (may be NULL).
@@ -223,6 +227,18 @@ int axr_batch_wait(axr_batch* b, float* gpu_ms);
 int axr_batch_get_rgba8(axr_batch* b, int job, uint32_t* out);
 int axr_batch_get_depth(axr_batch* b, int job, float* out);
 int axr_batch_get_ids(axr_batch* b, int job, uint32_t* out);
+/* HDR planes of a job of a batch rendered with AXR_OUT_HDR. */
+int axr_batch_get_hdr(axr_batch* b, int job, uint16_t* r, uint16_t* g, uint16_t* bl);
+/* The cube map pass of the full-frame pipeline: the scene seen from pos as
+ * six size x size HDR faces (one batch of six jobs, 90 degree field of
+ * view, its groups as the scene's setShaders binds them) written into
+ * reflection cube slot `slot` (< 0: a new slot) in the strip layout of
+ * setReflTex that the SSR shader samples (Shaders_src/av_SSR.c:107-128):
+ * face f spans columns [f*size, (f+1)*size).  The engine's makecubemap
+ * (3SSR.py:67-76) asks for this through a drawCubeMap it never defines.
+ * Returns the slot.  The scene's later draws are ordered after it. */
+int axr_render_cube(axr_batch* b, axr_ctx* scene, int slot, const float pos[3], int size,
+                    int shadows, int pcf_taps);
 /* Device bytes the batch holds (records, bins, outputs). */
 int axr_batch_get_bytes(axr_batch* b, uint64_t* out);
 

@@ -120,3 +120,69 @@ def test_ssr_errors():
         assert rc < 0
     finally:
         r.close()
+
+
+def oracle_cube(sc, pos, size):
+    """The cube pass restated: six oracle renders of the scene (water
+    groups not drawn) from pos, laid side by side."""
+    import copy
+    strip = np.empty((size, 6 * size, 3), np.uint16)
+    for f in range(6):
+        s2 = _no_water(sc)
+        s2.W = s2.H = size
+        s2.view = scenes.cube_face_camera(f, pos, size)
+        s2.exposure = 0.0
+        o = pyoracle.render(s2)
+        for c, k in enumerate("rgb"):
+            strip[:, f * size:(f + 1) * size, c] = o[k]
+    return strip
+
+
+def _cube_probe(sc):
+    ex = float(sc.groups[0].xyz[:, 0].max())
+    return np.array([ex * 0.5, 12.0, ex * 0.5], np.float32)
+
+
+def test_cube_pass_matches_oracle():
+    from axi_visualizer_amd.batch import Batch
+    sc = scenes.water_scene(W=160, H=128)
+    pos, size = _cube_probe(sc), 64
+    r = Renderer(sc)
+    b = Batch()
+    try:
+        d = r.draw
+        d.setShaders([g.flags for g in sc.groups], mask=[g.ssr >= 0 for g in sc.groups])
+        b.render_cube(d, "probe", pos, size, shadows=False)
+        got = d.getReflTex("probe")
+    finally:
+        b.close()
+        r.close()
+    want = oracle_cube(sc, pos, size)
+    for c in range(3):
+        bad = np.argwhere(got[c] != want[..., c])
+        assert bad.size == 0, f"cube channel {c}: {len(bad)} texels differ, first {bad[:3].tolist()}"
+    assert len(np.unique(want[..., 0])) > 50  # the faces see the scene, not only sky
+
+
+def test_ssr_reflects_the_rendered_cube():
+    from axi_visualizer_amd.batch import Batch
+    sc = scenes.water_scene(W=192, H=144, ssr_steps=64)
+    pos, size = _cube_probe(sc), 32
+    r = Renderer(sc)
+    b = Batch()
+    try:
+        d = r.draw
+        d.setShaders([g.flags for g in sc.groups], mask=[g.ssr >= 0 for g in sc.groups])
+        b.render_cube(d, 0, pos, size, shadows=False)
+        r.frame(gamma=False)
+        rr, gg, bb, dd = d.getFrame()
+    finally:
+        b.close()
+        r.close()
+    sc.refl = [oracle_cube(sc, pos, size)]
+    sc.exposure = 0.0
+    ref = pyoracle.render(sc)
+    assert np.array_equal(dd.view(np.uint32), ref["depth"].view(np.uint32))
+    for c, p in zip("rgb", (rr, gg, bb)):
+        bad = np.argwhere(p != ref[c])
+        assert bad.size == 0, f"{c}: {len(bad)} px differ, first {bad[:3].tolist()}"
