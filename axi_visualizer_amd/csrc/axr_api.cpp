@@ -163,13 +163,14 @@ int reserve_tris(axr_ctx* c, size_t extra) {
   dfree(c->d_rec);
   dfree(c->d_cls);
   dfree(c->d_chunks);
+  dfree(c->d_large);
   for (int k = 0; k < AXR_MAX_SHADOWS; k++) {
     dfree(c->d_shrec[k]);
     dfree(c->d_ok[k]);
   }
   c->last_ok = nullptr;
   if ((rc = dalloc(&c->d_rec, cap)) || (rc = dalloc(&c->d_cls, cap)) ||
-      (rc = dalloc(&c->d_chunks, cap / 64)))
+      (rc = dalloc(&c->d_chunks, cap / 64)) || (rc = dalloc(&c->d_large, cap)))
     return rc;
   for (int k = 0; k < AXR_MAX_SHADOWS; k++)
     if ((rc = dalloc(&c->d_shrec[k], cap)) || (rc = dalloc(&c->d_ok[k], cap))) return rc;
@@ -182,20 +183,21 @@ int reserve_tris(axr_ctx* c, size_t extra) {
   return 0;
 }
 
+AxrGroupDev group_dev(const HostGroup& g) {
+  AxrGroupDev d{};
+  d.base = g.base;
+  d.ntri = g.ntri;
+  d.flags = g.flags;
+  d.tex_size = g.tex_size;
+  d.mip_bias = g.mip_bias;
+  d.spec_pow = g.spec_pow;
+  d.tex = g.d_tex;
+  return d;
+}
+
 int upload_groups(axr_ctx* c) {
   std::vector<AxrGroupDev> h(c->groups.size());
-  for (size_t i = 0; i < h.size(); i++) {
-    const HostGroup& g = c->groups[i];
-    h[i].base = g.base;
-    h[i].ntri = g.ntri;
-    h[i].flags = g.flags;
-    h[i].tex_size = g.tex_size;
-    h[i].mip_bias = g.mip_bias;
-    h[i].spec_pow = g.spec_pow;
-    h[i].tex = g.d_tex;
-    h[i].refl = g.d_refl;
-    h[i].refl_dim = g.refl_dim;
-  }
+  for (size_t i = 0; i < h.size(); i++) h[i] = group_dev(c->groups[i]);
   // Kernels in flight read the table: let them finish before rewriting it.
   HIPCHK(hipStreamSynchronize(c->stream));
   if (!h.empty())
@@ -308,7 +310,8 @@ AxrDrawConst base_dc(const axr_ctx* c) {
   return dc;
 }
 
-int run_setup(axr_ctx* c, const AxrDrawConst* dc_main) {
+int run_setup(axr_ctx* c, const AxrDrawConst* dc_main, const AxrGroupDev* table, uint32_t t0,
+              uint32_t t1) {
   AxrPassSet ps{};
   int nb_total = 0;
   ps.n = (int)c->pend.size();
@@ -341,13 +344,19 @@ int run_setup(axr_ctx* c, const AxrDrawConst* dc_main) {
   }
   {
     Timed tm(c, dc_main ? "setup_bin" : "shadow_setup_bin");
-    HIPCHK(launch_setup(c->d_vpos, c->d_vattr, c->d_groups, dc, dc_main ? 1 : 0, c->d_rec,
-                        c->d_cls, c->mbins.B, c->NBX, ps, c->sbins.B, c->d_chunks, c->stream));
+    // shadow passes take every caster: the range is the whole scene then
+    if (ps.n) {
+      t0 = 0;
+      t1 = c->T;
+    }
+    HIPCHK(launch_setup(c->d_vpos, c->d_vattr, table ? table : c->d_groups, dc, dc_main ? 1 : 0,
+                        c->d_rec, c->d_cls, c->mbins.B, c->NBX, ps, c->sbins.B, c->d_chunks, t0,
+                        t1, c->d_large, c->d_large_ctl, c->stream));
   }
   {
     Timed tm(c, dc_main ? "bin_order" : "shadow_order");
     HIPCHK(launch_bin_order(dc_main ? &c->mbins.B : nullptr, ps.n ? &c->sbins.B : nullptr,
-                            c->d_counters, c->stream));
+                            c->d_counters, c->d_large_ctl, c->stream));
   }
   if (ps.n) {
     Timed tm(c, "shadow_raster");
@@ -359,7 +368,7 @@ int run_setup(axr_ctx* c, const AxrDrawConst* dc_main) {
 
 int flush_shadows(axr_ctx* c) {
   if (c->pend.empty()) return 0;
-  return run_setup(c, nullptr);
+  return run_setup(c, nullptr, nullptr, 0, c->T);
 }
 
 }  // namespace axri
@@ -394,9 +403,10 @@ int axr_create(int w, int h, axr_ctx** out) {
   if ((rc = dalloc(&c->d_depth, np)) || (rc = dalloc(&c->d_r, np)) ||
       (rc = dalloc(&c->d_g, np)) || (rc = dalloc(&c->d_b, np)) || (rc = dalloc(&c->d_rgba, np)) ||
       (rc = dalloc(&c->d_ids, np)) || (rc = dalloc(&c->d_counters, kCounters)) ||
-      (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)))
+      (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)) || (rc = dalloc(&c->d_large_ctl, 1)))
     return bail(rc);
   if (launch_fill_u32(c->d_counters, kCounters, 0, c->stream) != hipSuccess ||
+      launch_fill_u32(c->d_large_ctl, 1, 0, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(fail(AXR_EHIP, "initial fill failed"));
   c->mbins.nseg = 2;
@@ -446,9 +456,12 @@ int axr_destroy(axr_ctx* c) {
   dfree(c->d_rgba);
   dfree(c->d_ids);
   dfree(c->d_counters);
+  dfree(c->d_large_ctl);
+  dfree(c->d_large);
   dfree(c->d_light);
   if (c->d_post) (void)hipFree(c->d_post);
   if (c->d_snap) (void)hipFree(c->d_snap);
+  dfree(c->d_groups_ssr);
   for (ReflTex& t : c->refl) dfree(t.d);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -678,7 +691,7 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
   }
   // the draw's setup runs with the pending shadow passes (which also
   // write any map this draw samples before its shading runs)
-  if (c->T > 0 && (rc = run_setup(c, &dc))) return rc;
+  if (c->T > 0 && (rc = run_setup(c, &dc, nullptr, 0, c->T))) return rc;
   c->last = dc;
   c->records_valid = true;
   c->state = c->T > 0 ? ST_PENDING : ST_CLEARED;
@@ -761,6 +774,7 @@ int axr_draw_ssr(axr_ctx* c, const int* groups, const int* refl, int n, int step
   if (c->snap_bytes < snap) {
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->d_snap) (void)hipFree(c->d_snap);
+  dfree(c->d_groups_ssr);
     c->d_snap = nullptr;
     c->snap_bytes = 0;
     HIPCHK(hipMalloc(&c->d_snap, snap));
@@ -782,23 +796,41 @@ int axr_draw_ssr(axr_ctx* c, const int* groups, const int* refl, int n, int step
     HIPCHK(hipMemcpyAsync((void*)S.G0, c->d_g, np * 2, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync((void*)S.B0, c->d_b, np * 2, hipMemcpyDeviceToDevice, c->stream));
   }
-  // only the water groups take part in this draw's setup
+  // Only the water groups take part in this draw: its own group table (the
+  // drawAll's table stays as it is) and the water groups' triangle range.
+  std::vector<AxrGroupDev> tab((size_t)ng);
+  uint32_t t0 = c->T, t1 = 0;
   for (int g = 0; g < ng; g++) {
-    HostGroup& G = c->groups[(size_t)g];
+    const HostGroup& G = c->groups[(size_t)g];
     const int s = slot_of[(size_t)g];
-    G.flags = s >= 0 ? AXR_GRP_DRAW : 0u;
-    G.d_refl = s >= 0 ? c->refl[(size_t)s].d : nullptr;
-    G.refl_dim = s >= 0 ? c->refl[(size_t)s].size / 2 : 0;
+    tab[(size_t)g] = group_dev(G);
+    tab[(size_t)g].flags = s >= 0 ? AXR_GRP_DRAW : 0u;
+    if (s >= 0) {
+      tab[(size_t)g].refl = c->refl[(size_t)s].d;
+      tab[(size_t)g].refl_dim = c->refl[(size_t)s].size / 2;
+      t0 = std::min(t0, G.base);
+      t1 = std::max(t1, G.base + G.ntri);
+    }
   }
-  if ((rc = upload_groups(c))) return rc;
+  if (!c->d_groups_ssr || c->h_groups_ssr.size() != tab.size() ||
+      std::memcmp(c->h_groups_ssr.data(), tab.data(), tab.size() * sizeof(AxrGroupDev)) != 0) {
+    HIPCHK(hipStreamSynchronize(c->stream));  // an SSR raster may read the old table
+    if (c->h_groups_ssr.size() != tab.size()) {
+      dfree(c->d_groups_ssr);
+      if ((rc = dalloc(&c->d_groups_ssr, tab.size()))) return rc;
+    }
+    HIPCHK(hipMemcpy(c->d_groups_ssr, tab.data(), tab.size() * sizeof(AxrGroupDev),
+                     hipMemcpyHostToDevice));
+    c->h_groups_ssr = tab;
+  }
   AxrDrawConst dc = base_dc(c);
-  if (c->T > 0) {
-    if ((rc = run_setup(c, &dc))) return rc;
+  if (t1 > t0) {
+    if ((rc = run_setup(c, &dc, c->d_groups_ssr, t0, t1))) return rc;
     Timed tm(c, "ssr_raster");
     AxrBins B = c->mbins.B;
     B.nb = c->NB;
-    HIPCHK(launch_ssr(c->d_rec, B, c->d_vpos, c->d_vattr, c->d_groups, dc, S, c->NBX, c->d_depth,
-                      c->d_r, c->d_g, c->d_b, c->stream));
+    HIPCHK(launch_ssr(c->d_rec, B, c->d_vpos, c->d_vattr, c->d_groups_ssr, dc, S, c->NBX,
+                      c->d_depth, c->d_r, c->d_g, c->d_b, c->stream));
   }
   c->state = ST_HDR;
   c->records_valid = false;
@@ -892,7 +924,14 @@ int axr_gamma(axr_ctx* c, float ex) {
     case ST_CLEARED:
     case ST_PENDING:
       return materialize(c, SHADE_LDR8, ex);
-    case ST_HDR:
+    case ST_HDR: {
+      // gamma'd colour straight into RGBA8 (getFrame expands it back)
+      Timed tm(c, "gamma");
+      HIPCHK(launch_gamma_pack(c->d_r, c->d_g, c->d_b, c->d_rgba, (size_t)c->W * c->H, ex,
+                               c->stream));
+      c->state = ST_LDR8;
+      return AXR_OK;
+    }
     case ST_LDR16: {
       Timed tm(c, "gamma");
       HIPCHK(launch_gamma_planes(c->d_r, c->d_g, c->d_b, (size_t)c->W * c->H, ex, c->stream));

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -122,6 +123,8 @@ struct axr_ctx {
   AxrGroupDev* d_groups = nullptr;
   AxrTriRec* d_rec = nullptr;   // [tri_cap] setup records of the last drawAll
   uint8_t* d_cls = nullptr;     // [tri_cap] 0 culled, 1 small, 2 large
+  uint32_t* d_large = nullptr;  // [tri_cap] large triangles of a setup, for coarse binning
+  uint32_t* d_large_ctl = nullptr;  // count of d_large, zeroed by the order pass
   // [tri_cap] ortho setup and caster flags, one array per pass of a batch
   AxrShRec* d_shrec[AXR_MAX_SHADOWS] = {};
   uint8_t* d_ok[AXR_MAX_SHADOWS] = {};
@@ -139,6 +142,8 @@ struct axr_ctx {
   size_t post_bytes = 0;
   void* d_snap = nullptr;     // SSR: copy of the opaque frame (depth, R, G, B)
   size_t snap_bytes = 0;
+  AxrGroupDev* d_groups_ssr = nullptr;   // group table of SSR draws
+  std::vector<AxrGroupDev> h_groups_ssr;  // as last uploaded
   std::vector<ReflTex> refl;
   float* d_light = nullptr;  // vertLight light arrays
   size_t light_cap = 0;

@@ -7,17 +7,26 @@ namespace axr {
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // Setup + bin appends of the drawAll (do_main) and of the pending shadow
-// passes in ps, one launch.
+// passes in ps, one launch over global triangles [t0, t1) (t0 is rounded
+// down to a 64-triangle chunk; the other triangles' records and classes are
+// left as they are).  The draw's large triangles go to the list `large`
+// (count in large_ctl[0]) and a second launch bins their tiles; the order
+// pass zeroes the count.
 hipError_t launch_setup(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                         const AxrDrawConst& dc, int do_main, AxrTriRec* rec, uint8_t* cls,
                         const AxrBins& MB, int NBX, const AxrPassSet& ps, const AxrBins& SB,
-                        const float4* chunks, hipStream_t s);
+                        const float4* chunks, uint32_t t0, uint32_t t1, uint32_t* large,
+                        uint32_t* large_ctl, hipStream_t s);
+// HDR planes -> gamma (Shaders/Post/gamma.c) packed as RGBA8.
+hipError_t launch_gamma_pack(const uint16_t* r, const uint16_t* g, const uint16_t* b,
+                             uint32_t* rgba, size_t n, float ex, hipStream_t s);
 // Bounding sphere per 64-triangle chunk (setup skips chunks out of view).
 hipError_t launch_chunk_bounds(const float4* vpos, uint32_t T, float4* chunks, hipStream_t s);
 // Counts of the pass, heavy-first order, cursor reset for the screen bins
-// and/or the shadow bins (either may be null), one launch.
+// and/or the shadow bins (either may be null), one launch; with the screen
+// bins it also zeroes *clear (the setup's large-triangle count) if given.
 hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins,
-                            uint32_t* ctr, hipStream_t s);
+                            uint32_t* ctr, uint32_t* clear, hipStream_t s);
 // Depth walk of every pending map's bins, one launch.
 hipError_t launch_shadow_bins(const AxrBins& B, const AxrPassSet& ps, hipStream_t s);
 hipError_t launch_stats(const uint8_t* cls, const uint8_t* ok, uint32_t T, uint32_t* out,

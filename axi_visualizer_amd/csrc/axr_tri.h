@@ -525,6 +525,78 @@ __device__ __forceinline__ void coarse_walk(const float px[3], const float py[3]
   }
 }
 
+// The rows of coarse_walk split over nl threads: thread `lane` takes rows
+// lane, lane + nl, ... of each half, reaching a row's edge positions by the
+// same repeated additions as the serial walk, and calls row(cy, first, xe)
+// for each row inside the grid: tiles first, first + 1, ... while
+// (float)tile <= xe, as coarse_walk emits them.  Together the threads emit
+// coarse_walk's entries.
+template <class R>
+__device__ __forceinline__ void coarse_rows(const float px[3], const float py[3], int WC, int HC,
+                                            int lane, int nl, R&& row) {
+  float x1 = px[0] / 16.0f - 0.5f, y1 = py[0] / 16.0f - 0.5f;
+  float x2 = px[1] / 16.0f - 0.5f, y2 = py[1] / 16.0f - 0.5f;
+  float x3 = px[2] / 16.0f - 0.5f, y3 = py[2] / 16.0f - 0.5f;
+  if (y1 > y2) { swp(y1, y2); swp(x1, x2); }
+  if (y2 > y3) { swp(y2, y3); swp(x2, x3); }
+  if (y1 > y2) { swp(y1, y2); swp(x1, x2); }
+  const float x4 = (x1 + ((y2 - y1) / (y3 - y1)) * (x3 - x1));
+  const float y4 = y2;
+  const float wmax = (float)WC - 1;
+  if (y1 != y2) {
+    float slope1 = (x2 - x1) / (y2 - y1);
+    float slope2 = (x4 - x1) / (y4 - y1);
+    float cx1 = x1, cx2 = x1;
+    if (slope1 < slope2) swp(slope1, slope2);
+    const float sbias = 1.0f + floorf(y1) - y1;
+    const float yEnd = ceilf(y2);
+    int cy = f2i(floorf(y1));
+    for (int k = 0; k < lane && (float)cy <= yEnd; k++, cy++) {
+      cx1 += slope1;
+      cx2 += slope2;
+    }
+    while ((float)cy <= yEnd) {
+      float mx1 = cx1 + sbias * slope1;
+      if (slope1 < 0) mx1 -= slope1;
+      float mx2 = cx2 + sbias * slope2;
+      if (slope2 > 0) mx2 -= slope2;
+      mx1 = omin(mx1, wmax);
+      mx2 = omax(mx2, 0.0f);
+      if (cy >= 0 && cy < HC) row(cy, f2i(floorf(mx2)), ceilf(mx1));
+      for (int k = 0; k < nl && (float)cy <= yEnd; k++, cy++) {
+        cx1 += slope1;
+        cx2 += slope2;
+      }
+    }
+  }
+  if (y2 != y3) {
+    float slope1 = (x3 - x2) / (y3 - y2);
+    float slope2 = (x3 - x4) / (y3 - y4);
+    float cx1 = x3, cx2 = x3;
+    if (slope1 < slope2) swp(slope1, slope2);
+    const float sbias = y3 - floorf(y3);
+    const float yEnd = floorf(y2);
+    int cy = f2i(ceilf(y3));
+    for (int k = 0; k < lane && (float)cy >= yEnd; k++, cy--) {
+      cx1 -= slope1;
+      cx2 -= slope2;
+    }
+    while ((float)cy >= yEnd) {
+      float mx1 = cx1 - sbias * slope1;
+      if (slope1 < 0) mx1 += slope1;
+      float mx2 = cx2 - sbias * slope2;
+      if (slope2 > 0) mx2 += slope2;
+      mx2 = omin(mx2, wmax);
+      mx1 = omax(mx1, 0.0f);
+      if (cy >= 0 && cy < HC) row(cy, f2i(floorf(mx1)), ceilf(mx2));
+      for (int k = 0; k < nl && (float)cy >= yEnd; k++, cy--) {
+        cx1 -= slope1;
+        cx2 -= slope2;
+      }
+    }
+  }
+}
+
 // ------------------------------------------------ a11: shadow depth walk
 // Shaders/drawmin.c:86-176 (int coordinates, strict top loop) on a dim x dim
 // map, emitting only the texels inside [xlo, xhi) x [ylo, yhi).

@@ -96,9 +96,12 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
                                                   AxrTriRec* __restrict__ rec,
                                                   uint8_t* __restrict__ cls_out, AxrBins MB,
                                                   int NBX, AxrPassSet ps, AxrBins SB,
-                                                  const float4* __restrict__ chunks) {
-  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = gi < dc.T;
+                                                  const float4* __restrict__ chunks,
+                                                  uint32_t t0, uint32_t t1,
+                                                  uint32_t* __restrict__ large,
+                                                  uint32_t* __restrict__ large_ctl) {
+  const uint32_t gi = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = gi < t1;
   // The wave's 64 triangles are one chunk: skip the passes it cannot reach.
   const float4 sph = chunks[imin((int)(gi >> 6), (int)((dc.T - 1) >> 6))];
   const bool main_on = do_main && !chunk_outside_view(sph, dc.view);
@@ -145,11 +148,9 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
       const uint32_t pos = wave_bin_add(MB.cnt, act, slot);
       if (act) bin_store(MB, slot, pos, gi);
     }
-    if (cls == 2) {
-      coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
-        append_tile<AXR_BIN_SHIFT>(MB, (uint32_t)MB.nb, t, dc.WC, NBX, gi);
-      });
-    }
+    // large triangles are binned by k_coarse, a block per triangle
+    const uint32_t at = wave_bin_add(large_ctl, cls == 2, 0u);
+    if (cls == 2) large[at] = gi;
   }
 #pragma unroll
   for (int p = 0; p < AXR_MAX_SHADOWS; p++) {
@@ -179,6 +180,46 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
   }
 }
 
+// Coarse binning of the setup's large triangles (a8, Pipe/coarse.c:26-236):
+// a block per triangle of the list, a thread per tile row (coarse_rows),
+// the row's tile appends issued eight at a time so their atomics overlap.
+// The order pass that follows resets the list for the next setup.
+constexpr int kCoarseBlocks = 256;
+constexpr int kCoarseThreads = 256;
+__global__ __launch_bounds__(kCoarseThreads) void k_coarse(const AxrTriRec* __restrict__ rec,
+                                                           const uint32_t* __restrict__ large,
+                                                           const uint32_t* __restrict__ ctl,
+                                                           AxrBins MB,
+                                                           int WC, int HC, int NBX) {
+  const uint32_t n = ctl[0];
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint32_t gi = large[i];
+    const AxrTriRec r = rec[gi];
+    coarse_rows(r.px, r.py, WC, HC, (int)threadIdx.x, (int)blockDim.x,
+                [&](int cy, int first, float xe) {
+                  for (int ax = first; (float)ax <= xe; ax += 8) {
+                    uint32_t slot[8], pos[8];
+                    bool on[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                      on[j] = (float)(ax + j) <= xe;
+                      const int t = WC * cy + ax + j;
+                      const int tx = t % WC, ty = t / WC;
+                      slot[j] = (uint32_t)MB.nb + (uint32_t)((ty >> 1) * NBX + (tx >> 1));
+                      pos[j] = on[j] ? atomicAdd(&MB.cnt[slot[j]], 1u) : 0u;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                      if (!on[j]) continue;
+                      const int tx = ax + j;
+                      const uint32_t sub = (uint32_t)(((cy & 1) << 1) | (tx & 1));
+                      bin_store(MB, slot[j], pos[j], (gi << 2) | sub);
+                    }
+                  }
+                });
+  }
+}
+
 constexpr int kOrderClasses = 64;
 // 0 for an empty bin, else 2*floor(log2 w) + (next bit) + 2, capped at 63.
 __device__ __forceinline__ uint32_t order_class(uint32_t w) {
@@ -205,7 +246,8 @@ constexpr int kOrderPer = 8;               // bins per thread per round
 constexpr int kOrderLds = 1024 * kOrderPer * 2;  // bins whose class is kept in LDS
 
 __global__ __launch_bounds__(1024) void k_bin_order(OrderJob j0, OrderJob j1,
-                                                    uint32_t* __restrict__ ctr) {
+                                                    uint32_t* __restrict__ ctr,
+                                                    uint32_t* __restrict__ clear) {
   __shared__ uint32_t hist[kOrderClasses];
   __shared__ uint32_t total;
   __shared__ uint8_t klass[kOrderLds];
@@ -216,6 +258,7 @@ __global__ __launch_bounds__(1024) void k_bin_order(OrderJob j0, OrderJob j1,
   const int tid = threadIdx.x, lane = tid & 63;
   if (tid < kOrderClasses) hist[tid] = 0;
   if (tid == 0) total = 0;
+  if (clear && tid == 0 && blockIdx.x == 0) *clear = 0;  // the large list, read by now
   __syncthreads();
   uint32_t tot = 0;
   // Counts are loaded a round at a time so the loads overlap.
@@ -380,10 +423,17 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
 hipError_t launch_setup(const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                         const AxrDrawConst& dc, int do_main, AxrTriRec* rec, uint8_t* cls,
                         const AxrBins& MB, int NBX, const AxrPassSet& ps, const AxrBins& SB,
-                        const float4* chunks, hipStream_t s) {
-  if (dc.T == 0 || (!do_main && ps.n == 0)) return hipSuccess;
-  hipLaunchKernelGGL(k_setup, dim3((dc.T + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
-                     vattr, groups, dc, do_main, rec, cls, MB, NBX, ps, SB, chunks);
+                        const float4* chunks, uint32_t t0, uint32_t t1, uint32_t* large,
+                        uint32_t* large_ctl, hipStream_t s) {
+  t0 &= ~63u;  // whole 64-triangle chunks per wave
+  if (t1 > dc.T) t1 = dc.T;
+  if (t1 <= t0 || (!do_main && ps.n == 0)) return hipSuccess;
+  hipLaunchKernelGGL(k_setup, dim3((t1 - t0 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, vpos,
+                     vattr, groups, dc, do_main, rec, cls, MB, NBX, ps, SB, chunks, t0, t1,
+                     large, large_ctl);
+  if (do_main)
+    hipLaunchKernelGGL(k_coarse, dim3(kCoarseBlocks), dim3(kCoarseThreads), 0, s, rec, large,
+                       large_ctl, MB, dc.WC, dc.HC, NBX);
   return hipGetLastError();
 }
 
@@ -395,14 +445,15 @@ hipError_t launch_chunk_bounds(const float4* vpos, uint32_t T, float4* chunks, h
 }
 
 hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins,
-                            uint32_t* ctr, hipStream_t s) {
+                            uint32_t* ctr, uint32_t* clear, hipStream_t s) {
   OrderJob j[2];
   int n = 0;
   if (main_bins) j[n++] = OrderJob{*main_bins, 2, AXR_CTR_BIN_TOTAL, AXR_CTR_BIN_SPILL};
   if (shadow_bins) j[n++] = OrderJob{*shadow_bins, 1, AXR_CTR_SH_TOTAL, AXR_CTR_SH_SPILL};
   if (n == 0) return hipSuccess;
   if (n == 1) j[1] = j[0];
-  hipLaunchKernelGGL(k_bin_order, dim3(n), dim3(1024), 0, s, j[0], j[1], ctr);
+  hipLaunchKernelGGL(k_bin_order, dim3(n), dim3(1024), 0, s, j[0], j[1], ctr,
+                     main_bins ? clear : nullptr);
   return hipGetLastError();
 }
 

@@ -48,6 +48,17 @@ __global__ void k_gamma_planes(uint16_t* __restrict__ r, uint16_t* __restrict__ 
   }
 }
 
+__global__ void k_gamma_pack(const uint16_t* __restrict__ r, const uint16_t* __restrict__ g,
+                             const uint16_t* __restrict__ b, uint32_t* __restrict__ rgba,
+                             size_t n, float ex) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    uint16_t v[3] = {r[i], g[i], b[i]};
+    gamma_px(ex, v);
+    rgba[i] = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | 0xFF000000u;
+  }
+}
+
 __global__ void k_expand_rgba8(const uint32_t* __restrict__ rgba, uint16_t* __restrict__ r,
                                uint16_t* __restrict__ g, uint16_t* __restrict__ b, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -161,6 +172,13 @@ hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const AxrBins& B, i
 hipError_t launch_gamma_planes(uint16_t* r, uint16_t* g, uint16_t* b, size_t n, float ex,
                                hipStream_t s) {
   hipLaunchKernelGGL(k_gamma_planes, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, r, g, b, n,
+                     ex);
+  return hipGetLastError();
+}
+
+hipError_t launch_gamma_pack(const uint16_t* r, const uint16_t* g, const uint16_t* b,
+                             uint32_t* rgba, size_t n, float ex, hipStream_t s) {
+  hipLaunchKernelGGL(k_gamma_pack, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, r, g, b, rgba, n,
                      ex);
   return hipGetLastError();
 }

@@ -67,21 +67,50 @@ __device__ __forceinline__ void ssr_px(const AxrDrawConst& dc, const AxrGroupDev
   const float slopex = (rpx - rx) / dt * vx;
   const float slopey = (rpy - ry) / dt * vx;
   const float slopez = (1.f / rpz - 1.f / tz) / dt * vx;
+  // The march, kMarch iterations at a time: their positions first (the
+  // shader's exact update sequence), then their depth loads together, then
+  // the hit tests in order.  Iteration i runs while i * step < steps, the
+  // position is on screen and sz > 0; a hit at iteration i ends the march
+  // with rn = (i + 1) * step, else rn = (first iteration not run) * step.
+  constexpr int kMarch = 4;
   float sx = rx, sy = ry, sz = 1.f / tz, sd = kReflDepth;
+  float iz = sz > 0 ? 1.f / sz : 0.f;  // 1/sz of the current iteration
   int rn = 0, hit = 0, hx = 0, hy = 0;
-  for (rn = 0; hit == 0 && rn < S.steps && sx >= 0 && sx < (float)W && sy >= 0 &&
-               sy < (float)H && sz > 0;
-       rn += S.step) {
-    const float f = S.F0[(size_t)W * f2i(sy) + f2i(sx)];
-    if (f < 1.f / sz && f > 1.f / sz - sd) {
-      hx = f2i(sx);
-      hy = f2i(sy);
-      hit = 1;
+  for (int it = 0, live = 1; live && !hit; it += kMarch) {
+    int px_[kMarch];
+    float iz_[kMarch], sd_[kMarch];
+    bool on[kMarch];
+#pragma unroll
+    for (int j = 0; j < kMarch; j++) {
+      on[j] = live && (it + j) * S.step < S.steps && sx >= 0 && sx < (float)W && sy >= 0 &&
+              sy < (float)H && sz > 0;
+      if (!on[j] && live) {
+        live = 0;
+        rn = (it + j) * S.step;
+      }
+      px_[j] = on[j] ? W * f2i(sy) + f2i(sx) : 0;
+      iz_[j] = iz;
+      sd_[j] = sd;
+      if (on[j]) {
+        sx += slopex;
+        sy += slopey;
+        sz += slopez;
+        iz = 1.f / sz;
+        sd = fabsf(iz - 1.f / (sz - slopez)) + kReflDBias;
+      }
     }
-    sx += slopex;
-    sy += slopey;
-    sz += slopez;
-    sd = fabsf(1.f / sz - 1.f / (sz - slopez)) + kReflDBias;
+    float f_[kMarch];
+#pragma unroll
+    for (int j = 0; j < kMarch; j++) f_[j] = on[j] ? S.F0[px_[j]] : 0.f;
+#pragma unroll
+    for (int j = 0; j < kMarch; j++) {
+      if (!hit && on[j] && f_[j] < iz_[j] && f_[j] > iz_[j] - sd_[j]) {
+        hit = 1;
+        hx = px_[j] % W;
+        hy = px_[j] / W;
+        rn = (it + j + 1) * S.step;
+      }
+    }
   }
 
   // cube map face and texel (nearest), as the shader picks them

@@ -71,3 +71,47 @@ class Renderer:
 
     def close(self):
         self.draw.close()
+
+
+class FullFrame(Renderer):
+    """The full-frame pipeline on one scene: shadow cascades; the cube map
+    pass (the scene from `probe` as six cube_size^2 HDR faces, one batch,
+    into reflection cube 0); drawAll, whose SSR water groups draw over the
+    opaque frame reflecting that cube; bloom on the HDR frame; gamma."""
+
+    def __init__(self, scene: Scene, probe, cube_size: int = 512, bloom: dict = None):
+        from .batch import Batch
+
+        super().__init__(scene)
+        self.probe = np.asarray(probe, np.float32)
+        self.cube_size = int(cube_size)
+        self.bloom = dict(levels=5, threshold=1.0, white=9835.0, strength=1.0)
+        if bloom:
+            self.bloom.update(bloom)
+        self.cube = Batch()
+        # the cube pass sees what drawAll's opaque pass draws (no water)
+        water = [isinstance(s, dict) for s in self.shaders]
+        self.draw.setShaders([0 if w else s for s, w in zip(self.shaders, water)],
+                             mask=[m or w for m, w in zip(self.mask, water)])
+
+    def frame(self, gamma=True, after_cascade=None):
+        sc, d = self.scene, self.draw
+        for i, c in enumerate(sc.cascades):
+            d.clearShadowMap(i)
+            d.shadowMap(i, self.cast, self.shaders, c.bias)
+            if after_cascade is not None:
+                after_cascade(i)
+        self.cube.render_cube(d, 0, self.probe, self.cube_size, shadows=bool(sc.cascades),
+                              pcf_taps=sc.pcf_taps)
+        d.setPos(sc.view.pos)
+        d.setVM(sc.view.vm)
+        d.clearZBuffer()
+        d.drawAll(self.shaders, mask=self.mask, shadowIds=list(range(len(sc.cascades))),
+                  pcf=sc.pcf_taps)
+        d.bloom(**self.bloom)
+        if gamma:
+            d.gamma(sc.exposure)
+
+    def close(self):
+        self.cube.close()
+        super().close()

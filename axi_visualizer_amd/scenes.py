@@ -376,8 +376,8 @@ def _rotations(seed: int, n: int) -> np.ndarray:
     return R
 
 
-def sphere_instances(n_inst: int, seed: int = 1, freq: int = 5, box: float = 200.0,
-                     noise: float = 0.05, smin: float = 0.5, smax: float = 3.0):
+def sphere_instances(n_inst: int, seed: int = 1, freq: int = 5, box=200.0,
+                     noise: float = 0.05, smin: float = 0.5, smax: float = 3.0, centre=0.0):
     """Seeded icosphere instances: per-vertex radial noise, random scale,
     rotation and position.  Returns per-instance triangle arrays
     (n_inst, F, 3, 3) xyz / normals and (n_inst, F, 3, 2) uv."""
@@ -385,7 +385,8 @@ def sphere_instances(n_inst: int, seed: int = 1, freq: int = 5, box: float = 200
     V, F = verts.shape[0], faces.shape[0]
     rad = 1.0 + noise * (rand_uniform(seed, 2002, n_inst * V).reshape(n_inst, V) * 2.0 - 1.0)
     scale = smin + (smax - smin) * rand_uniform(seed, 2003, n_inst)
-    pos = (rand_uniform(seed, 2004, n_inst * 3).reshape(n_inst, 3) - 0.5) * box
+    pos = (rand_uniform(seed, 2004, n_inst * 3).reshape(n_inst, 3) - 0.5) * np.asarray(box) \
+        + np.asarray(centre)
     R = _rotations(seed, n_inst)
     local = verts[None] * rad[..., None] * scale[:, None, None]          # (n, V, 3)
     world = np.einsum("nvk,nlk->nvl", local, R) + pos[:, None, :]
@@ -510,6 +511,61 @@ def water_scene(W=320, H=240, grid=48, extent=60.0, seed=1, n_inst=24, refl_side
                  light_dir=view_vec(la, lb).astype(np.float32), amb=0.2, cascades=cascades,
                  pcf_taps=9 if shadow_dim else 5, refl=[cube_strip(seed, refl_side)],
                  ssr_steps=ssr_steps)
+
+
+def full_frame_scene(W=2560, H=1440, grid=388, extent=100.0, n_inst=400, n_tex=8,
+                     tex_side=1024, terrain_tex=256, shadow_dim=2048, refl_side=512,
+                     water_level=-3.0, ssr_steps=64, seed=1, name="c3_full_frame") -> Scene:
+    """Config 3 at defaults: the config-1 terrain over the same 100 x 100
+    extent upscaled to a 388 x 388 grid (299,538 triangles) plus 400
+    config-2 spheres (200,000 triangles, 8 mipmapped 1024^2 textures) above
+    it, an SSR water plane at water_level over the terrain (64 march steps),
+    a 512^2-per-face reflection cube, two 2048^2 shadow cascades with 3x3
+    PCF, 2560x1440.  The cube strip here is a seeded placeholder: the
+    pipeline renders the cube from the scene every frame."""
+    t = np.arange(grid) / (grid - 1)
+    xs, zs = np.meshgrid(t, t, indexing="ij")
+    h = value_noise(seed, xs, zs)
+    spacing = extent / (grid - 1)
+    xyz, vn, uv = terrain_mesh(h, spacing)
+    flags = SH_MIP | SH_PHONG | (SH_SHADOW if shadow_dim else 0)
+    tex, side = texture_for(noise_texture(seed, 77, terrain_tex), True)
+    groups = [Group(xyz=xyz, vn=vn, uv=uv, tex=tex, tex_side=side, mip=True, flags=flags)]
+    sx, sv, su = sphere_instances(n_inst, seed, box=(2 * extent, 30.0, 2 * extent),
+                                  centre=(extent / 2, 18.0, extent / 2))
+    for k in range(n_tex):
+        sel = np.arange(k, n_inst, n_tex)
+        stex, sside = texture_for(noise_texture(seed, 300 + k, tex_side), True)
+        groups.append(Group(xyz=sx[sel].reshape(-1, 3).astype(np.float32),
+                            vn=sv[sel].reshape(-1, 3).astype(np.float32),
+                            uv=np.ascontiguousarray(su[sel]).reshape(-1, 2).astype(np.float32),
+                            tex=stex, tex_side=sside, mip=True, flags=flags))
+    wx, wn, wu = water_plane(-extent, 2 * extent, -extent, 2 * extent, water_level, n=16)
+    wtex, wside = texture_for(noise_texture(seed, 90, 256), False)
+    groups.append(Group(xyz=wx, vn=wn, uv=wu, tex=(wtex // 4 + 500).astype(np.uint16),
+                        tex_side=wside, mip=False, flags=0, cast=False, ssr=0))
+    centre = np.array([extent / 2, float(h.mean()), extent / 2])
+    yaw, pitch = pi / 4, pi / 6
+    pos = centre - 80.0 * view_vec(yaw, pitch)
+    la, lb = 0.5, 0.9
+    cascades = []
+    if shadow_dim:
+        cascades = [make_cascade(shadow_dim, centre, extent * 0.6, la, lb),
+                    make_cascade(shadow_dim, centre, extent * 1.5, la, lb)]
+    return Scene(name=name, W=W, H=H, groups=groups,
+                 view=make_camera(W, H, pos, yaw, pitch),
+                 light_int=np.array([0.9, 0.85, 0.8], np.float32),
+                 light_dir=view_vec(la, lb).astype(np.float32), amb=0.2, cascades=cascades,
+                 pcf_taps=9 if shadow_dim else 5, refl=[cube_strip(seed, refl_side)],
+                 ssr_steps=ssr_steps)
+
+
+def full_frame_probe(sc: Scene) -> np.ndarray:
+    """Cube map probe of a full-frame scene: above the terrain centre, 6
+    units over the water."""
+    w = [g for g in sc.groups if g.ssr >= 0][0]
+    ext = float(sc.groups[0].xyz[:, 0].max())
+    return np.array([ext / 2, float(w.xyz[0, 1]) + 6.0, ext / 2], np.float32)
 
 
 def random_tris_scene(W=256, H=192, ntri=400, seed=3, size=30.0, depth=(5.0, 40.0),
