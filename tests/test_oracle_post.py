@@ -60,6 +60,20 @@ def test_blur_uniform_frame_interior_closed_form():
         assert int(p[30, 30]) == c + u
 
 
+def test_blur_coloured_frame_weights_channels_by_luminance():
+    # lum = 0.3626 R + 0.5152 G + 0.1222 B (blur.c:31) scales every channel
+    H, W = 64, 64
+    c = (4000, 2000, 8000)
+    planes_ = [np.full((H, W), v, np.uint16) for v in c]
+    out = pyoracle.blur(*planes_)
+    lum = 0.3626 * c[0] + 0.5152 * c[1] + 0.1222 * c[2]
+    s = (TAPS[0] + 2 * sum(TAPS[1:])) / 0.9
+    for k in range(3):
+        d = _trunc_safe(c[k] * lum / 65536.0)
+        u = _trunc_safe(_trunc_safe(d * s) * s)
+        assert int(out[k][32, 32]) == c[k] + u, k
+
+
 def test_blur_impulse_is_separable_with_the_tap_table():
     # One white pixel at (2ry, 2rx) feeds half-res texels (ry, rx) and
     # (ry, rx-1) with d = trunc(q^2 / 65536), q = v/4; the horizontal pass
