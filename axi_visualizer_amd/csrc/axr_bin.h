@@ -88,6 +88,136 @@ __device__ __forceinline__ int entry_tile(uint32_t v, int bx, int by, int WC) {
   return (by >> 1) * WC + (bx >> 1);
 }
 
+// ---------------------------------------------- wave-cooperative span walk
+// A row span of the small walk: the row's edge and depth positions (as the
+// serial walk holds them at that row), the fragment sequence number and the
+// pixel range [xa, xa + len) already clipped to the bin.
+struct SpanRec {
+  float cx1, cx2, cz1, cz2;
+  uint32_t seq;
+  int cy, xa;
+};
+// Triangles whose box inside the bin has more pixels than this walk
+// spread over the wave; smaller ones walk on their own lane.
+#ifndef AXR_COOP_AREA
+#define AXR_COOP_AREA 48
+#endif
+struct WaveSpans {
+  SpanRec r[64];
+  uint32_t end[64];  // inclusive prefix sums of the spans' lengths
+};
+
+// The small walk (a7) of one triangle per lane, spread over the wave: each
+// step every lane still walking produces its next row span inside the
+// rectangle [xlo, xhi) x [ylo, yhi) -- accumulating its edges row by row
+// exactly as small_walk does, skipped rows included -- and the wave then
+// takes the pixels of all the step's spans 64 at a time (prefix sum of the
+// lengths, binary search for each pixel's span).  frag(ax, cy, tz, seq)
+// sees the same fragments as small_walk; only their order differs.  All
+// lanes of the wave must call it; `valid` marks lanes with a triangle.
+template <class F>
+__device__ __forceinline__ void coop_small_walk(WaveSpans& ws, bool valid, const Geom<int>& g,
+                                                uint32_t order, int xlo, int ylo, int xhi,
+                                                int yhi, F&& frag) {
+  const int lane = (int)__lane_id();
+  int half = valid ? 0 : 2;
+  Edges e;
+  float cx1 = 0.f, cx2 = 0.f, cz1 = 0.f, cz2 = 0.f;
+  int cy = 0;
+  if (valid) {
+    e = small_edges(g, 0);
+    cx1 = cx2 = (float)g.x1;
+    cz1 = cz2 = g.z1;
+    cy = g.y1;
+  }
+  while (__ballot(half < 2)) {
+    int len = 0;
+    SpanRec sr;
+    if (half == 0) {
+      while (cy <= g.y2 && cy < yhi && cy < ylo) {  // rows above the rectangle
+        cx1 += e.s1;
+        cx2 += e.s2;
+        cz1 += e.sz1;
+        cz2 += e.sz2;
+        cy++;
+      }
+      if (cy <= g.y2 && cy < yhi) {
+        const int xa = imax(f2i(cx2), xlo);
+        if (cx1 >= (float)xa) len = imin(xhi - 1, f2i(floorf(cx1))) - xa + 1;
+        len = imax(len, 0);
+        sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 0), cy, xa};
+        cx1 += e.s1;
+        cx2 += e.s2;
+        cz1 += e.sz1;
+        cz2 += e.sz2;
+        cy++;
+      } else {
+        half = 1;
+        e = small_edges(g, 1);
+        cx1 = cx2 = (float)g.x3;
+        cz1 = cz2 = g.z3;
+        cy = g.y3;
+      }
+    } else if (half == 1) {
+      while (cy >= g.y2 && cy >= ylo && cy >= yhi) {  // rows below the rectangle
+        cx1 -= e.s1;
+        cx2 -= e.s2;
+        cz1 -= e.sz1;
+        cz2 -= e.sz2;
+        cy--;
+      }
+      if (cy >= g.y2 && cy >= ylo) {
+        const int xa = imax(f2i(cx1), xlo);
+        if (cx2 >= (float)xa) len = imin(xhi - 1, f2i(floorf(cx2))) - xa + 1;
+        len = imax(len, 0);
+        sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 1), cy, xa};
+        cx1 -= e.s1;
+        cx2 -= e.s2;
+        cz1 -= e.sz1;
+        cz2 -= e.sz2;
+        cy--;
+      } else {
+        half = 2;
+      }
+    }
+    // inclusive prefix of the lengths over the wave
+    uint32_t incl = (uint32_t)len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t v = __shfl_up(incl, (unsigned)d);
+      if (lane >= d) incl += v;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    if (total == 0) continue;
+    ws.end[lane] = incl;
+    if (len) ws.r[lane] = sr;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t base = 0; base < total; base += 64) {
+      const uint32_t i = base + (uint32_t)lane;
+      if (i < total) {
+        int lo = 0, hi = 63;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (ws.end[mid] > i)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        const uint32_t start = lo ? ws.end[lo - 1] : 0u;
+        const SpanRec r = ws.r[lo];
+        const int ax = r.xa + (int)(i - start);
+        const float t = span_t(ax, r.cx1, r.cx2);
+        frag(ax, r.cy, span_tz(t, r.cz1, r.cz2), r.seq);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // Every fragment of the bin's entries takes an LDS min of its depth key:
 // small triangles of slot s_small walk drawSmall's scanlines clipped to the
 // bin (a7), large-triangle tile entries of slot s_large walk the tiled pass
@@ -105,14 +235,33 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
   auto key = [](float tz, uint32_t seq) -> unsigned long long {
     return FIRST ? make_key_first(tz, seq) : make_key(tz, seq);
   };
-  for_bin_entries(B, s_small, B.num[s_small], tid, nthr, [&](uint32_t gi) {
-    const AxrTriRec r = rec[gi];
-    const Geom<int> g = small_geom(r.px, r.py, r.z);
-    small_walk(g, x0, y0, xe, ye, [&](int ax, int cy, float tz, int half) {
-      if (!(tz <= AXR_DEPTH_CLEAR)) return;
-      const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
-      if (lc < (unsigned)NPX) atomicMin(&lk[lc], key(tz, frag_seq(gi, half)));
-    });
+  // one span table per wave of the block: blocks of the bin kernels have
+  // min(bin pixels, 512) threads
+  constexpr int kWaves = (NPX < 512 ? NPX : 512) / 64;
+  static_assert(AXR_RASTER_THREADS <= 512, "span tables are sized for 512-thread blocks");
+  __shared__ WaveSpans spans[kWaves];
+  WaveSpans& ws = spans[(tid >> 6) < kWaves ? (tid >> 6) : kWaves - 1];
+  auto put = [&](int ax, int cy, float tz, uint32_t seq) {
+    if (!(tz <= AXR_DEPTH_CLEAR)) return;
+    const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
+    if (lc < (unsigned)NPX) atomicMin(&lk[lc], key(tz, seq));
+  };
+  // A triangle whose box inside the bin is small walks on its lane; larger
+  // ones walk spread over the wave.
+  for_bin_entries_waves(B, s_small, B.num[s_small], tid, nthr, [&](uint32_t gi, bool valid) {
+    Geom<int> g{};
+    bool big = false;
+    if (valid) {
+      const AxrTriRec r = rec[gi];
+      g = small_geom(r.px, r.py, r.z);
+      const int bw = imin(imax(g.x1, imax(g.x2, g.x3)), xe - 1) - imax(imin(g.x1, imin(g.x2, g.x3)), x0);
+      const int bh = imin(g.y3, ye - 1) - imax(g.y1, y0);
+      big = (bw + 1) * (bh + 1) > AXR_COOP_AREA;
+      if (!big)
+        small_walk(g, x0, y0, xe, ye,
+                   [&](int ax, int cy, float tz, int half) { put(ax, cy, tz, frag_seq(gi, half)); });
+    }
+    coop_small_walk(ws, valid && big, g, gi, x0, y0, xe, ye, put);
   });
   for_bin_entries(B, s_large, B.num[s_large], tid, nthr, [&](uint32_t v) {
     const uint32_t gi = v >> 2;

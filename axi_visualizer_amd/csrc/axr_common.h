@@ -207,6 +207,35 @@ __device__ __forceinline__ void for_bin_entries(const AxrBins& B, uint32_t s, ui
   }
 }
 
+// f(entry, valid) for the n entries of slot s in rounds of the block's
+// threads, every lane of every wave calling f in every round (valid marks
+// lanes holding an entry), so f may use wave-wide operations.
+template <class F>
+__device__ __forceinline__ void for_bin_entries_waves(const AxrBins& B, uint32_t s, uint32_t n,
+                                                      int tid, int nthr, F&& f) {
+  const uint32_t cap = s < (uint32_t)B.nb ? B.cap0 : B.cap1;
+  const uint32_t* __restrict__ ent = B.ent + bin_base(B, s);
+  const uint32_t nb = n < cap ? n : cap;
+  for (uint32_t e0 = 0; e0 < nb; e0 += (uint32_t)nthr) {
+    const uint32_t e = e0 + (uint32_t)tid;
+    f(e < nb ? ent[e] : 0u, e < nb);
+  }
+  if (n > cap) {
+    const uint32_t ns = B.spill_ctr[1];
+    for (uint32_t q0 = 0; q0 < ns; q0 += (uint32_t)nthr) {
+      const uint32_t q = q0 + (uint32_t)tid;
+      bool ok = false;
+      uint32_t v = 0;
+      if (q < ns) {
+        const uint2 p = B.spill[q];
+        ok = p.x == s;
+        v = p.y;
+      }
+      f(v, ok);
+    }
+  }
+}
+
 // -------------------------------------------------------------- helpers
 
 AXR_HD float omin(float x, float y) { return y < x ? y : x; }
