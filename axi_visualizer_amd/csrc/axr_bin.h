@@ -107,6 +107,50 @@ struct WaveSpans {
   uint32_t end[64];  // inclusive prefix sums of the spans' lengths
 };
 
+// One step of a cooperative walk: every lane offers one span (len may be
+// 0); the wave walks all their pixels 64 at a time -- prefix sum of the
+// lengths, binary search for each pixel's span -- calling frag(ax, cy, tz,
+// seq) with the span's own arithmetic (span_t, span_tz).  All lanes of the
+// wave must call it.
+template <class F>
+__device__ __forceinline__ void coop_spans(WaveSpans& ws, int len, const SpanRec& sr, F&& frag) {
+  const int lane = (int)__lane_id();
+  uint32_t incl = (uint32_t)len;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t v = __shfl_up(incl, (unsigned)d);
+    if (lane >= d) incl += v;
+  }
+  const uint32_t total = __shfl(incl, 63);
+  if (total == 0) return;
+  ws.end[lane] = incl;
+  if (len) ws.r[lane] = sr;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t i = base + (uint32_t)lane;
+    if (i < total) {
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ws.end[mid] > i)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      const uint32_t start = lo ? ws.end[lo - 1] : 0u;
+      const SpanRec r = ws.r[lo];
+      const int ax = r.xa + (int)(i - start);
+      const float t = span_t(ax, r.cx1, r.cx2);
+      frag(ax, r.cy, span_tz(t, r.cz1, r.cz2), r.seq);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // The small walk (a7) of one triangle per lane, spread over the wave: each
 // step every lane still walking produces its next row span inside the
 // rectangle [xlo, xhi) x [ylo, yhi) -- accumulating its edges row by row
@@ -119,7 +163,6 @@ template <class F>
 __device__ __forceinline__ void coop_small_walk(WaveSpans& ws, bool valid, const Geom<int>& g,
                                                 uint32_t order, int xlo, int ylo, int xhi,
                                                 int yhi, F&& frag) {
-  const int lane = (int)__lane_id();
   int half = valid ? 0 : 2;
   Edges e;
   float cx1 = 0.f, cx2 = 0.f, cz1 = 0.f, cz2 = 0.f;
@@ -180,41 +223,82 @@ __device__ __forceinline__ void coop_small_walk(WaveSpans& ws, bool valid, const
         half = 2;
       }
     }
-    // inclusive prefix of the lengths over the wave
-    uint32_t incl = (uint32_t)len;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t v = __shfl_up(incl, (unsigned)d);
-      if (lane >= d) incl += v;
+    coop_spans(ws, len, sr, frag);
+  }
+}
+
+// The tiled walk (a9) of one tile entry per lane, spread over the wave the
+// same way: each lane emits its tile's row spans (both halves, each row's
+// edges by the tile walk's own arithmetic), the wave walks their pixels.
+// frag(ax, cy, tz, seq) sees tile_walk's fragments.  All lanes must call it.
+template <class F>
+__device__ __forceinline__ void coop_tile_walk(WaveSpans& ws, bool valid, const Geom<float>& g,
+                                               const TileBox& b, uint32_t order, F&& frag) {
+  int half = 2;
+  Edges e;
+  float cx1 = 0.f, cx2 = 0.f, cz1 = 0.f, cz2 = 0.f, yEnd = 0.f;
+  int cy = 0;
+  auto start = [&](int h) {
+    e = tile_edges(g, h);
+    const float yv = h == 0 ? g.y1 : g.y3;
+    cy = f2i(oclamp(yv, b.yMin, b.yMax));
+    const float dy = (float)cy - yv;
+    const float xv = h == 0 ? g.x1 : g.x3, zv = h == 0 ? g.z1 : g.z3;
+    cx1 = xv + dy * e.s1;
+    cx2 = xv + dy * e.s2;
+    cz1 = zv + dy * e.sz1;
+    cz2 = zv + dy * e.sz2;
+    yEnd = oclamp(g.y2, b.yMin, b.yMax);
+  };
+  if (valid) {
+    if (g.y2 > b.yMin) {
+      start(0);
+      half = 0;
+    } else if (g.y2 < b.yMax) {
+      start(1);
+      half = 1;
     }
-    const uint32_t total = __shfl(incl, 63);
-    if (total == 0) continue;
-    ws.end[lane] = incl;
-    if (len) ws.r[lane] = sr;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t base = 0; base < total; base += 64) {
-      const uint32_t i = base + (uint32_t)lane;
-      if (i < total) {
-        int lo = 0, hi = 63;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (ws.end[mid] > i)
-            hi = mid;
-          else
-            lo = mid + 1;
+  }
+  while (__ballot(half < 2)) {
+    int len = 0;
+    SpanRec sr;
+    if (half == 0) {
+      if ((float)cy <= yEnd) {
+        if (cx2 < b.xMax && cx1 > b.xMin) {
+          const float xe = oclamp(cx1, b.xMin, b.xMax);
+          const int xa = f2i(oclamp(cx2, b.xMin, b.xMax));
+          if ((float)xa <= xe) len = f2i(floorf(xe)) - xa + 1;
+          sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 0), cy, xa};
         }
-        const uint32_t start = lo ? ws.end[lo - 1] : 0u;
-        const SpanRec r = ws.r[lo];
-        const int ax = r.xa + (int)(i - start);
-        const float t = span_t(ax, r.cx1, r.cx2);
-        frag(ax, r.cy, span_tz(t, r.cz1, r.cz2), r.seq);
+        cx1 += e.s1;
+        cx2 += e.s2;
+        cz1 += e.sz1;
+        cz2 += e.sz2;
+        cy++;
+      } else if (g.y2 < b.yMax) {
+        start(1);
+        half = 1;
+      } else {
+        half = 2;
+      }
+    } else if (half == 1) {
+      if ((float)cy >= yEnd) {
+        if (cx1 < b.xMax && cx2 > b.xMin) {
+          const float xe = oclamp(cx2, b.xMin, b.xMax);
+          const int xa = f2i(oclamp(cx1, b.xMin, b.xMax));
+          if ((float)xa <= xe) len = f2i(floorf(xe)) - xa + 1;
+          sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 1), cy, xa};
+        }
+        cx1 -= e.s1;
+        cx2 -= e.s2;
+        cz1 -= e.sz1;
+        cz2 -= e.sz2;
+        cy--;
+      } else {
+        half = 2;
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    coop_spans(ws, len, sr, frag);
   }
 }
 
@@ -263,20 +347,36 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
     }
     coop_small_walk(ws, valid && big, g, gi, x0, y0, xe, ye, put);
   });
-  for_bin_entries(B, s_large, B.num[s_large], tid, nthr, [&](uint32_t v) {
-    const uint32_t gi = v >> 2;
-    const TileBox box = tile_box(entry_tile<SHIFT>(v, bx, by, dc.WC), dc.W);
-    const AxrTriRec r = rec[gi];
-    const Geom<float> g = tile_geom(r.px, r.py, r.z);
-    const uint32_t order = dc.T + gi;
-    tile_walk(g, box, [&](int ax, int cy, float tz, int half) {
-      if (!(tz <= AXR_DEPTH_CLEAR)) return;
-      // a tile is larger than an 8-px bin: keep the bin's part of it
-      if (SHIFT < 4 && ((unsigned)(ax - x0) >= (unsigned)BIN || (unsigned)(cy - y0) >= (unsigned)BIN))
-        return;
-      const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
-      if (lc < (unsigned)NPX) atomicMin(&lk[lc], key(tz, frag_seq(order, half)));
-    });
+  auto put_tile = [&](int ax, int cy, float tz, uint32_t seq) {
+    if (!(tz <= AXR_DEPTH_CLEAR)) return;
+    // a tile is larger than an 8-px bin: keep the bin's part of it
+    if (SHIFT < 4 && ((unsigned)(ax - x0) >= (unsigned)BIN || (unsigned)(cy - y0) >= (unsigned)BIN))
+      return;
+    const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
+    if (lc < (unsigned)NPX) atomicMin(&lk[lc], key(tz, seq));
+  };
+  for_bin_entries_waves(B, s_large, B.num[s_large], tid, nthr, [&](uint32_t v, bool valid) {
+    Geom<float> g{};
+    TileBox box{};
+    uint32_t order = 0;
+    bool big = false;
+    if (valid) {
+      const uint32_t gi = v >> 2;
+      box = tile_box(entry_tile<SHIFT>(v, bx, by, dc.WC), dc.W);
+      const AxrTriRec r = rec[gi];
+      g = tile_geom(r.px, r.py, r.z);
+      order = dc.T + gi;
+      // the triangle's box inside the tile
+      const float bw = omin(omax(g.x1, omax(g.x2, g.x3)), box.xMax) -
+                       omax(omin(g.x1, omin(g.x2, g.x3)), box.xMin);
+      const float bh = omin(g.y3, box.yMax) - omax(g.y1, box.yMin);
+      big = (bw + 1.0f) * (bh + 1.0f) > (float)AXR_COOP_AREA;
+      if (!big)
+        tile_walk(g, box, [&](int ax, int cy, float tz, int half) {
+          put_tile(ax, cy, tz, frag_seq(order, half));
+        });
+    }
+    coop_tile_walk(ws, valid && big, g, box, order, put_tile);
   });
 }
 

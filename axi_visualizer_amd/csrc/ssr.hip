@@ -94,9 +94,13 @@ __device__ __forceinline__ void ssr_px(const AxrDrawConst& dc, const AxrGroupDev
       if (on[j]) {
         sx += slopex;
         sy += slopey;
+        const float sz0 = sz, iz0 = iz;
         sz += slopez;
         iz = 1.f / sz;
-        sd = fabsf(iz - 1.f / (sz - slopez)) + kReflDBias;
+        // 1/(sz - slopez): usually sz - slopez gives back sz0 exactly, whose
+        // reciprocal is at hand (the same division, the same bits)
+        const float back = sz - slopez;
+        sd = fabsf(iz - (back == sz0 ? iz0 : 1.f / back)) + kReflDBias;
       }
     }
     float f_[kMarch];
