@@ -225,7 +225,7 @@ int bin_raster(axr_ctx* c, int mode, float exposure) {
   B.nb = c->NB;
   HIPCHK(launch_bin_raster(mode, c->d_rec, B, empty, c->d_vpos, c->d_vattr, c->d_groups, dc,
                            c->NBX, exposure, c->d_depth, c->d_r, c->d_g, c->d_b, c->d_rgba,
-                           c->d_ids, c->stream));
+                           c->d_ids, c->d_large_ctl + 1, c->stream));
   return 0;
 }
 
@@ -356,7 +356,7 @@ int run_setup(axr_ctx* c, const AxrDrawConst* dc_main, const AxrGroupDev* table,
   {
     Timed tm(c, dc_main ? "bin_order" : "shadow_order");
     HIPCHK(launch_bin_order(dc_main ? &c->mbins.B : nullptr, ps.n ? &c->sbins.B : nullptr,
-                            c->d_counters, c->d_large_ctl, c->stream));
+                            c->d_counters, c->d_large_ctl, c->d_large_ctl + 1, c->stream));
   }
   if (ps.n) {
     Timed tm(c, "shadow_raster");
@@ -403,10 +403,10 @@ int axr_create(int w, int h, axr_ctx** out) {
   if ((rc = dalloc(&c->d_depth, np)) || (rc = dalloc(&c->d_r, np)) ||
       (rc = dalloc(&c->d_g, np)) || (rc = dalloc(&c->d_b, np)) || (rc = dalloc(&c->d_rgba, np)) ||
       (rc = dalloc(&c->d_ids, np)) || (rc = dalloc(&c->d_counters, kCounters)) ||
-      (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)) || (rc = dalloc(&c->d_large_ctl, 1)))
+      (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)) || (rc = dalloc(&c->d_large_ctl, 2)))
     return bail(rc);
   if (launch_fill_u32(c->d_counters, kCounters, 0, c->stream) != hipSuccess ||
-      launch_fill_u32(c->d_large_ctl, 1, 0, c->stream) != hipSuccess ||
+      launch_fill_u32(c->d_large_ctl, 2, 0, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(fail(AXR_EHIP, "initial fill failed"));
   c->mbins.nseg = 2;

@@ -311,11 +311,11 @@ __device__ __forceinline__ void coop_tile_walk(WaveSpans& ws, bool valid, const 
 template <int SHIFT, bool FIRST>
 __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec* __restrict__ rec,
                                          const AxrBins& B, uint32_t s_small, uint32_t s_large,
-                                         int bx, int by, const AxrDrawConst& dc) {
+                                         int bx, int by, const AxrDrawConst& dc, int tid,
+                                         int nthr) {
   constexpr int BIN = 1 << SHIFT, NPX = BIN * BIN;
   const int x0 = bx << SHIFT, y0 = by << SHIFT;
   const int xe = imin(x0 + BIN, dc.W), ye = imin(y0 + BIN, dc.H);
-  const int tid = threadIdx.x, nthr = blockDim.x;
   auto key = [](float tz, uint32_t seq) -> unsigned long long {
     return FIRST ? make_key_first(tz, seq) : make_key(tz, seq);
   };
@@ -324,7 +324,8 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
   constexpr int kWaves = (NPX < 512 ? NPX : 512) / 64;
   static_assert(AXR_RASTER_THREADS <= 512, "span tables are sized for 512-thread blocks");
   __shared__ WaveSpans spans[kWaves];
-  WaveSpans& ws = spans[(tid >> 6) < kWaves ? (tid >> 6) : kWaves - 1];
+  const int wv = (int)(threadIdx.x >> 6);  // the block's wave (threads split into bins keep theirs)
+  WaveSpans& ws = spans[wv < kWaves ? wv : kWaves - 1];
   auto put = [&](int ax, int cy, float tz, uint32_t seq) {
     if (!(tz <= AXR_DEPTH_CLEAR)) return;
     const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
@@ -386,7 +387,10 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
 // on their reference tile (a9); each fragment takes an LDS min of
 // (orderable tz << 32) | ~seq.  Covered pixels are then compacted and
 // shaded once from their winning key; empty ones get the clear colour.
-// `empty` draws only the background.  lk, list, ncov are the block's LDS.
+// `empty` draws only the background.  lk, list, ncov are the bin's LDS;
+// threads tid of nthr work on it (a block may hold two bins, each with half
+// of its threads: every group passes the same barriers, an inactive group
+// with no work).
 template <int SHIFT, int MODE>
 __device__ __forceinline__ void raster_bin(
     unsigned long long* lk, uint16_t* list, uint32_t* ncov, const AxrTriRec* __restrict__ rec,
@@ -394,15 +398,21 @@ __device__ __forceinline__ void raster_bin(
     const float4* __restrict__ vpos, const float4* __restrict__ vattr,
     const AxrGroupDev* __restrict__ groups, const AxrDrawConst& dc, float exposure,
     float* __restrict__ depth, uint16_t* __restrict__ ro, uint16_t* __restrict__ go,
-    uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba, uint32_t* __restrict__ ids) {
+    uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba, uint32_t* __restrict__ ids,
+    int tid, int nthr, bool active = true) {
   constexpr int BIN = 1 << SHIFT, NPX = BIN * BIN;
+  if (!active) {  // a thread group without a bin: the same barriers, no work
+    __syncthreads();
+    __syncthreads();
+    if (MODE != SHADE_IDS) __syncthreads();
+    return;
+  }
   const int x0 = bx << SHIFT, y0 = by << SHIFT;
   const int xe = imin(x0 + BIN, dc.W), ye = imin(y0 + BIN, dc.H);
-  const int tid = threadIdx.x, nthr = blockDim.x;
   for (int i = tid; i < NPX; i += nthr) lk[i] = key_clear();
   if (tid == 0) *ncov = 0;
   __syncthreads();
-  if (!empty) walk_bin<SHIFT, false>(lk, rec, B, s_small, s_large, bx, by, dc);
+  if (!empty) walk_bin<SHIFT, false>(lk, rec, B, s_small, s_large, bx, by, dc, tid, nthr);
   __syncthreads();
   for (int i = tid; i < NPX; i += nthr) {
     const int x = x0 + (i & (BIN - 1)), y = y0 + (i >> SHIFT);

@@ -22,6 +22,9 @@ using namespace axr;
 namespace {
 
 constexpr int kSetupBlock = 256;
+#ifndef AXR_BATCH_THREADS
+#define AXR_BATCH_THREADS 256  // threads of a raster block (a bin of up to 32x32)
+#endif
 
 // Job owning setup block blk (jobs ascend in blk0).
 __device__ __forceinline__ int job_of_block(const AxrJobDev* __restrict__ jobs, int n,
@@ -94,7 +97,8 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
 }
 
 template <int SHIFT, int MODE>
-__global__ __launch_bounds__(1 << (2 * SHIFT) > 512 ? 512 : 1 << (2 * SHIFT)) void k_batch_raster(
+__global__ __launch_bounds__(1 << (2 * SHIFT) > AXR_BATCH_THREADS ? AXR_BATCH_THREADS
+                                                                  : 1 << (2 * SHIFT)) void k_batch_raster(
     const AxrJobDev* __restrict__ jobs, int njobs, AxrBins B) {
   constexpr int NPX = 1 << (2 * SHIFT);
   __shared__ unsigned long long lk[NPX];
@@ -110,7 +114,8 @@ __global__ __launch_bounds__(1 << (2 * SHIFT) > 512 ? 512 : 1 << (2 * SHIFT)) vo
     const int b = (int)(slot - J.slot0);
     raster_bin<SHIFT, MODE>(lk, list, &ncov, J.rec, B, slot, (uint32_t)B.nb + slot, 0,
                             b % J.nbx, b / J.nbx, J.vpos, J.vattr, J.groups, J.dc, J.exposure,
-                            J.depth, J.r, J.g, J.b, J.rgba, J.ids);
+                            J.depth, J.r, J.g, J.b, J.rgba, J.ids, (int)threadIdx.x,
+                            (int)blockDim.x);
     __syncthreads();  // the block's LDS is reused by the next bin
   }
 }
@@ -118,7 +123,7 @@ __global__ __launch_bounds__(1 << (2 * SHIFT) > 512 ? 512 : 1 << (2 * SHIFT)) vo
 template <int SHIFT, int MODE>
 hipError_t raster_launch(const AxrJobDev* jobs, int njobs, const AxrBins& B, int grid,
                          hipStream_t s) {
-  constexpr int threads = (1 << (2 * SHIFT)) > 512 ? 512 : (1 << (2 * SHIFT));
+  constexpr int threads = (1 << (2 * SHIFT)) > AXR_BATCH_THREADS ? AXR_BATCH_THREADS : (1 << (2 * SHIFT));
   hipLaunchKernelGGL((k_batch_raster<SHIFT, MODE>), dim3(grid), dim3(threads), 0, s, jobs, njobs,
                      B);
   return hipGetLastError();
@@ -126,7 +131,7 @@ hipError_t raster_launch(const AxrJobDev* jobs, int njobs, const AxrBins& B, int
 
 template <int SHIFT, int MODE>
 int raster_occupancy() {
-  constexpr int threads = (1 << (2 * SHIFT)) > 512 ? 512 : (1 << (2 * SHIFT));
+  constexpr int threads = (1 << (2 * SHIFT)) > AXR_BATCH_THREADS ? AXR_BATCH_THREADS : (1 << (2 * SHIFT));
   int blocks = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_batch_raster<SHIFT, MODE>, threads,
                                                    0) != hipSuccess)

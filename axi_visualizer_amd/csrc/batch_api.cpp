@@ -184,7 +184,7 @@ int enqueue(axr_batch* b) {
   const int grid_max = b->cu_count * occ;
   const int grid = b->nb_total < grid_max ? b->nb_total : grid_max;
   HIPCHK(launch_batch_setup(b->bin_shift, b->d_jobs, b->njobs, b->setup_blocks, B, b->stream));
-  HIPCHK(launch_bin_order(&B, nullptr, b->d_counters, nullptr, b->stream));
+  HIPCHK(launch_bin_order(&B, nullptr, b->d_counters, nullptr, nullptr, b->stream));
   HIPCHK(launch_batch_raster(b->bin_shift, b->mode, b->d_jobs, b->njobs, B, grid, b->stream));
   return 0;
 }

@@ -234,6 +234,7 @@ struct OrderJob {
   AxrBins B;
   int nseg;
   int ctr_total, ctr_spill;
+  uint32_t* heavy;  // if set: the number of bins of weight >= kHeavyWeight
 };
 
 // K2, one block per bin store: num = cnt and cnt = 0 for the next pass; the
@@ -317,6 +318,15 @@ __global__ __launch_bounds__(1024) void k_bin_order(OrderJob j0, OrderJob j1,
     if (used > B.spill_cap) atomicAdd(&ctr[AXR_CTR_LOST], used - B.spill_cap);
   }
   __syncthreads();
+  uint32_t* heavy = second ? j1.heavy : j0.heavy;
+  if (heavy) {
+    // hist[c] now counts the bins of classes above c; weight 2^k starts
+    // class 2k + 2
+    constexpr int kHeavyClass = 2 * (31 - __builtin_clz(kHeavyWeight)) + 2;
+    static_assert((kHeavyWeight & (kHeavyWeight - 1)) == 0, "a power of two");
+    if (tid == 0) *heavy = hist[kHeavyClass - 1];
+    __syncthreads();
+  }
   for (int b0 = 0; b0 < nb; b0 += 1024) {
     const int b = b0 + tid;
     const bool valid = b < nb;
@@ -445,11 +455,11 @@ hipError_t launch_chunk_bounds(const float4* vpos, uint32_t T, float4* chunks, h
 }
 
 hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins,
-                            uint32_t* ctr, uint32_t* clear, hipStream_t s) {
+                            uint32_t* ctr, uint32_t* clear, uint32_t* heavy, hipStream_t s) {
   OrderJob j[2];
   int n = 0;
-  if (main_bins) j[n++] = OrderJob{*main_bins, 2, AXR_CTR_BIN_TOTAL, AXR_CTR_BIN_SPILL};
-  if (shadow_bins) j[n++] = OrderJob{*shadow_bins, 1, AXR_CTR_SH_TOTAL, AXR_CTR_SH_SPILL};
+  if (main_bins) j[n++] = OrderJob{*main_bins, 2, AXR_CTR_BIN_TOTAL, AXR_CTR_BIN_SPILL, heavy};
+  if (shadow_bins) j[n++] = OrderJob{*shadow_bins, 1, AXR_CTR_SH_TOTAL, AXR_CTR_SH_SPILL, nullptr};
   if (n == 0) return hipSuccess;
   if (n == 1) j[1] = j[0];
   hipLaunchKernelGGL(k_bin_order, dim3(n), dim3(1024), 0, s, j[0], j[1], ctr,

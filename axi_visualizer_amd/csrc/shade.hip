@@ -19,21 +19,50 @@ using namespace axr;
 
 namespace {
 
+// A block takes one heavy bin with all its threads, or two light bins
+// with half of them each: heavy = weight >= kHeavyWeight (the order pass
+// counts them into *heavy; they lead the order).  Small-entry bins thus
+// keep most lanes busy while heavy bins keep their parallelism.
 template <int MODE>
 __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
     const AxrTriRec* __restrict__ rec, AxrBins B, int empty, const float4* __restrict__ vpos,
     const float4* __restrict__ vattr, const AxrGroupDev* __restrict__ groups, AxrDrawConst dc,
     int NBX, float exposure, float* __restrict__ depth, uint16_t* __restrict__ ro,
     uint16_t* __restrict__ go, uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba,
-    uint32_t* __restrict__ ids) {
-  __shared__ unsigned long long lk[AXR_BIN * AXR_BIN];
-  __shared__ uint16_t list[AXR_BIN * AXR_BIN];
-  __shared__ uint32_t ncov;
-  // Heavy bins first (order from the order pass); an empty draw has none.
-  const int b = empty ? (int)blockIdx.x : (int)B.order[blockIdx.x];
-  raster_bin<AXR_BIN_SHIFT, MODE>(lk, list, &ncov, rec, B, (uint32_t)b, (uint32_t)(B.nb + b),
-                                  empty, b % NBX, b / NBX, vpos, vattr, groups, dc, exposure,
-                                  depth, ro, go, bo, rgba, ids);
+    uint32_t* __restrict__ ids, const uint32_t* __restrict__ heavy) {
+  constexpr int NPX = AXR_BIN * AXR_BIN, HALF = AXR_RASTER_THREADS / 2;
+  __shared__ unsigned long long lk[2][NPX];
+  __shared__ uint16_t list[2][NPX];
+  __shared__ uint32_t ncov[2];
+  const int tid = (int)threadIdx.x;
+  if (empty) {  // no order pass: one bin per block
+    const int b = (int)blockIdx.x;
+    raster_bin<AXR_BIN_SHIFT, MODE>(lk[0], list[0], &ncov[0], rec, B, (uint32_t)b,
+                                    (uint32_t)(B.nb + b), empty, b % NBX, b / NBX, vpos, vattr,
+                                    groups, dc, exposure, depth, ro, go, bo, rgba, ids, tid,
+                                    (int)blockDim.x);
+    return;
+  }
+  const uint32_t nb = (uint32_t)B.nb, H = *heavy;
+  const uint32_t blk = blockIdx.x;
+  if (blk < H) {
+    const int b = (int)B.order[blk];
+    raster_bin<AXR_BIN_SHIFT, MODE>(lk[0], list[0], &ncov[0], rec, B, (uint32_t)b,
+                                    (uint32_t)(B.nb + b), 0, b % NBX, b / NBX, vpos, vattr,
+                                    groups, dc, exposure, depth, ro, go, bo, rgba, ids, tid,
+                                    (int)blockDim.x);
+    return;
+  }
+  const uint32_t j0 = H + 2 * (blk - H);
+  if (j0 >= nb) return;  // the launch has one block per bin; the light bins need half as many
+  const int h = tid >= HALF ? 1 : 0;
+  const uint32_t j = j0 + (uint32_t)h;
+  const bool active = j < nb;
+  const int b = active ? (int)B.order[j] : 0;
+  raster_bin<AXR_BIN_SHIFT, MODE>(lk[h], list[h], &ncov[h], rec, B, (uint32_t)b,
+                                  (uint32_t)(B.nb + b), 0, b % NBX, b / NBX, vpos, vattr, groups,
+                                  dc, exposure, depth, ro, go, bo, rgba, ids, tid - h * HALF, HALF,
+                                  active);
 }
 
 __global__ void k_gamma_planes(uint16_t* __restrict__ r, uint16_t* __restrict__ g,
@@ -154,11 +183,11 @@ hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const AxrBins& B, i
                              const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                              const AxrDrawConst& dc, int NBX, float exposure, float* depth,
                              uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
-                             uint32_t* ids, hipStream_t s) {
+                             uint32_t* ids, const uint32_t* heavy, hipStream_t s) {
   if (B.nb == 0) return hipSuccess;
 #define AXR_LAUNCH_BIN(M)                                                                     \
   hipLaunchKernelGGL(k_bin_raster<M>, dim3(B.nb), dim3(AXR_RASTER_THREADS), 0, s, rec, B, empty, \
-                     vpos, vattr, groups, dc, NBX, exposure, depth, r, g, b, rgba, ids)
+                     vpos, vattr, groups, dc, NBX, exposure, depth, r, g, b, rgba, ids, heavy)
   if (mode == SHADE_LDR8)
     AXR_LAUNCH_BIN(SHADE_LDR8);
   else if (mode == SHADE_HDR)

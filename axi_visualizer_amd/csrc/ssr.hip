@@ -177,7 +177,8 @@ __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_ssr_raster(
   }
   if (tid == 0) ncov = 0;
   __syncthreads();
-  walk_bin<AXR_BIN_SHIFT, true>(lk, rec, B, (uint32_t)b, (uint32_t)(B.nb + b), bx, by, dc);
+  walk_bin<AXR_BIN_SHIFT, true>(lk, rec, B, (uint32_t)b, (uint32_t)(B.nb + b), bx, by, dc, tid,
+                                nthr);
   __syncthreads();
   for (int i = tid; i < NPX; i += nthr) {
     const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
