@@ -302,6 +302,15 @@ __device__ __forceinline__ void coop_tile_walk(WaveSpans& ws, bool valid, const 
   }
 }
 
+// Strict-test passes (FIRST) only: true when no fragment of the triangle
+// can be nearer than zfar, the farthest depth already in the bin.  A
+// fragment's depth interpolates the vertices' (within a few ulps of their
+// range), so a nearest vertex depth beyond zfar by 0.1 % rules it out.
+__device__ __forceinline__ bool behind_bin(const float z[3], float zfar) {
+  const float zmax = omax(z[0], omax(z[1], z[2]));  // z = 1/depth
+  return zmax > 0.0f && (1.0f / zmax) * 0.999f > zfar;
+}
+
 // Every fragment of the bin's entries takes an LDS min of its depth key:
 // small triangles of slot s_small walk drawSmall's scanlines clipped to the
 // bin (a7), large-triangle tile entries of slot s_large walk the tiled pass
@@ -312,7 +321,7 @@ template <int SHIFT, bool FIRST>
 __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec* __restrict__ rec,
                                          const AxrBins& B, uint32_t s_small, uint32_t s_large,
                                          int bx, int by, const AxrDrawConst& dc, int tid,
-                                         int nthr) {
+                                         int nthr, float zfar = 0.0f) {
   constexpr int BIN = 1 << SHIFT, NPX = BIN * BIN;
   const int x0 = bx << SHIFT, y0 = by << SHIFT;
   const int xe = imin(x0 + BIN, dc.W), ye = imin(y0 + BIN, dc.H);
@@ -336,8 +345,12 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
   for_bin_entries_waves(B, s_small, B.num[s_small], tid, nthr, [&](uint32_t gi, bool valid) {
     Geom<int> g{};
     bool big = false;
+    AxrTriRec r;
     if (valid) {
-      const AxrTriRec r = rec[gi];
+      r = rec[gi];
+      if (FIRST && behind_bin(r.z, zfar)) valid = false;  // cannot pass the strict test
+    }
+    if (valid) {
       g = small_geom(r.px, r.py, r.z);
       const int bw = imin(imax(g.x1, imax(g.x2, g.x3)), xe - 1) - imax(imin(g.x1, imin(g.x2, g.x3)), x0);
       const int bh = imin(g.y3, ye - 1) - imax(g.y1, y0);
@@ -361,10 +374,14 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
     TileBox box{};
     uint32_t order = 0;
     bool big = false;
+    AxrTriRec r;
+    if (valid) {
+      r = rec[v >> 2];
+      if (FIRST && behind_bin(r.z, zfar)) valid = false;  // cannot pass the strict test
+    }
     if (valid) {
       const uint32_t gi = v >> 2;
       box = tile_box(entry_tile<SHIFT>(v, bx, by, dc.WC), dc.W);
-      const AxrTriRec r = rec[gi];
       g = tile_geom(r.px, r.py, r.z);
       order = dc.T + gi;
       // the triangle's box inside the tile

@@ -164,21 +164,32 @@ __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_ssr_raster(
   constexpr int NPX = AXR_BIN * AXR_BIN;
   __shared__ unsigned long long lk[NPX];
   __shared__ uint16_t list[NPX];
-  __shared__ uint32_t ncov;
+  __shared__ uint32_t ncov, bin_far;
   const int b = (int)B.order[blockIdx.x];
   const int bx = b % NBX, by = b / NBX;
   const int x0 = bx << AXR_BIN_SHIFT, y0 = by << AXR_BIN_SHIFT;
   const int xe = imin(x0 + AXR_BIN, dc.W), ye = imin(y0 + AXR_BIN, dc.H);
   const int tid = threadIdx.x, nthr = blockDim.x;
   if (B.num[b] == 0 && B.num[B.nb + b] == 0) return;  // no water here
+  float zfar = 0.0f;  // farthest opaque depth of the bin (depths are positive)
   for (int i = tid; i < NPX; i += nthr) {
     const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
-    lk[i] = (x < xe && y < ye) ? make_key_first(S.F0[(size_t)y * dc.W + x], 0u) : 0ull;
+    const bool in = x < xe && y < ye;
+    const float f = in ? S.F0[(size_t)y * dc.W + x] : 0.0f;
+    lk[i] = in ? make_key_first(f, 0u) : 0ull;
+    zfar = omax(zfar, f);
   }
-  if (tid == 0) ncov = 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) zfar = omax(zfar, __shfl_xor(zfar, o));
+  if (tid == 0) {
+    ncov = 0;
+    bin_far = 0u;
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) atomicMax(&bin_far, __float_as_uint(zfar));  // non-negative: bits order
   __syncthreads();
   walk_bin<AXR_BIN_SHIFT, true>(lk, rec, B, (uint32_t)b, (uint32_t)(B.nb + b), bx, by, dc, tid,
-                                nthr);
+                                nthr, __uint_as_float(bin_far));
   __syncthreads();
   for (int i = tid; i < NPX; i += nthr) {
     const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
