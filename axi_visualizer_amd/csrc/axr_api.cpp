@@ -852,7 +852,7 @@ int axr_add_shadow_map(axr_ctx* c, int i, int size, float scale, float amb) {
   s.placed = false;
   s.clear_pending = true;  // addShadowMap fills 256*256-1 (Ops.py:1049)
   s.dim = size;
-  s.nbx = (size + AXR_BIN - 1) / AXR_BIN;
+  s.nbx = (size + AXR_SHADOW_BIN - 1) / AXR_SHADOW_BIN;
   s.nb = s.nbx * s.nbx;
   s.scale = scale;
   s.cam.scale = scale;

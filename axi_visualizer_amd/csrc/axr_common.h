@@ -117,6 +117,11 @@ AXR_HD bool cast_bit(const AxrCastMask& m, int g) {
 // with the bin's depth keys in LDS.
 #define AXR_BIN 32
 #define AXR_BIN_SHIFT 5
+// shadow-map texel bins (k_shadow_bins' block tile)
+#ifndef AXR_SHADOW_SHIFT
+#define AXR_SHADOW_SHIFT 5
+#endif
+#define AXR_SHADOW_BIN (1 << AXR_SHADOW_SHIFT)
 
 // Threads of one bin's block in the per-bin depth walks (screen and shadow).
 #ifndef AXR_RASTER_THREADS
@@ -243,16 +248,34 @@ AXR_HD float omax(float x, float y) { return x < y ? y : x; }
 AXR_HD int imin(int x, int y) { return y < x ? y : x; }
 AXR_HD int imax(int x, int y) { return x < y ? y : x; }
 AXR_HD float oclamp(float x, float lo, float hi) { return omin(omax(x, lo), hi); }
-AXR_HD int f2i(float x) {
-  if (x != x) return 0;
-  if (x >= 2147483648.0f) return 2147483647;
-  if (x < -2147483648.0f) return (-2147483647 - 1);
-  return (int)x;
+// ---------------------------------------------- correctly rounded division
+// rcp_rn: RN(1/b) from the hardware's approximate reciprocal and one Newton
+// step, exact for every b with |b| in [2^-60, 2^60] (axr_check_division
+// compares it with IEEE division for each such b).  div_rn: from y = RN(1/b)
+// and q = RN(a*y), which lies within one ulp of a/b, Markstein's step
+// q + (a - b*q)*y rounds to RN(a/b) when no intermediate under- or
+// overflows (P. Markstein, IBM J. Res. Dev. 34(1):111-119, 1990); with
+// |a|, |b| in [2^-30, 2^30] (or a = 0) the remainder a - b*q is exact.
+__device__ __forceinline__ float rcp_rn(float b) {
+  const float y0 = __builtin_amdgcn_rcpf(b);
+  return __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
 }
+__device__ __forceinline__ float div_rn(float a, float b, float y) {
+  const float q = a * y;
+  return __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+}
+
+// Written as selects over an in-range conversion so the device code has no
+// branches: NaN -> 0, x >= 2^31 -> INT_MAX, x < -2^31 -> INT_MIN.
+AXR_HD int f2i(float x) {
+  const int v = (int)fminf(fmaxf(x, -2147483648.0f), 2147483520.0f);
+  const int s = x >= 2147483648.0f ? 2147483647 : v;
+  return x != x ? 0 : s;
+}
+// NaN and x <= 0 -> 0, x >= 65535 -> 65535, else truncation.
 AXR_HD uint16_t f2u16(float x) {
-  if (!(x > 0.0f)) return 0;
-  if (x >= 65535.0f) return 65535;
-  return (uint16_t)x;
+  const int v = (int)fminf(x, 65535.0f);
+  return x > 0.0f ? (uint16_t)v : (uint16_t)0;
 }
 AXR_HD int abs_mask(int v, int m) {
   unsigned u = v < 0 ? 0u - (unsigned)v : (unsigned)v;

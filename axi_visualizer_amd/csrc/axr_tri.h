@@ -335,6 +335,40 @@ __device__ __forceinline__ float span_tz(float t, float cz1, float cz2) {
   return 1 / ((1 - t) * cz2 + t * cz1);
 }
 
+// A row whose span quotients and depth reciprocals stay where div_rn and
+// rcp_rn are exact: |cx1 - cx2| in [2^-30, 2^30]; cx2 zero or |cx2| in
+// [2^-7, 2^29], so every nonzero ax - cx2 (ax an integer below 2^24 in
+// magnitude) lies in [2^-30, 2^30]; cz1, cz2 of one sign with magnitudes in
+// [2^-59, 2^59], so their convex combinations lie in [2^-60, 2^60].
+__device__ __forceinline__ bool rn_row(int xa, int xb, float cx1, float cx2, float cz1, float cz2) {
+  const float d = fabsf(cx1 - cx2), c = fabsf(cx2), z1 = fabsf(cz1), z2 = fabsf(cz2);
+  const bool dx = d >= 0x1p-30f && d <= 0x1p30f;
+  const bool x2 = cx2 == 0.0f || (c >= 0x1p-7f && c <= 0x1p29f);
+  const bool ax = xa >= -(1 << 24) && xb <= (1 << 24);
+  const bool zs = (cz1 > 0.0f) == (cz2 > 0.0f) && z1 >= 0x1p-59f && z1 <= 0x1p59f &&
+                  z2 >= 0x1p-59f && z2 <= 0x1p59f;
+  return dx && x2 && ax && zs;
+}
+
+// frag(ax, cy, tz) for texels xa..xb of row cy, tz = span_tz(span_t(ax)).
+// Rows inside rn_row's ranges take the span width's reciprocal once and
+// two fused steps a texel; others divide in IEEE arithmetic.  Both give
+// the same bits.
+template <class F>
+__device__ __forceinline__ void span_run(int xa, int xb, int cy, float cx1, float cx2, float cz1,
+                                         float cz2, F&& frag) {
+  if (xa > xb) return;
+  if (rn_row(xa, xb, cx1, cx2, cz1, cz2)) {
+    const float d = cx1 - cx2, y = rcp_rn(d);
+    for (int ax = xa; ax <= xb; ax++) {
+      const float t = omax(0.0f, omin(1.0f, div_rn((float)ax - cx2, d, y)));
+      frag(ax, cy, rcp_rn((1 - t) * cz2 + t * cz1));
+    }
+  } else {
+    for (int ax = xa; ax <= xb; ax++) frag(ax, cy, span_tz(span_t(ax, cx1, cx2), cz1, cz2));
+  }
+}
+
 // ------------------------------------------------- a7: small-path walk
 // Templates.py:250-321.  frag(ax, cy, tz, half) for the pixels inside the
 // clip rectangle [xlo, xhi) x [ylo, yhi), which the caller keeps inside the
@@ -625,11 +659,7 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
     for (int cy = y1; cy <= y2; cy++) {
       if (cy >= yhi) break;
       if (cy >= ylo) {
-        const int xe = imin(f2i(cx1), xhi - 1);
-        for (int ax = imax(f2i(cx2), xlo); ax <= xe; ax++) {
-          float t = span_t(ax, cx1, cx2);
-          frag(ax, cy, span_tz(t, cz1, cz2));
-        }
+        span_run(imax(f2i(cx2), xlo), imin(f2i(cx1), xhi - 1), cy, cx1, cx2, cz1, cz2, frag);
       }
       cx1 += slope1;
       cx2 += slope2;
@@ -650,11 +680,7 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
     for (int cy = y3; cy > y2; cy--) {
       if (cy < ylo) break;
       if (cy < yhi) {
-        const int xe = imin(f2i(cx2), xhi - 1);
-        for (int ax = imax(f2i(cx1), xlo); ax <= xe; ax++) {
-          float t = span_t(ax, cx1, cx2);
-          frag(ax, cy, span_tz(t, cz1, cz2));
-        }
+        span_run(imax(f2i(cx1), xlo), imin(f2i(cx2), xhi - 1), cy, cx1, cx2, cz1, cz2, frag);
       }
       cx1 -= slope1;
       cx2 -= slope2;

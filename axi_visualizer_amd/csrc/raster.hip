@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
     if (ok) {
       int x0, y0, x1, y1;
       shadow_bounds(r, x0, y0, x1, y1);
-      br = bin_rect(x0, y0, x1, y1, P.dim, P.dim, AXR_BIN_SHIFT);
+      br = bin_rect(x0, y0, x1, y1, P.dim, P.dim, AXR_SHADOW_SHIFT);
     }
     for (int k = 0;; k++) {
       const bool act = k < br.n;
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(1024) void k_bin_order(OrderJob j0, OrderJob j1,
 // minimum depth keys in LDS, written once.  A bin starts from its map
 // (from_map) or from the clear depth.
 __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, AxrPassSet ps) {
-  __shared__ uint32_t lk[AXR_BIN * AXR_BIN];
+  __shared__ uint32_t lk[AXR_SHADOW_BIN * AXR_SHADOW_BIN];
   const uint32_t slot = B.order[blockIdx.x];
   // the pass owning the slot (slot0 ascends); selected field by field so the
   // by-value pass table is never indexed dynamically
@@ -377,11 +377,11 @@ __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, A
   const int b = (int)(slot - slot0);
   const uint32_t n = B.num[slot];
   if (from_map && n == 0) return;
-  const int x0 = (b % nbx) << AXR_BIN_SHIFT, y0 = (b / nbx) << AXR_BIN_SHIFT;
-  const int xe = imin(x0 + AXR_BIN, dim), ye = imin(y0 + AXR_BIN, dim);
+  const int x0 = (b % nbx) << AXR_SHADOW_SHIFT, y0 = (b / nbx) << AXR_SHADOW_SHIFT;
+  const int xe = imin(x0 + AXR_SHADOW_BIN, dim), ye = imin(y0 + AXR_SHADOW_BIN, dim);
   const uint32_t clear = ordf(AXR_DEPTH_CLEAR);
-  for (int i = threadIdx.x; i < AXR_BIN * AXR_BIN; i += blockDim.x) {
-    const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
+  for (int i = threadIdx.x; i < AXR_SHADOW_BIN * AXR_SHADOW_BIN; i += blockDim.x) {
+    const int x = x0 + (i & (AXR_SHADOW_BIN - 1)), y = y0 + (i >> AXR_SHADOW_SHIFT);
     lk[i] = (from_map && x < xe && y < ye) ? map[(size_t)y * dim + x] : clear;
   }
   __syncthreads();
@@ -389,13 +389,13 @@ __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, A
     const AxrShRec r = rec[gi];
     shadow_walk(r, dim, x0, y0, xe, ye, [&](int ax, int cy, float tz) {
       if (tz != tz) return;
-      const unsigned lc = (unsigned)(((cy - y0) << AXR_BIN_SHIFT) + (ax - x0));
-      if (lc < (unsigned)(AXR_BIN * AXR_BIN)) atomicMin(&lk[lc], ordf(tz == 0.0f ? 0.0f : tz));
+      const unsigned lc = (unsigned)(((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0));
+      if (lc < (unsigned)(AXR_SHADOW_BIN * AXR_SHADOW_BIN)) atomicMin(&lk[lc], ordf(tz == 0.0f ? 0.0f : tz));
     });
   });
   __syncthreads();
-  for (int i = threadIdx.x; i < AXR_BIN * AXR_BIN; i += blockDim.x) {
-    const int x = x0 + (i & (AXR_BIN - 1)), y = y0 + (i >> AXR_BIN_SHIFT);
+  for (int i = threadIdx.x; i < AXR_SHADOW_BIN * AXR_SHADOW_BIN; i += blockDim.x) {
+    const int x = x0 + (i & (AXR_SHADOW_BIN - 1)), y = y0 + (i >> AXR_SHADOW_SHIFT);
     if (x < xe && y < ye) map[(size_t)y * dim + x] = lk[i];
   }
 }
