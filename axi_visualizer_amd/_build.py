@@ -13,7 +13,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["raster.hip", "shade.hip", "batch.hip", "post.hip", "ssr.hip", "axr_api.cpp", "batch_api.cpp"]
+SOURCES = ["raster.hip", "shade.hip", "batch.hip", "post.hip", "ssr.hip", "check.hip", "axr_api.cpp", "batch_api.cpp"]
 HEADERS = ["axr_common.h", "axr_tri.h", "axr_launch.h", "axr_pixel.h", "axr_bin.h", "axr_ctx.h"]
 OUT = os.path.join(HERE, "libaxr.so")
 

@@ -112,14 +112,14 @@ AXR_HD bool cast_bit(const AxrCastMask& m, int g) {
   return (w >> (g & 31)) & 1u;
 }
 
-// Screen and shadow maps are binned into 32 x 32 pixel bins (two by two of
-// the reference's 16-px tiles).  A bin's triangles are walked by one block
-// with the bin's depth keys in LDS.
+// Screens are binned into 32 x 32 pixel bins (two by two of the
+// reference's 16-px tiles), shadow maps into 64 x 64 texel bins (their
+// casters are cheaper to walk than to append to many bins).  A bin's
+// triangles are walked by one block with the bin's depth keys in LDS.
 #define AXR_BIN 32
 #define AXR_BIN_SHIFT 5
-// shadow-map texel bins (k_shadow_bins' block tile)
 #ifndef AXR_SHADOW_SHIFT
-#define AXR_SHADOW_SHIFT 5
+#define AXR_SHADOW_SHIFT 6
 #endif
 #define AXR_SHADOW_BIN (1 << AXR_SHADOW_SHIFT)
 
@@ -128,7 +128,7 @@ AXR_HD bool cast_bit(const AxrCastMask& m, int g) {
 #define AXR_RASTER_THREADS 512
 #endif
 #ifndef AXR_SHADOW_THREADS
-#define AXR_SHADOW_THREADS 512
+#define AXR_SHADOW_THREADS 1024
 #endif
 
 // Device counter block.
@@ -254,8 +254,10 @@ AXR_HD float oclamp(float x, float lo, float hi) { return omin(omax(x, lo), hi);
 // compares it with IEEE division for each such b).  div_rn: from y = RN(1/b)
 // and q = RN(a*y), which lies within one ulp of a/b, Markstein's step
 // q + (a - b*q)*y rounds to RN(a/b) when no intermediate under- or
-// overflows (P. Markstein, IBM J. Res. Dev. 34(1):111-119, 1990); with
-// |a|, |b| in [2^-30, 2^30] (or a = 0) the remainder a - b*q is exact.
+// overflows (P. Markstein, IBM J. Res. Dev. 34(1):111-119, 1990).  The
+// remainder a - b*q is exact for |a|, |b| in [2^-30, 2^30] and for b an
+// integer in [1, 2^32] with a an integer-valued float; a = +0 gives the
+// IEEE signed zero, a = -0 needs div_rn0.
 __device__ __forceinline__ float rcp_rn(float b) {
   const float y0 = __builtin_amdgcn_rcpf(b);
   return __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
@@ -263,6 +265,18 @@ __device__ __forceinline__ float rcp_rn(float b) {
 __device__ __forceinline__ float div_rn(float a, float b, float y) {
   const float q = a * y;
   return __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+}
+// div_rn for a numerator that may be -0 (the correction step would turn
+// -0/b into +0; a*y carries the quotient's signed zero).
+__device__ __forceinline__ float div_rn0(float a, float b, float y) {
+  return a == 0.0f ? a * y : div_rn(a, b, y);
+}
+// Numerators that divide exactly by an integer b in [1, 2^32] through
+// div_rn0: zero, or finite with |a| >= 2^-94, so the quotient stays normal
+// and the remainder a - b*q, a multiple of ulp(q), is representable.
+__device__ __forceinline__ bool rn_num(float a) {
+  const float m = fabsf(a);
+  return (a == 0.0f) | ((m >= 0x1p-94f) & (m <= 0x1.fffffep127f));
 }
 
 // Written as selects over an in-range conversion so the device code has no

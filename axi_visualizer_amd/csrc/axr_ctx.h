@@ -52,7 +52,7 @@ struct HostShadow {
   bool placed = false;
   bool clear_pending = false;  // cleared since its last pass; map not yet written
   int dim = 0;
-  int nbx = 0, nb = 0;  // 32x32 bins
+  int nbx = 0, nb = 0;  // 64x64 texel bins
   float scale = 0.0f;
   AxrCam cam{};
   uint32_t* d_map = nullptr;

@@ -643,14 +643,37 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
   if (y2 > y3) { swp(y2, y3); swp(x2, x3); swp(z2, z3); }
   if (y1 > y2) { swp(y1, y2); swp(x1, x2); swp(z1, z2); }
   if (!(y1 < dim && y3 >= 0)) return;
-  const int x4 = f2i((float)x1 + ((float)(y2 - y1) / (float)(y3 - y1)) * (float)(x3 - x1));
-  const int y4 = y2;
-  const float z4 = z1 + (float)(y2 - y1) / (float)(y3 - y1) * (z3 - z1);
+  // The quotients of the row counts D are integers' quotients, so every
+  // integer numerator, and every z numerator rn_num accepts, divides
+  // exactly through div_rn.  A zero D leaves its half one row (or none)
+  // that starts at a single point, so the slopes D = 0 would make are never
+  // used.
+  const float D31 = (float)(y3 - y1), D21 = (float)(y2 - y1), D32 = (float)(y3 - y2);
+  const float r21 = rcp_rn(D21), r32 = rcp_rn(D32);
+  const float q = div_rn(D21, D31, rcp_rn(D31));
+  const int x4 = f2i((float)x1 + q * (float)(x3 - x1));
+  const float z4 = z1 + q * (z3 - z1);
+  const float a1 = z2 - z1, a2 = z4 - z1, a3 = z3 - z2, a4 = z3 - z4;
+  float zs[4] = {div_rn0(a1, D21, r21), div_rn0(a2, D21, r21), div_rn0(a3, D32, r32),
+                 div_rn0(a4, D32, r32)};
+  bool zok = rn_num(a1);  // no short-circuit: one branch below, not four
+  zok &= rn_num(a2);
+  zok &= rn_num(a3);
+  zok &= rn_num(a4);
+  if (!zok) {
+    // IEEE division, kept inside the branch by the empty asm
+    float b1 = a1, b2 = a2, b3 = a3, b4 = a4;
+    asm volatile("" : "+v"(b1), "+v"(b2), "+v"(b3), "+v"(b4));
+    zs[0] = b1 / D21;
+    zs[1] = b2 / D21;
+    zs[2] = b3 / D32;
+    zs[3] = b4 / D32;
+  }
   {
-    float slope1 = (float)(x2 - x1) / (float)(y2 - y1);
-    float slope2 = (float)(x4 - x1) / (float)(y4 - y1);
-    float slopez1 = (z2 - z1) / (float)(y2 - y1);
-    float slopez2 = (z4 - z1) / (float)(y4 - y1);
+    float slope1 = div_rn((float)(x2 - x1), D21, r21);
+    float slope2 = div_rn((float)(x4 - x1), D21, r21);
+    float slopez1 = zs[0];
+    float slopez2 = zs[1];
     float cx1 = (float)x1, cx2 = cx1, cz1 = z1, cz2 = z1;
     if (slope1 < slope2) {
       swp(slope1, slope2);
@@ -668,10 +691,10 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
     }
   }
   {
-    float slope1 = (float)(x3 - x2) / (float)(y3 - y2);
-    float slope2 = (float)(x3 - x4) / (float)(y3 - y4);
-    float slopez1 = (z3 - z2) / (float)(y3 - y2);
-    float slopez2 = (z3 - z4) / (float)(y3 - y4);
+    float slope1 = div_rn((float)(x3 - x2), D32, r32);
+    float slope2 = div_rn((float)(x3 - x4), D32, r32);
+    float slopez1 = zs[2];
+    float slopez2 = zs[3];
     float cx1 = (float)x3, cx2 = cx1, cz1 = z3, cz2 = z3;
     if (slope1 < slope2) {
       swp(slope1, slope2);

@@ -8,7 +8,7 @@
 //                touches and large triangles to the bins of their coarse
 //                16-px tiles (a8, Pipe/coarse.c); then the ortho setup of each
 //                pending shadow pass (a11, Pipe/trisetupOrtho.c) appended to
-//                that map's 32x32 texel bins
+//                that map's 64x64 texel bins
 //   k_bin_order  entry counts of the pass, heavy-first bin order; resets the
 //                append cursors for the next pass (one block per bin store)
 //   k_shadow_bins  each pending map's bins walked with the bin's texels in
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(1024) void k_bin_order(OrderJob j0, OrderJob j1,
   }
 }
 
-// K3: one block per 32x32 texel bin of every pending map, heavy bins first;
+// K3: one block per 64x64 texel bin of every pending map, heavy bins first;
 // minimum depth keys in LDS, written once.  A bin starts from its map
 // (from_map) or from the clear depth.
 __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, AxrPassSet ps) {
@@ -390,7 +390,8 @@ __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, A
     shadow_walk(r, dim, x0, y0, xe, ye, [&](int ax, int cy, float tz) {
       if (tz != tz) return;
       const unsigned lc = (unsigned)(((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0));
-      if (lc < (unsigned)(AXR_SHADOW_BIN * AXR_SHADOW_BIN)) atomicMin(&lk[lc], ordf(tz == 0.0f ? 0.0f : tz));
+      if (lc < (unsigned)(AXR_SHADOW_BIN * AXR_SHADOW_BIN))
+        atomicMin(&lk[lc], ordf(tz == 0.0f ? 0.0f : tz));
     });
   });
   __syncthreads();

@@ -35,33 +35,30 @@ __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
   __shared__ uint16_t list[2][NPX];
   __shared__ uint32_t ncov[2];
   const int tid = (int)threadIdx.x;
-  if (empty) {  // no order pass: one bin per block
-    const int b = (int)blockIdx.x;
-    raster_bin<AXR_BIN_SHIFT, MODE>(lk[0], list[0], &ncov[0], rec, B, (uint32_t)b,
-                                    (uint32_t)(B.nb + b), empty, b % NBX, b / NBX, vpos, vattr,
-                                    groups, dc, exposure, depth, ro, go, bo, rgba, ids, tid,
-                                    (int)blockDim.x);
-    return;
+  // one inlined walk for the three block kinds (no order pass: a bin per
+  // block; a heavy bin; a pair of light bins), so the kernel's code stays
+  // small enough for the instruction cache
+  int h = 0, t = tid, nthr = (int)blockDim.x, b = (int)blockIdx.x;
+  bool active = true;
+  if (!empty) {
+    const uint32_t nb = (uint32_t)B.nb, H = *heavy;
+    const uint32_t blk = blockIdx.x;
+    if (blk < H) {
+      b = (int)B.order[blk];
+    } else {
+      const uint32_t j0 = H + 2 * (blk - H);
+      if (j0 >= nb) return;  // one block per bin is launched; light pairs need half as many
+      h = tid >= HALF ? 1 : 0;
+      const uint32_t j = j0 + (uint32_t)h;
+      active = j < nb;
+      b = active ? (int)B.order[j] : 0;
+      t = tid - h * HALF;
+      nthr = HALF;
+    }
   }
-  const uint32_t nb = (uint32_t)B.nb, H = *heavy;
-  const uint32_t blk = blockIdx.x;
-  if (blk < H) {
-    const int b = (int)B.order[blk];
-    raster_bin<AXR_BIN_SHIFT, MODE>(lk[0], list[0], &ncov[0], rec, B, (uint32_t)b,
-                                    (uint32_t)(B.nb + b), 0, b % NBX, b / NBX, vpos, vattr,
-                                    groups, dc, exposure, depth, ro, go, bo, rgba, ids, tid,
-                                    (int)blockDim.x);
-    return;
-  }
-  const uint32_t j0 = H + 2 * (blk - H);
-  if (j0 >= nb) return;  // the launch has one block per bin; the light bins need half as many
-  const int h = tid >= HALF ? 1 : 0;
-  const uint32_t j = j0 + (uint32_t)h;
-  const bool active = j < nb;
-  const int b = active ? (int)B.order[j] : 0;
   raster_bin<AXR_BIN_SHIFT, MODE>(lk[h], list[h], &ncov[h], rec, B, (uint32_t)b,
-                                  (uint32_t)(B.nb + b), 0, b % NBX, b / NBX, vpos, vattr, groups,
-                                  dc, exposure, depth, ro, go, bo, rgba, ids, tid - h * HALF, HALF,
+                                  (uint32_t)(B.nb + b), empty, b % NBX, b / NBX, vpos, vattr,
+                                  groups, dc, exposure, depth, ro, go, bo, rgba, ids, t, nthr,
                                   active);
 }
 
