@@ -55,7 +55,7 @@ SIGNATURES = {
     "axr_shadow_map": (_i, [_ctx, _i, _p, _i, _f]),
     "axr_gamma": (_i, [_ctx, _f]),
     "axr_blur": (_i, [_ctx]),
-    "axr_check_division": (_i, [ctypes.c_uint64, ctypes.c_uint32, _p]),
+    "axr_check_arith": (_i, [ctypes.c_uint64, ctypes.c_uint32, _p]),
     "axr_set_refl_tex": (_i, [_ctx, _i, _p, _p, _p, _i]),
     "axr_get_refl_tex": (_i, [_ctx, _i, _p, _p, _p]),
     "axr_draw_ssr": (_i, [_ctx, _p, _p, _i, _i, _i, _f, _i]),

@@ -250,7 +250,7 @@ AXR_HD int imax(int x, int y) { return x < y ? y : x; }
 AXR_HD float oclamp(float x, float lo, float hi) { return omin(omax(x, lo), hi); }
 // ---------------------------------------------- correctly rounded division
 // rcp_rn: RN(1/b) from the hardware's approximate reciprocal and one Newton
-// step, exact for every b with |b| in [2^-60, 2^60] (axr_check_division
+// step, exact for every b with |b| in [2^-60, 2^60] (axr_check_arith
 // compares it with IEEE division for each such b).  div_rn: from y = RN(1/b)
 // and q = RN(a*y), which lies within one ulp of a/b, Markstein's step
 // q + (a - b*q)*y rounds to RN(a/b) when no intermediate under- or
@@ -279,17 +279,39 @@ __device__ __forceinline__ bool rn_num(float a) {
   return (a == 0.0f) | ((m >= 0x1p-94f) & (m <= 0x1.fffffep127f));
 }
 
-// Written as selects over an in-range conversion so the device code has no
-// branches: NaN -> 0, x >= 2^31 -> INT_MAX, x < -2^31 -> INT_MIN.
-AXR_HD int f2i(float x) {
-  const int v = (int)fminf(fmaxf(x, -2147483648.0f), 2147483520.0f);
-  const int s = x >= 2147483648.0f ? 2147483647 : v;
-  return x != x ? 0 : s;
+// Saturating conversions: NaN -> 0, out-of-range -> the nearest limit,
+// else truncation toward zero.  On the device these are the hardware's
+// v_cvt_i32_f32 / v_cvt_u32_f32, which saturate exactly so (axr_check_arith
+// compares them with the formulas below for every float); one instruction
+// and one register where the formulas take branches or selects.
+AXR_HD int f2i_ref(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return 2147483647;
+  if (x < -2147483648.0f) return (-2147483647 - 1);
+  return (int)x;
 }
-// NaN and x <= 0 -> 0, x >= 65535 -> 65535, else truncation.
+AXR_HD uint16_t f2u16_ref(float x) {
+  if (!(x > 0.0f)) return 0;
+  if (x >= 65535.0f) return 65535;
+  return (uint16_t)x;
+}
+AXR_HD int f2i(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  int r;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+#else
+  return f2i_ref(x);
+#endif
+}
 AXR_HD uint16_t f2u16(float x) {
-  const int v = (int)fminf(x, 65535.0f);
-  return x > 0.0f ? (uint16_t)v : (uint16_t)0;
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t r;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return (uint16_t)(r < 65535u ? r : 65535u);
+#else
+  return f2u16_ref(x);
+#endif
 }
 AXR_HD int abs_mask(int v, int m) {
   unsigned u = v < 0 ? 0u - (unsigned)v : (unsigned)v;

@@ -1,6 +1,8 @@
-// Self-check of the fast correctly rounded division the raster kernels use
-// (rcp_rn, div_rn, div_rn0 in axr_common.h) against the device's IEEE
-// division, on the operand ranges the kernels guard them with.
+// Self-check of the device arithmetic the raster kernels rely on beyond
+// plain IEEE operations (axr_common.h): the fast correctly rounded division
+// (rcp_rn, div_rn, div_rn0) against the device's IEEE division on the
+// operand ranges the kernels guard it with, and the hardware saturating
+// conversions (f2i, f2u16) against their formulas for every float.
 #include "axr_common.h"
 #include "axr_ctx.h"
 
@@ -78,29 +80,40 @@ __global__ void k_check_div(uint64_t seed, uint32_t per, unsigned long long* __r
   if (nb) atomicAdd(bad, nb);
 }
 
+// Every float bit pattern: f2i and f2u16 against f2i_ref and f2u16_ref.
+__global__ void k_check_cvt(unsigned long long* __restrict__ bad) {
+  unsigned long long n = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint64_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= 0xffffffffull; i += stride) {
+    const float x = __uint_as_float((uint32_t)i);
+    n += f2i(x) != f2i_ref(x);
+    n += f2u16(x) != f2u16_ref(x);
+  }
+  if (n) atomicAdd(bad, n);
+}
+
 }  // namespace
 
-extern "C" int axr_check_division(uint64_t seed, uint32_t rounds, uint64_t out[3]) {
-  if (!out) return fail(AXR_EINVAL, "check_division: out is NULL");
-  if (rounds > 1024) return fail(AXR_EINVAL, "check_division: rounds %u > 1024", rounds);
+extern "C" int axr_check_arith(uint64_t seed, uint32_t rounds, uint64_t out[4]) {
+  if (!out) return fail(AXR_EINVAL, "check_arith: out is NULL");
+  if (rounds > 1024) return fail(AXR_EINVAL, "check_arith: rounds %u > 1024", rounds);
   unsigned long long* d = nullptr;
-  HIPCHK(hipMalloc(&d, 3 * sizeof(unsigned long long)));
-  hipError_t e = hipMemset(d, 0, 3 * sizeof(unsigned long long));
+  HIPCHK(hipMalloc(&d, 4 * sizeof(unsigned long long)));
+  hipError_t e = hipMemset(d, 0, 4 * sizeof(unsigned long long));
   constexpr unsigned kBlocks = 4096, kThreads = 256, kPer = 64;
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_check_rcp, dim3(kBlocks), dim3(kThreads), 0, 0, d);
     for (uint32_t r = 0; r < rounds; r++)
       hipLaunchKernelGGL(k_check_div, dim3(kBlocks), dim3(kThreads), 0, 0,
                          seed + (uint64_t)r * kBlocks * kThreads * kPer, kPer, d + 1, d + 2);
+    hipLaunchKernelGGL(k_check_cvt, dim3(kBlocks), dim3(kThreads), 0, 0, d + 3);
     e = hipGetLastError();
   }
-  unsigned long long h[3] = {0, 0, 0};
+  unsigned long long h[4] = {0, 0, 0, 0};
   if (e == hipSuccess) e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
   const hipError_t ef = hipFree(d);
-  if (e != hipSuccess) return fail(AXR_EHIP, "check_division: %s", hipGetErrorString(e));
-  if (ef != hipSuccess) return fail(AXR_EHIP, "check_division: %s", hipGetErrorString(ef));
-  out[0] = h[0];
-  out[1] = h[1];
-  out[2] = h[2];
+  if (e != hipSuccess) return fail(AXR_EHIP, "check_arith: %s", hipGetErrorString(e));
+  if (ef != hipSuccess) return fail(AXR_EHIP, "check_arith: %s", hipGetErrorString(ef));
+  for (int k = 0; k < 4; k++) out[k] = h[k];
   return AXR_OK;
 }

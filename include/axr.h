@@ -242,14 +242,17 @@ int axr_render_cube(axr_batch* b, axr_ctx* scene, int slot, const float pos[3], 
 /* Device bytes the batch holds (records, bins, outputs). */
 int axr_batch_get_bytes(axr_batch* b, uint64_t* out);
 
-/* Self-check of the fast correctly rounded division the raster kernels use
- * (no counterpart in the reference, whose Python/numpy arithmetic divides
- * in IEEE fp32): on the current device, compares the Newton reciprocal
- * with IEEE 1/b for every b with |b| in [2^-60, 2^60], and the Markstein
- * quotient with IEEE a/b on rounds x 2^26 random pairs from the ranges the
- * kernels admit.  out[0] = reciprocal mismatches, out[1] = pairs tested,
- * out[2] = quotient mismatches; both counts must be 0. */
-int axr_check_division(uint64_t seed, uint32_t rounds, uint64_t out[3]);
+/* Self-check of the device arithmetic the raster kernels use beyond plain
+ * IEEE operations (no counterpart in the reference, whose Python/numpy
+ * arithmetic divides in IEEE fp32 and converts with saturating casts): on
+ * the current device, compares the Newton reciprocal with IEEE 1/b for
+ * every b with |b| in [2^-60, 2^60], the Markstein quotient with IEEE a/b
+ * on rounds x 2^26 random pairs from the ranges the kernels admit, and the
+ * hardware's saturating float-to-int conversions with their formulas for
+ * every float.  out[0] = reciprocal mismatches, out[1] = pairs tested,
+ * out[2] = quotient mismatches, out[3] = conversion mismatches; the
+ * mismatch counts must all be 0. */
+int axr_check_arith(uint64_t seed, uint32_t rounds, uint64_t out[4]);
 
 #ifdef __cplusplus
 }
