@@ -246,6 +246,7 @@ __device__ __forceinline__ void for_bin_entries_waves(const AxrBins& B, uint32_t
 AXR_HD float omin(float x, float y) { return y < x ? y : x; }
 AXR_HD float omax(float x, float y) { return x < y ? y : x; }
 AXR_HD int imin(int x, int y) { return y < x ? y : x; }
+AXR_HD unsigned umin(unsigned a, unsigned b) { return a < b ? a : b; }
 AXR_HD int imax(int x, int y) { return x < y ? y : x; }
 AXR_HD float oclamp(float x, float lo, float hi) { return omin(omax(x, lo), hi); }
 // ---------------------------------------------- correctly rounded division

@@ -350,19 +350,22 @@ __device__ __forceinline__ bool rn_row(int xa, int xb, float cx1, float cx2, flo
   return dx && x2 && ax && zs;
 }
 
-// frag(ax, cy, tz) for texels xa..xb of row cy, tz = span_tz(span_t(ax)).
-// Rows inside rn_row's ranges take the span width's reciprocal once and
-// two fused steps a texel; others divide in IEEE arithmetic.  Both give
-// the same bits.
-template <class F>
+// The texels xa..xb of row cy with depth tz = span_tz(span_t(ax)).  Rows
+// inside rn_row's ranges take the span width's reciprocal once and two
+// fused steps a texel, and hand frag_key(ax, cy, ordf(tz)) the ordered key
+// directly: tz there is finite, nonzero and of the sign cz1 and cz2 share,
+// so ordf is one xor.  Other rows divide in IEEE arithmetic and call
+// frag(ax, cy, tz).  Both give the same bits.
+template <class F, class K>
 __device__ __forceinline__ void span_run(int xa, int xb, int cy, float cx1, float cx2, float cz1,
-                                         float cz2, F&& frag) {
+                                         float cz2, F&& frag, K&& frag_key) {
   if (xa > xb) return;
   if (rn_row(xa, xb, cx1, cx2, cz1, cz2)) {
     const float d = cx1 - cx2, y = rcp_rn(d);
+    const uint32_t flip = cz1 > 0.0f ? 0x80000000u : 0xffffffffu;
     for (int ax = xa; ax <= xb; ax++) {
       const float t = omax(0.0f, omin(1.0f, div_rn((float)ax - cx2, d, y)));
-      frag(ax, cy, rcp_rn((1 - t) * cz2 + t * cz1));
+      frag_key(ax, cy, __float_as_uint(rcp_rn((1 - t) * cz2 + t * cz1)) ^ flip);
     }
   } else {
     for (int ax = xa; ax <= xb; ax++) frag(ax, cy, span_tz(span_t(ax, cx1, cx2), cz1, cz2));
@@ -633,10 +636,11 @@ __device__ __forceinline__ void coarse_rows(const float px[3], const float py[3]
 
 // ------------------------------------------------ a11: shadow depth walk
 // Shaders/drawmin.c:86-176 (int coordinates, strict top loop) on a dim x dim
-// map, emitting only the texels inside [xlo, xhi) x [ylo, yhi).
-template <class F>
+// map, emitting only the texels inside [xlo, xhi) x [ylo, yhi): a texel as
+// frag(ax, cy, tz) or as frag_key(ax, cy, ordf(tz)) (span_run).
+template <class F, class K>
 __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo, int ylo, int xhi,
-                                            int yhi, F&& frag) {
+                                            int yhi, F&& frag, K&& frag_key) {
   int x1 = r.x[0], y1 = r.y[0], x2 = r.x[1], y2 = r.y[1], x3 = r.x[2], y3 = r.y[2];
   float z1 = r.z[0], z2 = r.z[1], z3 = r.z[2];
   if (y1 > y2) { swp(y1, y2); swp(x1, x2); swp(z1, z2); }
@@ -682,7 +686,8 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
     for (int cy = y1; cy <= y2; cy++) {
       if (cy >= yhi) break;
       if (cy >= ylo) {
-        span_run(imax(f2i(cx2), xlo), imin(f2i(cx1), xhi - 1), cy, cx1, cx2, cz1, cz2, frag);
+        span_run(imax(f2i(cx2), xlo), imin(f2i(cx1), xhi - 1), cy, cx1, cx2, cz1, cz2, frag,
+                 frag_key);
       }
       cx1 += slope1;
       cx2 += slope2;
@@ -703,7 +708,8 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
     for (int cy = y3; cy > y2; cy--) {
       if (cy < ylo) break;
       if (cy < yhi) {
-        span_run(imax(f2i(cx1), xlo), imin(f2i(cx2), xhi - 1), cy, cx1, cx2, cz1, cz2, frag);
+        span_run(imax(f2i(cx1), xlo), imin(f2i(cx2), xhi - 1), cy, cx1, cx2, cz1, cz2, frag,
+                 frag_key);
       }
       cx1 -= slope1;
       cx2 -= slope2;

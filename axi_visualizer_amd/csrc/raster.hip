@@ -387,12 +387,20 @@ __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, A
   __syncthreads();
   for_bin_entries(B, slot, n, (int)threadIdx.x, (int)blockDim.x, [&](uint32_t gi) {
     const AxrShRec r = rec[gi];
-    shadow_walk(r, dim, x0, y0, xe, ye, [&](int ax, int cy, float tz) {
-      if (tz != tz) return;
-      const unsigned lc = (unsigned)(((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0));
-      if (lc < (unsigned)(AXR_SHADOW_BIN * AXR_SHADOW_BIN))
-        atomicMin(&lk[lc], ordf(tz == 0.0f ? 0.0f : tz));
-    });
+    // texels lie in [x0, xe) x [y0, ye), inside the bin; the index clamp
+    // only keeps a stray one from leaving the LDS tile
+    constexpr unsigned kLast = AXR_SHADOW_BIN * AXR_SHADOW_BIN - 1;
+    shadow_walk(
+        r, dim, x0, y0, xe, ye,
+        [&](int ax, int cy, float tz) {
+          if (tz != tz) return;
+          const unsigned lc = (unsigned)(((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0));
+          atomicMin(&lk[umin(lc, kLast)], ordf(tz == 0.0f ? 0.0f : tz));
+        },
+        [&](int ax, int cy, uint32_t key) {
+          const unsigned lc = (unsigned)(((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0));
+          atomicMin(&lk[umin(lc, kLast)], key);
+        });
   });
   __syncthreads();
   for (int i = threadIdx.x; i < AXR_SHADOW_BIN * AXR_SHADOW_BIN; i += blockDim.x) {
