@@ -340,6 +340,17 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
     const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
     if (lc < (unsigned)NPX) atomicMin(&lk[lc], key(tz, seq));
   };
+  // A fragment given as its ordered depth ord = ordf(tz) (tz finite and
+  // nonzero): no test against the clear depth is needed, since a key above
+  // the bin's initial one (the clear depth's, or for the strict test the
+  // opaque depth's with a zero tie word) leaves the minimum unchanged.  The
+  // small walk keeps its pixels inside the bin; the index clamp only keeps a
+  // stray one inside the LDS tile.
+  auto put_key = [&](int ax, int cy, uint32_t ord, uint32_t seq) {
+    const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
+    atomicMin(&lk[umin(lc, (unsigned)NPX - 1u)],
+              ((unsigned long long)ord << 32) | (FIRST ? seq : ~seq));
+  };
   // A triangle whose box inside the bin is small walks on its lane; larger
   // ones walk spread over the wave.
   for_bin_entries_waves(B, s_small, B.num[s_small], tid, nthr, [&](uint32_t gi, bool valid) {
@@ -356,8 +367,12 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
       const int bh = imin(g.y3, ye - 1) - imax(g.y1, y0);
       big = (bw + 1) * (bh + 1) > AXR_COOP_AREA;
       if (!big)
-        small_walk(g, x0, y0, xe, ye,
-                   [&](int ax, int cy, float tz, int half) { put(ax, cy, tz, frag_seq(gi, half)); });
+        small_walk(
+            g, x0, y0, xe, ye,
+            [&](int ax, int cy, float tz, int half) { put(ax, cy, tz, frag_seq(gi, half)); },
+            [&](int ax, int cy, uint32_t ord, int half) {
+              put_key(ax, cy, ord, frag_seq(gi, half));
+            });
     }
     coop_small_walk(ws, valid && big, g, gi, x0, y0, xe, ye, put);
   });

@@ -373,24 +373,59 @@ __device__ __forceinline__ void span_run(int xa, int xb, int cy, float cx1, floa
 }
 
 // ------------------------------------------------- a7: small-path walk
-// Templates.py:250-321.  frag(ax, cy, tz, half) for the pixels inside the
-// clip rectangle [xlo, xhi) x [ylo, yhi), which the caller keeps inside the
-// screen (the reference's own bounds test).  Every row still accumulates,
-// so clipping never changes a fragment's value.
-template <class F>
+// Small triangles whose rows can all take div_rn and rcp_rn: vertex x in
+// [1, 2^20], so every span end (rounding accumulated over fewer than 128
+// rows) lies in [0.99, 2^21] and a nonzero cx1 - cx2 or ax - cx2 has
+// magnitude in [2^-24, 2^22]; z of one sign,
+// magnitudes in [2^-57, 2^57] within a factor 256 of each other, so the
+// fewer than 128 rows' accumulated rounding keeps cz, and so the span
+// interpolant w, inside [2^-58, 2^58].  Rows with cx1 == cx2 (a half's
+// apex: at most one pixel) still divide in IEEE arithmetic.
+__device__ __forceinline__ bool small_fast(const Geom<int>& g) {
+  const float a = fabsf(g.z1), b = fabsf(g.z2), c = fabsf(g.z3);
+  bool ok = imin(g.x1, imin(g.x2, g.x3)) >= 1;
+  ok &= imax(g.x1, imax(g.x2, g.x3)) <= (1 << 20);
+  ok &= (a >= 0x1p-57f) & (a <= 0x1p57f) & (b >= 0x1p-57f) & (b <= 0x1p57f);
+  ok &= (c >= 0x1p-57f) & (c <= 0x1p57f);
+  ok &= ((g.z1 > 0.0f) == (g.z2 > 0.0f)) & ((g.z1 > 0.0f) == (g.z3 > 0.0f));
+  ok &= fmaxf(a, fmaxf(b, c)) <= 256.0f * fminf(a, fminf(b, c));
+  return ok;
+}
+
+// Templates.py:250-321.  The pixels inside the clip rectangle [xlo, xhi) x
+// [ylo, yhi), which the caller keeps inside the screen (the reference's own
+// bounds test), as frag(ax, cy, tz, half) or, on small_fast rows, as
+// frag_key(ax, cy, ordf(tz), half): tz there is finite, nonzero and of the
+// vertices' sign, so ordf is one xor and tz needs no NaN test.  Every row
+// still accumulates, so clipping never changes a fragment's value.
+template <class F, class K>
 __device__ __forceinline__ void small_walk(const Geom<int>& g, int xlo, int ylo, int xhi, int yhi,
-                                           F&& frag) {
+                                           F&& frag, K&& frag_key) {
+  const bool fast = small_fast(g);
+  const uint32_t flip = g.z1 > 0.0f ? 0x80000000u : 0xffffffffu;
+  // the row's pixels xa.. with (float)ax <= xr and ax < xhi
+  auto row = [&](int xa, float xr, int cy, float cx1, float cx2, float cz1, float cz2, int half) {
+    const float d = cx1 - cx2;
+    if (fast && d != 0.0f) {
+      const int xb = imin(xhi - 1, f2i(floorf(xr)));
+      const float y = rcp_rn(d);
+      for (int ax = xa; ax <= xb; ax++) {
+        const float t = omax(0.0f, omin(1.0f, div_rn((float)ax - cx2, d, y)));
+        frag_key(ax, cy, __float_as_uint(rcp_rn((1 - t) * cz2 + t * cz1)) ^ flip, half);
+      }
+    } else {
+      for (int ax = xa; ax < xhi && (float)ax <= xr; ax++) {
+        float t = span_t(ax, cx1, cx2);
+        frag(ax, cy, span_tz(t, cz1, cz2), half);
+      }
+    }
+  };
   {
     Edges e = small_edges(g, 0);
     float cx1 = (float)g.x1, cx2 = cx1, cz1 = g.z1, cz2 = g.z1;
     for (int cy = g.y1; cy <= g.y2; cy++) {
       if (cy >= yhi) break;
-      if (cy >= ylo) {
-        for (int ax = imax(f2i(cx2), xlo); ax < xhi && (float)ax <= cx1; ax++) {
-          float t = span_t(ax, cx1, cx2);
-          frag(ax, cy, span_tz(t, cz1, cz2), 0);
-        }
-      }
+      if (cy >= ylo) row(imax(f2i(cx2), xlo), cx1, cy, cx1, cx2, cz1, cz2, 0);
       cx1 += e.s1;
       cx2 += e.s2;
       cz1 += e.sz1;
@@ -402,12 +437,7 @@ __device__ __forceinline__ void small_walk(const Geom<int>& g, int xlo, int ylo,
     float cx1 = (float)g.x3, cx2 = cx1, cz1 = g.z3, cz2 = g.z3;
     for (int cy = g.y3; cy >= g.y2; cy--) {
       if (cy < ylo) break;
-      if (cy < yhi) {
-        for (int ax = imax(f2i(cx1), xlo); ax < xhi && (float)ax <= cx2; ax++) {
-          float t = span_t(ax, cx1, cx2);
-          frag(ax, cy, span_tz(t, cz1, cz2), 1);
-        }
-      }
+      if (cy < yhi) row(imax(f2i(cx1), xlo), cx2, cy, cx1, cx2, cz1, cz2, 1);
       cx1 -= e.s1;
       cx2 -= e.s2;
       cz1 -= e.sz1;
