@@ -43,6 +43,11 @@ struct axr_batch {
   BinStore bins;
   uint32_t* d_head = nullptr;
   uint32_t* d_counters = nullptr;
+  // pinned copy of d_counters, written on the stream after every render so
+  // that a wait never issues a copy of its own: a synchronous copy would
+  // queue behind whatever other batch holds the device
+  uint32_t* h_ctr = nullptr;
+  std::vector<hipEvent_t> scene_ev;  // reused: one per distinct scene of a render
   AxrTriRec* d_rec = nullptr;
   uint8_t* d_cls = nullptr;
   size_t rec_cap = 0;
@@ -211,6 +216,9 @@ int axr_batch_create(axr_batch** out) {
     return bail(fail(AXR_EHIP, "stream/event creation failed"));
   int rc;
   if ((rc = dalloc(&b->d_head, 1)) || (rc = dalloc(&b->d_counters, AXR_CTR_WORDS))) return bail(rc);
+  if (hipHostMalloc((void**)&b->h_ctr, sizeof(uint32_t) * AXR_CTR_WORDS) != hipSuccess)
+    return bail(fail(AXR_EHIP, "pinned counter allocation failed"));
+  std::memset(b->h_ctr, 0, sizeof(uint32_t) * AXR_CTR_WORDS);
   if (launch_fill_u32(b->d_counters, AXR_CTR_WORDS, 0, b->stream) != hipSuccess ||
       launch_fill_u32(b->d_head, 1, 0, b->stream) != hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess)
@@ -239,6 +247,8 @@ int axr_batch_destroy(axr_batch* b) {
   dfree(b->d_hdr);
   dfree(b->d_jobs);
   if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+  if (b->h_ctr) (void)hipHostFree(b->h_ctr);
+  for (hipEvent_t e : b->scene_ev) (void)hipEventDestroy(e);
   if (b->ev_start) (void)hipEventDestroy(b->ev_start);
   if (b->ev_done) (void)hipEventDestroy(b->ev_done);
   if (b->ev_copied) (void)hipEventDestroy(b->ev_copied);
@@ -296,11 +306,14 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
     if ((rc = flush_shadows(sc))) return rc;
     for (int m = 0; m < AXR_MAX_SHADOWS; m++)
       if (sc->sm[m].exists && sc->sm[m].placed && (rc = apply_shadow_clear(sc, m))) return rc;
-    hipEvent_t ev;
-    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    if (scenes.size() > b->scene_ev.size()) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      b->scene_ev.push_back(ev);
+    }
+    hipEvent_t ev = b->scene_ev[scenes.size() - 1];
     HIPCHK(hipEventRecord(ev, sc->stream));
     HIPCHK(hipStreamWaitEvent(b->stream, ev, 0));
-    HIPCHK(hipEventDestroy(ev));
   }
   // Layout: bins, setup blocks, record and pixel offsets, in job order.
   const int bin = 1 << b->bin_shift;
@@ -376,6 +389,8 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
     } else if ((rc = enqueue(b))) {
       return rc;
     }
+    HIPCHK(hipMemcpyAsync(b->h_ctr, b->d_counters, sizeof(uint32_t) * AXR_CTR_WORDS,
+                          hipMemcpyDeviceToHost, b->stream));
   }
   HIPCHK(hipEventRecord(b->ev_done, b->stream));
   return AXR_OK;
@@ -397,12 +412,14 @@ int axr_batch_wait(axr_batch* b, float* gpu_ms) {
     *gpu_ms = 0.0f;
     if (b->rendered) HIPCHK(hipEventElapsedTime(gpu_ms, b->ev_start, b->ev_done));
   }
+  // the counters as the last render left them (copied on the stream before
+  // ev_done); spill and loss accumulate on the device until seen here
   uint32_t h[AXR_CTR_WORDS];
-  HIPCHK(hipMemcpy(h, b->d_counters, sizeof h, hipMemcpyDeviceToHost));
+  std::memcpy(h, (const void*)b->h_ctr, sizeof h);
   if (h[AXR_CTR_BIN_SPILL] || h[AXR_CTR_LOST]) {
     bins_grow(b->bins, h[AXR_CTR_BIN_SPILL], h[AXR_CTR_LOST]);
-    const uint32_t zero[AXR_CTR_WORDS] = {};
-    HIPCHK(hipMemcpy(b->d_counters, zero, sizeof zero, hipMemcpyHostToDevice));
+    HIPCHK(launch_fill_u32(b->d_counters, AXR_CTR_WORDS, 0, b->stream));
+    std::memset(b->h_ctr, 0, sizeof h);
     if (h[AXR_CTR_LOST])
       return fail(AXR_ECAP, "%u bin entries did not fit the spill list; it has been grown, "
                             "render the batch again",

@@ -24,7 +24,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from .batch import Batch, Job, SceneContext, settle
-from .scenes import Camera
+from .scenes import Camera, make_camera
 
 # Default bucket capacities of a 32-px bin (entries) used for admission estimates.
 _CAP_SMALL, _CAP_LARGE = 4096, 1024
@@ -91,6 +91,20 @@ class RenderServer:
         self.max_in_flight = 0     # bytes, for the admission check
         self.max_busy = 0          # batches on the device at once
         self.t0 = time.perf_counter()
+
+    def reserve(self, width: int, height: int, shadows: bool = True):
+        """Grows every slot's device pools to a full batch of the largest
+        scene at width x height before serving.  A pool that grows while
+        batches are in flight frees its old buffers, and a free waits for
+        the whole device, which would stall every slot."""
+        big = max(range(len(self.contexts)), key=lambda i: self.ntri[i])
+        ctx = self.contexts[big]
+        sc = ctx.scene
+        cam = make_camera(width, height, sc.view.pos, 0.0, 0.0)
+        cam.pos, cam.vm = sc.view.pos, sc.view.vm
+        jobs = [Job(ctx, cam, width, height, shadows=shadows) for _ in range(self.max_batch)]
+        for s in self.slots:
+            settle(s.batch, jobs)
 
     # -- clock
     def now(self) -> float:

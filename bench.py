@@ -259,7 +259,8 @@ def bench_c5(args):
     def cam(s, w, h, rng):
         return scenes.serving_camera(kinds[s], w, h, rng.random(4))
 
-    # warm-up: grow every slot's buffers once
+    # warm-up: every slot's buffers at their largest, then one short burst
+    srv.reserve(*C5_SIZES[-1])
     warm = poisson_requests(len(mix), 1e6, 200e-6, C5_SIZES, seed=99, camera_for=cam)
     srv.run(warm)
     srv.completed.clear()
