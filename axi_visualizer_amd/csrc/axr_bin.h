@@ -12,22 +12,30 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 
 // Adds one per active lane to ctr[bin] with one atomic per distinct bin in
 // the wave; returns the counter's previous value plus the lane's rank among
-// the lanes that share its bin.  All lanes of the wave must call it.
+// the lanes that share its bin.  All lanes of the wave must call it.  The
+// ballot loop only groups the lanes (no memory access); the groups' first
+// lanes then issue their atomics in one instruction, so the wave waits for
+// one round trip however many bins it touches.
 __device__ __forceinline__ uint32_t wave_bin_add(uint32_t* __restrict__ ctr, bool active,
                                                  uint32_t bin) {
   uint64_t pending = __ballot(active);
-  uint32_t result = 0;
+  int leader_of = 0;           // first lane of this lane's group
+  uint32_t rank = 0, cnt = 0;  // rank in the group; the group's size on its first lane
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
     const uint32_t b = __shfl(bin, leader);
     const uint64_t grp = __ballot(active && bin == b) & pending;
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(&ctr[b], (uint32_t)__popcll(grp));
-    base = __shfl(base, leader);
-    if ((grp >> __lane_id()) & 1ull) result = base + (uint32_t)__popcll(grp & lanemask_lt());
+    if ((grp >> __lane_id()) & 1ull) {
+      leader_of = leader;
+      rank = (uint32_t)__popcll(grp & lanemask_lt());
+    }
+    if ((int)__lane_id() == leader) cnt = (uint32_t)__popcll(grp);
     pending &= ~grp;
   }
-  return result;
+  uint32_t base = 0;
+  if (cnt) base = atomicAdd(&ctr[bin], cnt);
+  base = __shfl(base, leader_of);
+  return active ? base + rank : 0u;
 }
 
 // Bin rectangle (bins of 1 << shift pixels) of a pixel rectangle clipped to

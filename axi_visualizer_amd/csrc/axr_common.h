@@ -174,7 +174,7 @@ struct AxrJobDev {
   uint16_t *r, *g, *b;  // HDR planes (batches rendered for HDR)
   float exposure;
   uint32_t slot0;  // first screen-bin slot of this job
-  uint32_t blk0;   // first setup block of this job
+  uint32_t blk0;   // first setup block of this job's scene group
   int nbx;         // bins per row
 };
 

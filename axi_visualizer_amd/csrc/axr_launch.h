@@ -79,7 +79,10 @@ hipError_t launch_bloom(uint16_t* r, uint16_t* g, uint16_t* b, int W, int H, int
 hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipStream_t s);
 
 // Batched jobs (batch.hip).  bin_shift 3/4/5 = 8/16/32-px bins.
-hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, int njobs, uint32_t blocks,
+// groups: per scene of the batch (first setup block, offset of its job
+// indices in jidx, job count, triangles), ascending in first block.
+hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, const uint4* groups,
+                              int ngroups, const uint32_t* jidx, uint32_t blocks,
                               const AxrBins& B, hipStream_t s);
 // mode: a ShadeMode (RGBA8 with gamma, winner ids, or HDR planes).
 hipError_t launch_batch_raster(int bin_shift, int mode, const AxrJobDev* jobs, int njobs,

@@ -174,6 +174,28 @@ __device__ __forceinline__ Attr load_attr(const float4* __restrict__ vpos,
 }
 
 // ------------------------------------------------------------ a5: setup
+// d[k] = 1 / d[k] for the three vertex depths, correctly rounded: rcp_rn
+// when every |d| lies in [2^-60, 2^60] (where it equals IEEE division),
+// IEEE division, kept inside the branch by the empty asm, otherwise.
+__device__ __forceinline__ void recip3(float (&d)[3]) {
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float a = fabsf(d[k]);
+    ok &= (a >= 0x1p-60f) & (a <= 0x1p60f);
+  }
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) d[k] = rcp_rn(d[k]);
+  } else {
+    float b0 = d[0], b1 = d[1], b2 = d[2];
+    asm volatile("" : "+v"(b0), "+v"(b1), "+v"(b2));
+    d[0] = 1.0f / b0;
+    d[1] = 1.0f / b1;
+    d[2] = 1.0f / b2;
+  }
+}
+
 // Pipe/trisetup.c:28-80 (+ trisetup_cull.c:49-54).  0 reject, 1 small, 2 large.
 __device__ __forceinline__ int tri_setup(const float4 (&vp4)[3],
                                          const float4* __restrict__ vattr, uint32_t gi,
@@ -191,9 +213,9 @@ __device__ __forceinline__ int tri_setup(const float4 (&vp4)[3],
     dx[k] = dot(x[k], SVx);
     dy[k] = dot(x[k], SVy);
   }
+  recip3(dd);
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    dd[k] = 1.0f / dd[k];
     dx[k] = dx[k] * dd[k];
     dy[k] = dy[k] * dd[k];
   }
@@ -767,19 +789,25 @@ __device__ __forceinline__ bool shadow_setup(const float4 (&vp4)[3], const AxrCa
   const V3 SVd = cam_row(cam, 0), SVx = cam_row(cam, 1), SVy = cam_row(cam, 2);
   const float half = (float)(dim / 2);
   bool yes = true;
+  float dd[3], dx[3], dy[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     const float4 p = vp4[k];
     V3 x = sub(v3(p.x, p.y, p.z), vp);
-    float dd = dot(x, SVd), dx = dot(x, SVx), dy = dot(x, SVy);
-    dd += bias;
-    dd = 1 / dd;
-    dx = dx * cam.scale + half;
-    dy = dy * -cam.scale + half;
-    r.x[k] = f2i(dx);
-    r.y[k] = f2i(dy);
-    r.z[k] = dd;
-    if (!(dd > 0)) yes = false;
+    dd[k] = dot(x, SVd);
+    dx[k] = dot(x, SVx);
+    dy[k] = dot(x, SVy);
+    dd[k] += bias;
+  }
+  recip3(dd);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    dx[k] = dx[k] * cam.scale + half;
+    dy[k] = dy[k] * -cam.scale + half;
+    r.x[k] = f2i(dx[k]);
+    r.y[k] = f2i(dy[k]);
+    r.z[k] = dd[k];
+    if (!(dd[k] > 0)) yes = false;
     if (!(r.x[k] >= -120 && r.x[k] < dim + 120 && r.y[k] >= -120 && r.y[k] < dim + 120))
       yes = false;
   }

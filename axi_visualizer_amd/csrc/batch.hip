@@ -2,10 +2,12 @@
 //
 // A job is a resident scene seen by one camera at one resolution.  A batch
 // of jobs renders with three launches for all of them:
-//   k_batch_setup  one thread per (job, triangle): setup (a5) with the job's
-//                  camera, record and class in the job's arrays, appends to
-//                  the job's screen bins (slots slot0.. of one shared store;
-//                  large triangles' coarse tiles in the second segment)
+//   k_batch_setup  one thread per triangle of each scene in the batch: the
+//                  triangle is loaded once and set up (a5) with the camera of
+//                  every job on that scene in turn -- record and class in
+//                  the job's arrays, appends to the job's screen bins (slots
+//                  slot0.. of one shared store; large triangles' coarse
+//                  tiles in the second segment)
 //   k_bin_order    the shared store's counts and heavy-first order across
 //                  every job's bins (raster.hip); it also resets the queue
 //   k_batch_raster a persistent kernel: each block takes the next bin from
@@ -26,13 +28,13 @@ constexpr int kSetupBlock = 256;
 #define AXR_BATCH_THREADS 256  // threads of a raster block (a bin of up to 32x32)
 #endif
 
-// Job owning setup block blk (jobs ascend in blk0).
-__device__ __forceinline__ int job_of_block(const AxrJobDev* __restrict__ jobs, int n,
-                                            uint32_t blk) {
+// Scene group owning setup block blk: groups (blk0, first job index, jobs,
+// triangles) ascend in blk0.
+__device__ __forceinline__ int group_of_block(const uint4* __restrict__ sg, int n, uint32_t blk) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].blk0 <= blk)
+    if (sg[mid].x <= blk)
       lo = mid;
     else
       hi = mid - 1;
@@ -53,46 +55,64 @@ __device__ __forceinline__ int job_of_slot(const AxrJobDev* __restrict__ jobs, i
   return lo;
 }
 
+// The jobs of one scene group share geometry, groups and chunk spheres, so
+// a thread loads its triangle once and runs every job's setup on it; the
+// job loop is uniform over the block, so the wave-wide appends work as in
+// a single frame.
 template <int SHIFT>
 __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __restrict__ jobs,
-                                                             int njobs, AxrBins B) {
-  const AxrJobDev& J = jobs[job_of_block(jobs, njobs, blockIdx.x)];
-  const AxrDrawConst& dc = J.dc;
-  const uint32_t gi = (blockIdx.x - J.blk0) * kSetupBlock + threadIdx.x;
-  int cls = 0;
-  AxrTriRec r;
-  // the wave's 64 triangles are one chunk: skip it when out of view
-  const float4 sph = J.chunks[imin((int)(gi >> 6), (int)((dc.T - 1) >> 6))];
-  if (gi < dc.T && chunk_outside_view(sph, dc.view)) {
-    J.cls[gi] = 0;
-  } else if (gi < dc.T) {
-    float4 vp[3];
+                                                             const uint4* __restrict__ sg, int nsg,
+                                                             const uint32_t* __restrict__ jidx,
+                                                             AxrBins B) {
+  const uint4 G = sg[group_of_block(sg, nsg, blockIdx.x)];
+  const AxrJobDev& J0 = jobs[jidx[G.y]];
+  const uint32_t T = G.w;
+  const uint32_t gi = (blockIdx.x - G.x) * kSetupBlock + threadIdx.x;
+  const bool live = gi < T;
+  // the wave's 64 triangles are one chunk: a job skips it when out of view
+  const float4 sph = J0.chunks[imin((int)(gi >> 6), (int)((T - 1) >> 6))];
+  float4 vp[3];
+  uint32_t f = 0;
+  if (live) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) vp[k] = J.vpos[3 * (size_t)gi + k];
-    const uint32_t f = J.groups[find_group(J.groups, (int)dc.ngroups, gi)].flags;
-    if (f & AXR_GRP_DRAW)
-      cls = tri_setup(vp, J.vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
-                      r.z);
-    J.cls[gi] = (uint8_t)cls;
-    if (cls) J.rec[gi] = r;
+    for (int k = 0; k < 3; k++) vp[k] = J0.vpos[3 * (size_t)gi + k];
+    f = J0.groups[find_group(J0.groups, (int)J0.dc.ngroups, gi)].flags;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) vp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
-  BinRect br = {0, 0, 1, 0};
-  if (cls == 1) {
-    int x0, y0, x1, y1;
-    small_bounds(r.px, r.py, x0, y0, x1, y1);
-    br = bin_rect(x0, y0, x1, y1, dc.W, dc.H, SHIFT);
-  }
-  for (int k = 0;; k++) {
-    const bool act = k < br.n;
-    if (__ballot(act) == 0) break;
-    const uint32_t slot = act ? J.slot0 + rect_bin(br, k, J.nbx) : 0u;
-    const uint32_t pos = wave_bin_add(B.cnt, act, slot);
-    if (act) bin_store(B, slot, pos, gi);
-  }
-  if (cls == 2) {
-    coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
-      append_tile<SHIFT>(B, (uint32_t)B.nb + J.slot0, t, dc.WC, J.nbx, gi);
-    });
+  for (uint32_t q = 0; q < G.z; q++) {
+    const AxrJobDev& J = jobs[jidx[G.y + q]];
+    const AxrDrawConst& dc = J.dc;
+    int cls = 0;
+    AxrTriRec r;
+    if (live && chunk_outside_view(sph, dc.view)) {
+      J.cls[gi] = 0;
+    } else if (live) {
+      if (f & AXR_GRP_DRAW)
+        cls = tri_setup(vp, J.vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
+                        r.z);
+      J.cls[gi] = (uint8_t)cls;
+      if (cls) J.rec[gi] = r;
+    }
+    BinRect br = {0, 0, 1, 0};
+    if (cls == 1) {
+      int x0, y0, x1, y1;
+      small_bounds(r.px, r.py, x0, y0, x1, y1);
+      br = bin_rect(x0, y0, x1, y1, dc.W, dc.H, SHIFT);
+    }
+    for (int k = 0;; k++) {
+      const bool act = k < br.n;
+      if (__ballot(act) == 0) break;
+      const uint32_t slot = act ? J.slot0 + rect_bin(br, k, J.nbx) : 0u;
+      const uint32_t pos = wave_bin_add(B.cnt, act, slot);
+      if (act) bin_store(B, slot, pos, gi);
+    }
+    if (cls == 2) {
+      coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
+        append_tile<SHIFT>(B, (uint32_t)B.nb + J.slot0, t, dc.WC, J.nbx, gi);
+      });
+    }
   }
 }
 
@@ -143,18 +163,22 @@ int raster_occupancy() {
 
 namespace axr {
 
-hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, int njobs, uint32_t blocks,
+hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, const uint4* groups,
+                              int ngroups, const uint32_t* jidx, uint32_t blocks,
                               const AxrBins& B, hipStream_t s) {
-  if (njobs == 0 || blocks == 0) return hipSuccess;
+  if (ngroups == 0 || blocks == 0) return hipSuccess;
   switch (bin_shift) {
     case 3:
-      hipLaunchKernelGGL(k_batch_setup<3>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, njobs, B);
+      hipLaunchKernelGGL(k_batch_setup<3>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, groups,
+                         ngroups, jidx, B);
       break;
     case 4:
-      hipLaunchKernelGGL(k_batch_setup<4>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, njobs, B);
+      hipLaunchKernelGGL(k_batch_setup<4>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, groups,
+                         ngroups, jidx, B);
       break;
     default:
-      hipLaunchKernelGGL(k_batch_setup<5>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, njobs, B);
+      hipLaunchKernelGGL(k_batch_setup<5>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, groups,
+                         ngroups, jidx, B);
   }
   return hipGetLastError();
 }

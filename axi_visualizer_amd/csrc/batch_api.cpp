@@ -59,6 +59,12 @@ struct axr_batch {
   size_t hdr_cap = 0;
   AxrJobDev* d_jobs = nullptr;
   AxrJobDev* h_jobs = nullptr;  // pinned staging of the job table
+  // setup groups, one per scene of the batch: uint4 (first setup block,
+  // offset of its jobs in the index list, job count, triangles) x njobs at
+  // most, then the job index list (njobs words); pinned staging beside it
+  uint32_t* d_sg = nullptr;
+  uint32_t* h_sg = nullptr;
+  int nsg = 0;
   int jobs_cap = 0;
   std::vector<JobInfo> info;
   int njobs = 0;
@@ -113,10 +119,15 @@ int grow_pools(axr_batch* b, size_t tris, size_t px, int njobs, bool hdr) {
   if (njobs > b->jobs_cap) {
     HIPCHK(hipStreamSynchronize(b->stream));
     dfree(b->d_jobs);
+    dfree(b->d_sg);
     if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+    if (b->h_sg) (void)hipHostFree(b->h_sg);
     b->h_jobs = nullptr;
-    if ((rc = dalloc(&b->d_jobs, (size_t)njobs))) return rc;
+    b->h_sg = nullptr;
+    if ((rc = dalloc(&b->d_jobs, (size_t)njobs)) || (rc = dalloc(&b->d_sg, 5 * (size_t)njobs)))
+      return rc;
     HIPCHK(hipHostMalloc((void**)&b->h_jobs, sizeof(AxrJobDev) * (size_t)njobs));
+    HIPCHK(hipHostMalloc((void**)&b->h_sg, sizeof(uint32_t) * 5 * (size_t)njobs));
     b->jobs_cap = njobs;
     free_graph(b);
   }
@@ -188,7 +199,8 @@ int enqueue(axr_batch* b) {
   if (!occ) occ = batch_raster_blocks_per_cu(b->bin_shift, b->mode);
   const int grid_max = b->cu_count * occ;
   const int grid = b->nb_total < grid_max ? b->nb_total : grid_max;
-  HIPCHK(launch_batch_setup(b->bin_shift, b->d_jobs, b->njobs, b->setup_blocks, B, b->stream));
+  HIPCHK(launch_batch_setup(b->bin_shift, b->d_jobs, (const uint4*)b->d_sg, b->nsg,
+                            b->d_sg + 4 * (size_t)b->njobs, b->setup_blocks, B, b->stream));
   HIPCHK(launch_bin_order(&B, nullptr, b->d_counters, nullptr, nullptr, b->stream));
   HIPCHK(launch_batch_raster(b->bin_shift, b->mode, b->d_jobs, b->njobs, B, grid, b->stream));
   return 0;
@@ -246,7 +258,9 @@ int axr_batch_destroy(axr_batch* b) {
   dfree(b->d_ids);
   dfree(b->d_hdr);
   dfree(b->d_jobs);
+  dfree(b->d_sg);
   if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+  if (b->h_sg) (void)hipHostFree(b->h_sg);
   if (b->h_ctr) (void)hipHostFree(b->h_ctr);
   for (hipEvent_t e : b->scene_ev) (void)hipEventDestroy(e);
   if (b->ev_start) (void)hipEventDestroy(b->ev_start);
@@ -315,7 +329,8 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
     HIPCHK(hipEventRecord(ev, sc->stream));
     HIPCHK(hipStreamWaitEvent(b->stream, ev, 0));
   }
-  // Layout: bins, setup blocks, record and pixel offsets, in job order.
+  // Layout: bins, record and pixel offsets in job order; setup blocks per
+  // scene (the jobs of one scene share its setup blocks, in job order).
   const int bin = 1 << b->bin_shift;
   b->info.assign((size_t)njobs, JobInfo{});
   size_t tris = 0, px = 0;
@@ -334,12 +349,27 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
     nbx[(size_t)i] = (j.width + bin - 1) / bin;
     slot0[(size_t)i] = slots;
     slots += (uint32_t)(nbx[(size_t)i] * ((j.height + bin - 1) / bin));
-    blk0[(size_t)i] = blocks;
-    blocks += (j.scene->T + 255u) / 256u;
     rec_off[(size_t)i] = tris;
     tris += j.scene->T;
   }
+  // scene groups, in order of first appearance (scenes lists them so)
+  std::vector<uint32_t> grp_jobs;  // job indices grouped by scene
+  std::vector<uint32_t> grp_desc;  // per group: blk0, first index, count, T
+  grp_jobs.reserve((size_t)njobs);
+  for (axr_ctx* sc : scenes) {
+    const uint32_t first = (uint32_t)grp_jobs.size();
+    for (int i = 0; i < njobs; i++) {
+      if (jobs[i].scene != sc) continue;
+      grp_jobs.push_back((uint32_t)i);
+      blk0[(size_t)i] = blocks;
+    }
+    const uint32_t nblk = (sc->T + 255u) / 256u;
+    if (nblk == 0) continue;  // no triangles: nothing to set up
+    grp_desc.insert(grp_desc.end(), {blocks, first, (uint32_t)grp_jobs.size() - first, sc->T});
+    blocks += nblk;
+  }
   b->njobs = njobs;
+  b->nsg = (int)(grp_desc.size() / 4);
   b->setup_blocks = blocks;
   b->nb_total = (int)slots;
   b->mode = output == AXR_OUT_IDS ? SHADE_IDS : output == AXR_OUT_HDR ? SHADE_HDR : SHADE_LDR8;
@@ -357,18 +387,24 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
   for (int i = 0; i < njobs; i++)
     b->h_jobs[i] = make_job(b, jobs[i], b->info[(size_t)i], slot0[(size_t)i], blk0[(size_t)i],
                             nbx[(size_t)i], rec_off[(size_t)i]);
+  std::memcpy(b->h_sg, grp_desc.data(), grp_desc.size() * sizeof(uint32_t));
+  std::memcpy(b->h_sg + 4 * (size_t)njobs, grp_jobs.data(), grp_jobs.size() * sizeof(uint32_t));
   b->rendered = true;
   HIPCHK(hipEventRecord(b->ev_start, b->stream));
   if (njobs) {
     HIPCHK(hipMemcpyAsync(b->d_jobs, b->h_jobs, sizeof(AxrJobDev) * (size_t)njobs,
                           hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_sg, b->h_sg, sizeof(uint32_t) * 5 * (size_t)njobs,
+                          hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipEventRecord(b->ev_copied, b->stream));
     if (b->use_graph) {
       // The graph holds the three launches; its arguments depend on this key.
-      std::vector<uint64_t> key = {(uint64_t)njobs, blocks, slots, (uint64_t)b->bin_shift,
+      std::vector<uint64_t> key = {(uint64_t)njobs, (uint64_t)b->nsg, blocks, slots,
+                                   (uint64_t)b->bin_shift,
                                    (uint64_t)b->mode, b->bins.B.cap0, b->bins.B.cap1,
                                    (uint64_t)(uintptr_t)b->bins.B.ent,
-                                   (uint64_t)(uintptr_t)b->d_jobs};
+                                   (uint64_t)(uintptr_t)b->d_jobs,
+                                   (uint64_t)(uintptr_t)b->d_sg};
       if (!b->gexec || key != b->graph_key) {
         free_graph(b);
         hipGraph_t g = nullptr;
