@@ -378,11 +378,13 @@ __device__ __forceinline__ bool rn_row(int xa, int xb, float cx1, float cx2, flo
 // directly: tz there is finite, nonzero and of the sign cz1 and cz2 share,
 // so ordf is one xor.  Other rows divide in IEEE arithmetic and call
 // frag(ax, cy, tz).  Both give the same bits.
+// `proven`: the caller has shown rn_row's ranges for every row of the
+// triangle (shadow_fast), so only a zero span width needs IEEE division.
 template <class F, class K>
 __device__ __forceinline__ void span_run(int xa, int xb, int cy, float cx1, float cx2, float cz1,
-                                         float cz2, F&& frag, K&& frag_key) {
+                                         float cz2, F&& frag, K&& frag_key, bool proven = false) {
   if (xa > xb) return;
-  if (rn_row(xa, xb, cx1, cx2, cz1, cz2)) {
+  if (proven ? cx1 != cx2 : rn_row(xa, xb, cx1, cx2, cz1, cz2)) {
     const float d = cx1 - cx2, y = rcp_rn(d);
     const uint32_t flip = cz1 > 0.0f ? 0x80000000u : 0xffffffffu;
     for (int ax = xa; ax <= xb; ax++) {
@@ -690,6 +692,25 @@ __device__ __forceinline__ void coarse_rows(const float px[3], const float py[3]
 // Shaders/drawmin.c:86-176 (int coordinates, strict top loop) on a dim x dim
 // map, emitting only the texels inside [xlo, xhi) x [ylo, yhi): a texel as
 // frag(ax, cy, tz) or as frag_key(ax, cy, ordf(tz)) (span_run).
+// Casters whose rows all lie in rn_row's ranges, by the argument small_fast
+// makes for screen triangles: vertex x in [1, 2^20] and an x extent below
+// 96, so over fewer than 128 rows every span end stays in [0.99, 2^21] and
+// a nonzero cx1 - cx2 or ax - cx2 has magnitude in [2^-24, 2^22] (floats
+// from 0.5 up are multiples of 2^-24); z of one sign with magnitudes in
+// [2^-57, 2^57] within a factor 256 of each other, so every row's cz and
+// span interpolant stay inside [2^-59, 2^59].
+__device__ __forceinline__ bool shadow_fast(int x1, int x2, int x3, int y1, int y3, float z1,
+                                            float z2, float z3) {
+  const int xmin = imin(x1, imin(x2, x3)), xmax = imax(x1, imax(x2, x3));
+  const float a = fabsf(z1), b = fabsf(z2), c = fabsf(z3);
+  bool ok = (xmin >= 1) & (xmax <= (1 << 20)) & (xmax - xmin < 96) & (y3 - y1 < 128);
+  ok &= (a >= 0x1p-57f) & (a <= 0x1p57f) & (b >= 0x1p-57f) & (b <= 0x1p57f);
+  ok &= (c >= 0x1p-57f) & (c <= 0x1p57f);
+  ok &= ((z1 > 0.0f) == (z2 > 0.0f)) & ((z1 > 0.0f) == (z3 > 0.0f));
+  ok &= fmaxf(a, fmaxf(b, c)) <= 256.0f * fminf(a, fminf(b, c));
+  return ok;
+}
+
 template <class F, class K>
 __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo, int ylo, int xhi,
                                             int yhi, F&& frag, K&& frag_key) {
@@ -699,6 +720,7 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
   if (y2 > y3) { swp(y2, y3); swp(x2, x3); swp(z2, z3); }
   if (y1 > y2) { swp(y1, y2); swp(x1, x2); swp(z1, z2); }
   if (!(y1 < dim && y3 >= 0)) return;
+  const bool proven = shadow_fast(x1, x2, x3, y1, y3, z1, z2, z3);
   // The quotients of the row counts D are integers' quotients, so every
   // integer numerator, and every z numerator rn_num accepts, divides
   // exactly through div_rn.  A zero D leaves its half one row (or none)
@@ -739,7 +761,7 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
       if (cy >= yhi) break;
       if (cy >= ylo) {
         span_run(imax(f2i(cx2), xlo), imin(f2i(cx1), xhi - 1), cy, cx1, cx2, cz1, cz2, frag,
-                 frag_key);
+                 frag_key, proven);
       }
       cx1 += slope1;
       cx2 += slope2;
@@ -761,7 +783,7 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
       if (cy < ylo) break;
       if (cy < yhi) {
         span_run(imax(f2i(cx1), xlo), imin(f2i(cx2), xhi - 1), cy, cx1, cx2, cz1, cz2, frag,
-                 frag_key);
+                 frag_key, proven);
       }
       cx1 -= slope1;
       cx2 -= slope2;
