@@ -279,7 +279,10 @@ __device__ __forceinline__ Geom<int> small_geom(const float px[3], const float p
   g.x2 = f2i(px[1]); g.y2 = f2i(py[1]); g.z2 = z[1]; g.k2 = 1;
   g.x3 = f2i(px[2]); g.y3 = f2i(py[2]); g.z3 = z[2]; g.k3 = 2;
   geom_sort(g);
-  g.yd = (float)(g.y2 - g.y1) / (float)(g.y3 - g.y1);
+  // integer row counts below 2^24: the Markstein quotient is IEEE's
+  // (y3 > y1 for every triangle setup accepts: its int box has height)
+  const float D = (float)(g.y3 - g.y1);
+  g.yd = div_rn((float)(g.y2 - g.y1), D, rcp_rn(D));
   g.x4 = (float)g.x1 + g.yd * (float)(g.x3 - g.x1);
   g.z4 = g.z1 + g.yd * (g.z3 - g.z1);
   return g;
@@ -304,16 +307,22 @@ struct Edges {
   float s1, s2, sz1, sz2, inv;
   bool swapped;
 };
+// 1/D for an integer row count D in [0, 2^24]: rcp_rn is IEEE's reciprocal
+// there, and D = 0 (a half of one row) gives IEEE's +inf.
+__device__ __forceinline__ float rcp_rows(int d) {
+  const float D = (float)d;
+  return d == 0 ? __builtin_inff() : rcp_rn(D);
+}
 __device__ __forceinline__ Edges small_edges(const Geom<int>& g, int half) {
   Edges e;
   if (half == 0) {
-    e.inv = 1 / (float)(g.y2 - g.y1);
+    e.inv = rcp_rows(g.y2 - g.y1);
     e.s1 = (float)(g.x2 - g.x1) * e.inv;
     e.s2 = (g.x4 - (float)g.x1) * e.inv;
     e.sz1 = (g.z2 - g.z1) * e.inv;
     e.sz2 = (g.z4 - g.z1) * e.inv;
   } else {
-    e.inv = 1 / (float)(g.y3 - g.y2);
+    e.inv = rcp_rows(g.y3 - g.y2);
     e.s1 = (float)(g.x3 - g.x2) * e.inv;
     e.s2 = ((float)g.x3 - g.x4) * e.inv;
     e.sz1 = (g.z3 - g.z2) * e.inv;
