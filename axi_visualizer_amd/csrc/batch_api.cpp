@@ -564,7 +564,7 @@ int axr_batch_get_bytes(axr_batch* b, uint64_t* out) {
   n += (uint64_t)b->px_cap * 12 + (uint64_t)b->hdr_cap * 6;
   n += (uint64_t)b->bins.ent_alloc * 4 + (uint64_t)b->bins.nb_alloc * 4 * 5 +
        (uint64_t)b->bins.B.spill_cap * 8;
-  n += (uint64_t)b->jobs_cap * sizeof(AxrJobDev);
+  n += (uint64_t)b->jobs_cap * (sizeof(AxrJobDev) + 5 * sizeof(uint32_t));
   *out = n;
   return AXR_OK;
 }

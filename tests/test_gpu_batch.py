@@ -206,3 +206,28 @@ def test_autotune_returns_a_measured_best(spheres_ctx):
     res = autotune(jobs, sizes=(16, 32), cap_scales=(1,), reps=2)
     assert res["best"]["bin_size"] in (16, 32)
     assert all(r["gpu_ms"] > 0 for r in res["all"])
+
+
+def test_spilling_batch_matches_roomy_batch(spheres_ctx):
+    """Bucket capacities far below the bins' entry counts: the overflow goes
+    to the spill list and is still drawn; the wait reads the spill counter
+    from the render's own copy, grows the buckets and clears the counters,
+    and the next render (no spill) draws the same pixels."""
+    jobs = make_jobs(spheres_ctx, SIZES, seed=7)
+    tiny, roomy = Batch(cap_small=64, cap_large=64), Batch()
+    try:
+        roomy.render(jobs)
+        roomy.wait()
+        held = []
+        for rep in range(3):
+            tiny.render(jobs)
+            tiny.wait()
+            held.append(tiny.device_bytes())
+            for j in range(len(jobs)):
+                assert np.array_equal(tiny.rgba8(j), roomy.rgba8(j)), f"render {rep} job {j}"
+                assert np.array_equal(tiny.depth(j).view(np.uint32),
+                                      roomy.depth(j).view(np.uint32)), f"render {rep} job {j}"
+        assert held[1] > held[0], held  # the first render spilled: its buckets grew
+    finally:
+        tiny.close()
+        roomy.close()

@@ -32,6 +32,7 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
     srv = RenderServer([s for s, _ in sc], slots=3, max_batch=16, window_ms=1.0,
                        readback=True)
     try:
+        srv.reserve(*SIZES[0])   # pools grown before serving, as the benchmark does
         reqs = requests(kinds)
         assert len(reqs) >= 40
         rep = srv.run(reqs)
