@@ -454,13 +454,19 @@ __device__ __forceinline__ void raster_bin(
   __syncthreads();
   if (!empty) walk_bin<SHIFT, false>(lk, rec, B, s_small, s_large, bx, by, dc, tid, nthr);
   __syncthreads();
-  for (int i = tid; i < NPX; i += nthr) {
+  // nthr is a multiple of the wave size, so every lane of a wave runs the
+  // same rounds and the covered pixels are appended with one LDS atomic a
+  // wave (rank by the ballot below the lane)
+  for (int i0 = 0; i0 < NPX; i0 += nthr) {
+    const int i = i0 + tid;
     const int x = x0 + (i & (BIN - 1)), y = y0 + (i >> SHIFT);
-    if (x >= xe || y >= ye) continue;
+    const bool here = i < NPX && x < xe && y < ye;
     const size_t p = (size_t)y * dc.W + x;
-    const uint64_t key = lk[i];
+    const uint64_t key = here ? lk[i] : 0ull;
     const uint32_t low = (uint32_t)key;
-    if (MODE == SHADE_IDS) {
+    bool covered = false;
+    if (!here) {
+    } else if (MODE == SHADE_IDS) {
       uint32_t id = AXR_NONE;
       if (low != 0xFFFFFFFFu) {
         const uint32_t order = ((~low) - 1u) >> 1;
@@ -472,7 +478,17 @@ __device__ __forceinline__ void raster_bin(
       uint16_t rgb[3] = {dc.clear_rgb[0], dc.clear_rgb[1], dc.clear_rgb[2]};
       put_px<MODE>(p, AXR_DEPTH_CLEAR, rgb, exposure, depth, ro, go, bo, rgba);
     } else {
-      list[atomicAdd(ncov, 1u)] = (uint16_t)i;
+      covered = true;
+    }
+    if (MODE != SHADE_IDS) {
+      const uint64_t m = __ballot(covered);
+      if (m != 0ull) {  // uniform over the wave
+        const int first = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if ((int)__lane_id() == first) base = atomicAdd(ncov, (uint32_t)__popcll(m));
+        base = __shfl(base, first);
+        if (covered) list[base + (uint32_t)__popcll(m & lanemask_lt())] = (uint16_t)i;
+      }
     }
   }
   if (MODE == SHADE_IDS) return;
