@@ -58,9 +58,20 @@ __device__ __forceinline__ BinRect bin_rect(int x0, int y0, int x1, int y1, int 
   }
   return r;
 }
-__device__ __forceinline__ uint32_t rect_bin(const BinRect& r, int k, int nbx_total) {
-  return (uint32_t)((r.by0 + k / r.nbx) * nbx_total + r.bx0 + k % r.nbx);
-}
+// The bins of a BinRect row by row, column kx of row ky, stepped without
+// dividing: bin k of the rectangle is (k / nbx, k % nbx).
+struct BinCursor {
+  int kx = 0, ky = 0;
+  __device__ __forceinline__ uint32_t bin(const BinRect& r, int nbx_total) const {
+    return (uint32_t)((r.by0 + ky) * nbx_total + r.bx0 + kx);
+  }
+  __device__ __forceinline__ void next(const BinRect& r) {
+    if (++kx == r.nbx) {
+      kx = 0;
+      ky++;
+    }
+  }
+};
 
 // Appends a large triangle's reference 16-px tile t to the bins of
 // 1 << SHIFT pixels that hold it: a 32-px bin gets (gi << 2) | the tile's

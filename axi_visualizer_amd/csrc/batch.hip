@@ -101,12 +101,14 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
       small_bounds(r.px, r.py, x0, y0, x1, y1);
       br = bin_rect(x0, y0, x1, y1, dc.W, dc.H, SHIFT);
     }
+    BinCursor bc;
     for (int k = 0;; k++) {
       const bool act = k < br.n;
       if (__ballot(act) == 0) break;
-      const uint32_t slot = act ? J.slot0 + rect_bin(br, k, J.nbx) : 0u;
+      const uint32_t slot = act ? J.slot0 + bc.bin(br, J.nbx) : 0u;
       const uint32_t pos = wave_bin_add(B.cnt, act, slot);
       if (act) bin_store(B, slot, pos, gi);
+      bc.next(br);
     }
     if (cls == 2) {
       coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {

@@ -141,12 +141,14 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
       small_bounds(r.px, r.py, x0, y0, x1, y1);
       br = bin_rect(x0, y0, x1, y1, dc.W, dc.H, AXR_BIN_SHIFT);
     }
+    BinCursor bc;
     for (int k = 0;; k++) {
       const bool act = k < br.n;
       if (__ballot(act) == 0) break;
-      const uint32_t slot = act ? rect_bin(br, k, NBX) : 0u;
+      const uint32_t slot = act ? bc.bin(br, NBX) : 0u;
       const uint32_t pos = wave_bin_add(MB.cnt, act, slot);
       if (act) bin_store(MB, slot, pos, gi);
+      bc.next(br);
     }
     // large triangles are binned by k_coarse, a block per triangle
     const uint32_t at = wave_bin_add(large_ctl, cls == 2, 0u);
@@ -170,12 +172,14 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
       shadow_bounds(r, x0, y0, x1, y1);
       br = bin_rect(x0, y0, x1, y1, P.dim, P.dim, AXR_SHADOW_SHIFT);
     }
+    BinCursor bc;
     for (int k = 0;; k++) {
       const bool act = k < br.n;
       if (__ballot(act) == 0) break;
-      const uint32_t slot = act ? P.slot0 + rect_bin(br, k, P.nbx) : 0u;
+      const uint32_t slot = act ? P.slot0 + bc.bin(br, P.nbx) : 0u;
       const uint32_t pos = wave_bin_add(SB.cnt, act, slot);
       if (act) bin_store(SB, slot, pos, gi);
+      bc.next(br);
     }
   }
 }
