@@ -403,10 +403,10 @@ int axr_create(int w, int h, axr_ctx** out) {
   if ((rc = dalloc(&c->d_depth, np)) || (rc = dalloc(&c->d_r, np)) ||
       (rc = dalloc(&c->d_g, np)) || (rc = dalloc(&c->d_b, np)) || (rc = dalloc(&c->d_rgba, np)) ||
       (rc = dalloc(&c->d_ids, np)) || (rc = dalloc(&c->d_counters, kCounters)) ||
-      (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)) || (rc = dalloc(&c->d_large_ctl, 2)))
+      (rc = dalloc(&c->d_groups, AXR_MAX_GROUPS)) || (rc = dalloc(&c->d_large_ctl, 3)))
     return bail(rc);
   if (launch_fill_u32(c->d_counters, kCounters, 0, c->stream) != hipSuccess ||
-      launch_fill_u32(c->d_large_ctl, 2, 0, c->stream) != hipSuccess ||
+      launch_fill_u32(c->d_large_ctl, 3, 0, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(fail(AXR_EHIP, "initial fill failed"));
   c->mbins.nseg = 2;

@@ -124,8 +124,8 @@ struct axr_ctx {
   AxrTriRec* d_rec = nullptr;   // [tri_cap] setup records of the last drawAll
   uint8_t* d_cls = nullptr;     // [tri_cap] 0 culled, 1 small, 2 large
   uint32_t* d_large = nullptr;  // [tri_cap] large triangles of a setup, for coarse binning
-  // {count of d_large (zeroed by the order pass), heavy screen bins of the
-  // last order pass}
+  // {count of d_large (zeroed by the order pass), heavy and at least
+  // medium screen bins of the last order pass}
   uint32_t* d_large_ctl = nullptr;
   // [tri_cap] ortho setup and caster flags, one array per pass of a batch
   AxrShRec* d_shrec[AXR_MAX_SHADOWS] = {};

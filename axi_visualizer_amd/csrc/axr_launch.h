@@ -24,16 +24,26 @@ hipError_t launch_gamma_pack(const uint16_t* r, const uint16_t* g, const uint16_
 hipError_t launch_chunk_bounds(const float4* vpos, uint32_t T, float4* chunks, hipStream_t s);
 // Counts of the pass, heavy-first order, cursor reset for the screen bins
 // and/or the shadow bins (either may be null), one launch; with the screen
-// bins it also zeroes *clear (the setup's large-triangle count) and writes
-// the number of heavy bins to *heavy, if given.
+// bins it also zeroes *clear (the setup's large-triangle count) and, if
+// tiers is given, writes the number of heavy bins to tiers[0] and of bins
+// at least medium to tiers[1].
 // A screen bin of weight (small entries + 16 x large-tile entries) of at
-// least this many is heavy: the single-frame raster gives it a whole block.
+// least kHeavyWeight is heavy: the single-frame raster gives it a whole
+// block; of at least kMediumWeight, medium: half a block; lighter bins a
+// quarter.  Both are powers of two.
+// Measured on MI355X over the thresholds 256..4096 and 16..512: these keep
+// the 1080p/1M-triangle frame's dense bins fast and take the 1440p full
+// frame's opaque raster from 351 to 300 us.
 #ifndef AXR_HEAVY_WEIGHT
-#define AXR_HEAVY_WEIGHT 256
+#define AXR_HEAVY_WEIGHT 1024
+#endif
+#ifndef AXR_MEDIUM_WEIGHT
+#define AXR_MEDIUM_WEIGHT 256
 #endif
 constexpr uint32_t kHeavyWeight = AXR_HEAVY_WEIGHT;
+constexpr uint32_t kMediumWeight = AXR_MEDIUM_WEIGHT;
 hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins,
-                            uint32_t* ctr, uint32_t* clear, uint32_t* heavy, hipStream_t s);
+                            uint32_t* ctr, uint32_t* clear, uint32_t* tiers, hipStream_t s);
 // Depth walk of every pending map's bins, one launch.
 hipError_t launch_shadow_bins(const AxrBins& B, const AxrPassSet& ps, hipStream_t s);
 hipError_t launch_stats(const uint8_t* cls, const uint8_t* ok, uint32_t T, uint32_t* out,
@@ -45,7 +55,7 @@ hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const AxrBins& B, i
                              const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                              const AxrDrawConst& dc, int NBX, float exposure, float* depth,
                              uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
-                             uint32_t* ids, const uint32_t* heavy, hipStream_t s);
+                             uint32_t* ids, const uint32_t* tiers, hipStream_t s);
 hipError_t launch_gamma_planes(uint16_t* r, uint16_t* g, uint16_t* b, size_t n, float ex,
                                hipStream_t s);
 hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, uint16_t* b,

@@ -238,7 +238,7 @@ struct OrderJob {
   AxrBins B;
   int nseg;
   int ctr_total, ctr_spill;
-  uint32_t* heavy;  // if set: the number of bins of weight >= kHeavyWeight
+  uint32_t* heavy;  // if set: bins of weight >= kHeavyWeight, then >= kMediumWeight
 };
 
 // K2, one block per bin store: num = cnt and cnt = 0 for the next pass; the
@@ -327,8 +327,14 @@ __global__ __launch_bounds__(1024) void k_bin_order(OrderJob j0, OrderJob j1,
     // hist[c] now counts the bins of classes above c; weight 2^k starts
     // class 2k + 2
     constexpr int kHeavyClass = 2 * (31 - __builtin_clz(kHeavyWeight)) + 2;
+    constexpr int kMediumClass = 2 * (31 - __builtin_clz(kMediumWeight)) + 2;
     static_assert((kHeavyWeight & (kHeavyWeight - 1)) == 0, "a power of two");
-    if (tid == 0) *heavy = hist[kHeavyClass - 1];
+    static_assert((kMediumWeight & (kMediumWeight - 1)) == 0, "a power of two");
+    static_assert(kMediumWeight <= kHeavyWeight, "medium below heavy");
+    if (tid == 0) {
+      heavy[0] = hist[kHeavyClass - 1];
+      heavy[1] = hist[kMediumClass - 1];
+    }
     __syncthreads();
   }
   for (int b0 = 0; b0 < nb; b0 += 1024) {

@@ -19,41 +19,52 @@ using namespace axr;
 
 namespace {
 
-// A block takes one heavy bin with all its threads, or two light bins
-// with half of them each: heavy = weight >= kHeavyWeight (the order pass
-// counts them into *heavy; they lead the order).  Small-entry bins thus
-// keep most lanes busy while heavy bins keep their parallelism.
+// A block takes one heavy bin with all its threads, two medium bins with
+// half of them each, or four light bins with a quarter each: heavy = weight
+// >= kHeavyWeight, medium >= kMediumWeight (the order pass counts both into
+// tiers; heavier bins lead the order).  Dense bins keep their parallelism
+// while sparse ones keep most lanes busy and more bins in flight.
 template <int MODE>
 __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
     const AxrTriRec* __restrict__ rec, AxrBins B, int empty, const float4* __restrict__ vpos,
     const float4* __restrict__ vattr, const AxrGroupDev* __restrict__ groups, AxrDrawConst dc,
     int NBX, float exposure, float* __restrict__ depth, uint16_t* __restrict__ ro,
     uint16_t* __restrict__ go, uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba,
-    uint32_t* __restrict__ ids, const uint32_t* __restrict__ heavy) {
-  constexpr int NPX = AXR_BIN * AXR_BIN, HALF = AXR_RASTER_THREADS / 2;
-  __shared__ unsigned long long lk[2][NPX];
-  __shared__ uint16_t list[2][NPX];
-  __shared__ uint32_t ncov[2];
+    uint32_t* __restrict__ ids, const uint32_t* __restrict__ tiers) {
+  constexpr int NPX = AXR_BIN * AXR_BIN, HALF = AXR_RASTER_THREADS / 2,
+                QUARTER = AXR_RASTER_THREADS / 4;
+  static_assert(QUARTER % 64 == 0, "a quarter block is whole waves");
+  __shared__ unsigned long long lk[4][NPX];
+  __shared__ uint16_t list[4][NPX];
+  __shared__ uint32_t ncov[4];
   const int tid = (int)threadIdx.x;
-  // one inlined walk for the three block kinds (no order pass: a bin per
-  // block; a heavy bin; a pair of light bins), so the kernel's code stays
-  // small enough for the instruction cache
+  // one inlined walk for every block kind (no order pass: a bin per block;
+  // a heavy bin; a pair of medium bins; four light bins), so the kernel's
+  // code stays small enough for the instruction cache
   int h = 0, t = tid, nthr = (int)blockDim.x, b = (int)blockIdx.x;
   bool active = true;
   if (!empty) {
-    const uint32_t nb = (uint32_t)B.nb, H = *heavy;
-    const uint32_t blk = blockIdx.x;
+    const uint32_t nb = (uint32_t)B.nb, H = tiers[0], M = tiers[1];
+    const uint32_t blk = blockIdx.x, NM = (M - H + 1) / 2;  // blocks of medium pairs
     if (blk < H) {
       b = (int)B.order[blk];
-    } else {
-      const uint32_t j0 = H + 2 * (blk - H);
-      if (j0 >= nb) return;  // one block per bin is launched; light pairs need half as many
+    } else if (blk < H + NM) {
       h = tid >= HALF ? 1 : 0;
-      const uint32_t j = j0 + (uint32_t)h;
-      active = j < nb;
+      const uint32_t j = H + 2 * (blk - H) + (uint32_t)h;
+      active = j < M;
       b = active ? (int)B.order[j] : 0;
       t = tid - h * HALF;
       nthr = HALF;
+    } else {
+      // one block per bin is launched; pairs and quads need fewer
+      const uint32_t j0 = M + 4 * (blk - H - NM);
+      if (j0 >= nb) return;
+      h = tid / QUARTER;
+      const uint32_t j = j0 + (uint32_t)h;
+      active = j < nb;
+      b = active ? (int)B.order[j] : 0;
+      t = tid - h * QUARTER;
+      nthr = QUARTER;
     }
   }
   raster_bin<AXR_BIN_SHIFT, MODE>(lk[h], list[h], &ncov[h], rec, B, (uint32_t)b,
@@ -180,11 +191,11 @@ hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const AxrBins& B, i
                              const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                              const AxrDrawConst& dc, int NBX, float exposure, float* depth,
                              uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
-                             uint32_t* ids, const uint32_t* heavy, hipStream_t s) {
+                             uint32_t* ids, const uint32_t* tiers, hipStream_t s) {
   if (B.nb == 0) return hipSuccess;
 #define AXR_LAUNCH_BIN(M)                                                                     \
   hipLaunchKernelGGL(k_bin_raster<M>, dim3(B.nb), dim3(AXR_RASTER_THREADS), 0, s, rec, B, empty, \
-                     vpos, vattr, groups, dc, NBX, exposure, depth, r, g, b, rgba, ids, heavy)
+                     vpos, vattr, groups, dc, NBX, exposure, depth, r, g, b, rgba, ids, tiers)
   if (mode == SHADE_LDR8)
     AXR_LAUNCH_BIN(SHADE_LDR8);
   else if (mode == SHADE_HDR)

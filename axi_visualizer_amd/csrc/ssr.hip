@@ -156,7 +156,14 @@ __device__ __forceinline__ void ssr_px(const AxrDrawConst& dc, const AxrGroupDev
   }
 }
 
-__global__ __launch_bounds__(AXR_RASTER_THREADS) void k_ssr_raster(
+// Threads of an SSR block (one bin each): fewer than the opaque raster's,
+// so more bins' marches are in flight.
+#ifndef AXR_SSR_THREADS
+#define AXR_SSR_THREADS 256
+#endif
+static_assert(AXR_SSR_THREADS % 64 == 0 && AXR_SSR_THREADS <= 512, "whole waves, span tables");
+
+__global__ __launch_bounds__(AXR_SSR_THREADS) void k_ssr_raster(
     const AxrTriRec* __restrict__ rec, AxrBins B, const float4* __restrict__ vpos,
     const float4* __restrict__ vattr, const AxrGroupDev* __restrict__ groups, AxrDrawConst dc,
     AxrSsrConst S, int NBX, float* __restrict__ depth, uint16_t* __restrict__ ro,
@@ -234,7 +241,7 @@ hipError_t launch_ssr(const AxrTriRec* rec, const AxrBins& B, const float4* vpos
                       const AxrSsrConst& S, int NBX, float* depth, uint16_t* r, uint16_t* g,
                       uint16_t* b, hipStream_t s) {
   if (B.nb == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ssr_raster, dim3((unsigned)B.nb), dim3(AXR_RASTER_THREADS), 0, s, rec, B,
+  hipLaunchKernelGGL(k_ssr_raster, dim3((unsigned)B.nb), dim3(AXR_SSR_THREADS), 0, s, rec, B,
                      vpos, vattr, groups, dc, S, NBX, depth, r, g, b);
   return hipGetLastError();
 }
