@@ -223,9 +223,12 @@ int bin_raster(axr_ctx* c, int mode, float exposure) {
   Timed tm(c, mode == SHADE_LDR8 ? "raster_shade" : mode == SHADE_HDR ? "raster_hdr" : "raster_ids");
   AxrBins B = c->mbins.B;
   B.nb = c->NB;
+  // Bins share blocks only when there are more of them than the device holds
+  // blocks at once (two per CU): a small frame gives every bin a whole block.
+  const bool share = c->NB > 2 * c->cu_count;
   HIPCHK(launch_bin_raster(mode, c->d_rec, B, empty, c->d_vpos, c->d_vattr, c->d_groups, dc,
                            c->NBX, exposure, c->d_depth, c->d_r, c->d_g, c->d_b, c->d_rgba,
-                           c->d_ids, c->d_large_ctl + 1, c->stream));
+                           c->d_ids, share ? c->d_large_ctl + 1 : nullptr, c->stream));
   return 0;
 }
 
@@ -386,6 +389,7 @@ int axr_create(int w, int h, axr_ctx** out) {
   if ((int64_t)w * h > (int64_t)1 << 28) return fail(AXR_EINVAL, "frame too large");
   axr_ctx* c = new axr_ctx();
   HIPCHK(hipGetDevice(&c->device));
+  HIPCHK(hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, c->device));
   c->W = w;
   c->H = h;
   c->WC = w / 16;

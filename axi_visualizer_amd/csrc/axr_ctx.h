@@ -110,6 +110,7 @@ using namespace axri;
 
 struct axr_ctx {
   int device = 0;
+  int cu_count = 0;  // compute units of the device
   int W = 0, H = 0, WC = 0, HC = 0;
   int NBX = 0, NB = 0;  // 32x32 screen bins
   hipStream_t stream = nullptr;

@@ -23,7 +23,9 @@ namespace {
 // half of them each, or four light bins with a quarter each: heavy = weight
 // >= kHeavyWeight, medium >= kMediumWeight (the order pass counts both into
 // tiers; heavier bins lead the order).  Dense bins keep their parallelism
-// while sparse ones keep most lanes busy and more bins in flight.
+// while sparse ones keep most lanes busy and more bins in flight.  Without
+// tiers (a frame of fewer bins than the device holds blocks) each bin gets
+// a whole block.
 template <int MODE>
 __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
     const AxrTriRec* __restrict__ rec, AxrBins B, int empty, const float4* __restrict__ vpos,
@@ -44,7 +46,8 @@ __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
   int h = 0, t = tid, nthr = (int)blockDim.x, b = (int)blockIdx.x;
   bool active = true;
   if (!empty) {
-    const uint32_t nb = (uint32_t)B.nb, H = tiers[0], M = tiers[1];
+    // no tiers: every bin is given a whole block, in the order pass's order
+    const uint32_t nb = (uint32_t)B.nb, H = tiers ? tiers[0] : nb, M = tiers ? tiers[1] : nb;
     const uint32_t blk = blockIdx.x, NM = (M - H + 1) / 2;  // blocks of medium pairs
     if (blk < H) {
       b = (int)B.order[blk];
