@@ -167,7 +167,6 @@ struct AxrJobDev {
   const AxrGroupDev* groups;
   const float4* chunks;  // bounding sphere per 64-triangle chunk of the scene
   AxrTriRec* rec;  // [dc.T] setup records of this job
-  uint8_t* cls;    // [dc.T]
   float* depth;    // [W*H] outputs
   uint32_t* rgba;
   uint32_t* ids;

@@ -86,14 +86,10 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
     const AxrDrawConst& dc = J.dc;
     int cls = 0;
     AxrTriRec r;
-    if (live && chunk_outside_view(sph, dc.view)) {
-      J.cls[gi] = 0;
-    } else if (live) {
-      if (f & AXR_GRP_DRAW)
-        cls = tri_setup(vp, J.vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
-                        r.z);
-      J.cls[gi] = (uint8_t)cls;
-      if (cls) J.rec[gi] = r;
+    if (live && (f & AXR_GRP_DRAW) && !chunk_outside_view(sph, dc.view)) {
+      cls = tri_setup(vp, J.vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
+                      r.z);
+      if (cls) J.rec[gi] = r;  // read by the raster through the bin entries only
     }
     BinRect br = {0, 0, 1, 0};
     if (cls == 1) {

@@ -49,7 +49,6 @@ struct axr_batch {
   uint32_t* h_ctr = nullptr;
   std::vector<hipEvent_t> scene_ev;  // reused: one per distinct scene of a render
   AxrTriRec* d_rec = nullptr;
-  uint8_t* d_cls = nullptr;
   size_t rec_cap = 0;
   float* d_depth = nullptr;
   uint32_t* d_rgba = nullptr;
@@ -93,8 +92,7 @@ int grow_pools(axr_batch* b, size_t tris, size_t px, int njobs, bool hdr) {
   if (tris > b->rec_cap) {
     HIPCHK(hipStreamSynchronize(b->stream));
     dfree(b->d_rec);
-    dfree(b->d_cls);
-    if ((rc = dalloc(&b->d_rec, tris)) || (rc = dalloc(&b->d_cls, tris))) return rc;
+    if ((rc = dalloc(&b->d_rec, tris))) return rc;
     b->rec_cap = tris;
     free_graph(b);
   }
@@ -176,7 +174,6 @@ AxrJobDev make_job(const axr_batch* b, const axr_job& j, const JobInfo& in, uint
   d.groups = sc->d_groups;
   d.chunks = sc->d_chunks;
   d.rec = b->d_rec + rec_off;
-  d.cls = b->d_cls + rec_off;
   d.depth = b->d_depth + in.out_off;
   d.rgba = b->d_rgba + in.out_off;
   d.ids = b->d_ids + in.out_off;
@@ -252,7 +249,6 @@ int axr_batch_destroy(axr_batch* b) {
   dfree(b->d_head);
   dfree(b->d_counters);
   dfree(b->d_rec);
-  dfree(b->d_cls);
   dfree(b->d_depth);
   dfree(b->d_rgba);
   dfree(b->d_ids);
@@ -560,7 +556,7 @@ int axr_render_cube(axr_batch* b, axr_ctx* scene, int slot, const float pos[3], 
 int axr_batch_get_bytes(axr_batch* b, uint64_t* out) {
   if (!b || !out) return fail(AXR_EINVAL, "batch bytes: bad arguments");
   uint64_t n = 0;
-  n += (uint64_t)b->rec_cap * (sizeof(AxrTriRec) + 1);
+  n += (uint64_t)b->rec_cap * sizeof(AxrTriRec);
   n += (uint64_t)b->px_cap * 12 + (uint64_t)b->hdr_cap * 6;
   n += (uint64_t)b->bins.ent_alloc * 4 + (uint64_t)b->bins.nb_alloc * 4 * 5 +
        (uint64_t)b->bins.B.spill_cap * 8;

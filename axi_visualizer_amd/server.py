@@ -52,9 +52,9 @@ class Request:
 
 def request_bytes(ntri: int, w: int, h: int, bin_size: int = 32) -> int:
     """Device bytes a request holds while in flight: its setup records
-    (37 B per scene triangle), outputs (12 B per pixel) and bin buckets."""
+    (36 B per scene triangle), outputs (12 B per pixel) and bin buckets."""
     nb = ((w + bin_size - 1) // bin_size) * ((h + bin_size - 1) // bin_size)
-    return ntri * 37 + w * h * 12 + nb * (_CAP_SMALL + _CAP_LARGE) * 4 + nb * 20
+    return ntri * 36 + w * h * 12 + nb * (_CAP_SMALL + _CAP_LARGE) * 4 + nb * 20
 
 
 class _Slot:
