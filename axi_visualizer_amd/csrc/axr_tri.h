@@ -655,14 +655,15 @@ __device__ __forceinline__ void coarse_rows(const float px[3], const float py[3]
       cx1 += slope1;
       cx2 += slope2;
     }
-    while ((float)cy <= yEnd) {
+    // rows only grow from here: past the last tile row none is emitted
+    while ((float)cy <= yEnd && cy < HC) {
       float mx1 = cx1 + sbias * slope1;
       if (slope1 < 0) mx1 -= slope1;
       float mx2 = cx2 + sbias * slope2;
       if (slope2 > 0) mx2 -= slope2;
       mx1 = omin(mx1, wmax);
       mx2 = omax(mx2, 0.0f);
-      if (cy >= 0 && cy < HC) row(cy, f2i(floorf(mx2)), ceilf(mx1));
+      if (cy >= 0) row(cy, f2i(floorf(mx2)), ceilf(mx1));
       for (int k = 0; k < nl && (float)cy <= yEnd; k++, cy++) {
         cx1 += slope1;
         cx2 += slope2;
@@ -681,14 +682,15 @@ __device__ __forceinline__ void coarse_rows(const float px[3], const float py[3]
       cx1 -= slope1;
       cx2 -= slope2;
     }
-    while ((float)cy >= yEnd) {
+    // rows only shrink from here: above the first tile row none is emitted
+    while ((float)cy >= yEnd && cy >= 0) {
       float mx1 = cx1 - sbias * slope1;
       if (slope1 < 0) mx1 += slope1;
       float mx2 = cx2 - sbias * slope2;
       if (slope2 > 0) mx2 += slope2;
       mx2 = omin(mx2, wmax);
       mx1 = omax(mx1, 0.0f);
-      if (cy >= 0 && cy < HC) row(cy, f2i(floorf(mx1)), ceilf(mx2));
+      if (cy < HC) row(cy, f2i(floorf(mx1)), ceilf(mx2));
       for (int k = 0; k < nl && (float)cy >= yEnd; k++, cy--) {
         cx1 -= slope1;
         cx2 -= slope2;
