@@ -266,6 +266,14 @@ __device__ __forceinline__ float div_rn(float a, float b, float y) {
   const float q = a * y;
   return __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
 }
+// IEEE 1/b for any b: rcp_rn where it is exact, IEEE division (kept inside
+// the branch by the empty asm) for |b| outside [2^-60, 2^60], inf and NaN.
+__device__ __forceinline__ float rcp_any(float b) {
+  const float a = fabsf(b);
+  if ((a >= 0x1p-60f) & (a <= 0x1p60f)) return rcp_rn(b);
+  asm volatile("" : "+v"(b));
+  return 1.0f / b;
+}
 // div_rn for a numerator that may be -0 (the correction step would turn
 // -0/b into +0; a*y carries the quotient's signed zero).
 __device__ __forceinline__ float div_rn0(float a, float b, float y) {

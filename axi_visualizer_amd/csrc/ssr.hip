@@ -26,6 +26,12 @@ constexpr float kReflDBias = 0.5f;
 constexpr float kReflFade = 80.0f;
 constexpr float kMaxAttn = 0.1f;
 
+// Iterations of the ray march whose positions are computed, and depths
+// loaded, together (the loads of one group are in flight at once).
+#ifndef AXR_SSR_MARCH
+#define AXR_SSR_MARCH 4
+#endif
+
 // av_SSR.c:37-148 for one fragment; returns the colour, the caller stores
 // it and the fragment depth.
 __device__ __forceinline__ void ssr_px(const AxrDrawConst& dc, const AxrGroupDev& grp,
@@ -72,7 +78,7 @@ __device__ __forceinline__ void ssr_px(const AxrDrawConst& dc, const AxrGroupDev
   // the hit tests in order.  Iteration i runs while i * step < steps, the
   // position is on screen and sz > 0; a hit at iteration i ends the march
   // with rn = (i + 1) * step, else rn = (first iteration not run) * step.
-  constexpr int kMarch = 4;
+  constexpr int kMarch = AXR_SSR_MARCH;
   float sx = rx, sy = ry, sz = 1.f / tz, sd = kReflDepth;
   float iz = sz > 0 ? 1.f / sz : 0.f;  // 1/sz of the current iteration
   int rn = 0, hit = 0, hx = 0, hy = 0;
@@ -96,11 +102,11 @@ __device__ __forceinline__ void ssr_px(const AxrDrawConst& dc, const AxrGroupDev
         sy += slopey;
         const float sz0 = sz, iz0 = iz;
         sz += slopez;
-        iz = 1.f / sz;
+        iz = rcp_any(sz);
         // 1/(sz - slopez): usually sz - slopez gives back sz0 exactly, whose
         // reciprocal is at hand (the same division, the same bits)
         const float back = sz - slopez;
-        sd = fabsf(iz - (back == sz0 ? iz0 : 1.f / back)) + kReflDBias;
+        sd = fabsf(iz - (back == sz0 ? iz0 : rcp_any(back))) + kReflDBias;
       }
     }
     float f_[kMarch];
@@ -219,6 +225,25 @@ __global__ __launch_bounds__(AXR_SSR_THREADS) void k_ssr_raster(
   }
 }
 
+// The opaque frame's snapshot for the pass in one launch: each thread copies
+// eight pixels, depth as two 16-byte vectors and each colour plane as one
+// (the host launches it when the pixel count is a multiple of eight, which
+// keeps every plane of the snapshot 16-byte aligned).
+__global__ void k_snap_frame(const float4* __restrict__ depth, const uint4* __restrict__ r,
+                             const uint4* __restrict__ g, const uint4* __restrict__ b, size_t n8,
+                             float4* __restrict__ F0, uint4* __restrict__ R0,
+                             uint4* __restrict__ G0, uint4* __restrict__ B0) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  const float4 d0 = depth[2 * i], d1 = depth[2 * i + 1];
+  const uint4 rr = r[i], gg = g[i], bb = b[i];
+  F0[2 * i] = d0;
+  F0[2 * i + 1] = d1;
+  R0[i] = rr;
+  G0[i] = gg;
+  B0[i] = bb;
+}
+
 // Six faces' HDR planes [6][S][S] into the strip the shader samples:
 // texel (x, y) of face f at column f * S + x of row y, as (R,G,B,0).
 __global__ void k_pack_cube(const uint16_t* __restrict__ r, const uint16_t* __restrict__ g,
@@ -243,6 +268,18 @@ hipError_t launch_ssr(const AxrTriRec* rec, const AxrBins& B, const float4* vpos
   if (B.nb == 0) return hipSuccess;
   hipLaunchKernelGGL(k_ssr_raster, dim3((unsigned)B.nb), dim3(AXR_SSR_THREADS), 0, s, rec, B,
                      vpos, vattr, groups, dc, S, NBX, depth, r, g, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_snap_frame(const float* depth, const uint16_t* r, const uint16_t* g,
+                             const uint16_t* b, size_t np, float* F0, uint16_t* R0,
+                             uint16_t* G0, uint16_t* B0, hipStream_t s) {
+  if (np % 8 != 0) return hipErrorInvalidValue;
+  const size_t n8 = np / 8;
+  if (n8 == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_snap_frame, dim3((unsigned)((n8 + 255) / 256)), dim3(256), 0, s,
+                     (const float4*)depth, (const uint4*)r, (const uint4*)g, (const uint4*)b, n8,
+                     (float4*)F0, (uint4*)R0, (uint4*)G0, (uint4*)B0);
   return hipGetLastError();
 }
 
