@@ -795,15 +795,10 @@ int axr_draw_ssr(axr_ctx* c, const int* groups, const int* refl, int n, int step
   S.fresnel = fresnel ? 1 : 0;
   {
     Timed tm(c, "ssr_copy");
-    if (np % 8 == 0) {  // one launch; each plane of the snapshot stays 16-byte aligned
-      HIPCHK(launch_snap_frame(c->d_depth, c->d_r, c->d_g, c->d_b, np, (float*)S.F0,
-                               (uint16_t*)S.R0, (uint16_t*)S.G0, (uint16_t*)S.B0, c->stream));
-    } else {
-      HIPCHK(hipMemcpyAsync((void*)S.F0, c->d_depth, np * 4, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync((void*)S.R0, c->d_r, np * 2, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync((void*)S.G0, c->d_g, np * 2, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync((void*)S.B0, c->d_b, np * 2, hipMemcpyDeviceToDevice, c->stream));
-    }
+    // W is a multiple of 16, so np is a multiple of 8 and every plane of the
+    // snapshot starts 16-byte aligned: one launch of 16-byte vectors
+    HIPCHK(launch_snap_frame(c->d_depth, c->d_r, c->d_g, c->d_b, np, (float*)S.F0,
+                             (uint16_t*)S.R0, (uint16_t*)S.G0, (uint16_t*)S.B0, c->stream));
   }
   // Only the water groups take part in this draw: its own group table (the
   // drawAll's table stays as it is) and the water groups' triangle range.

@@ -227,8 +227,8 @@ __global__ __launch_bounds__(AXR_SSR_THREADS) void k_ssr_raster(
 
 // The opaque frame's snapshot for the pass in one launch: each thread copies
 // eight pixels, depth as two 16-byte vectors and each colour plane as one
-// (the host launches it when the pixel count is a multiple of eight, which
-// keeps every plane of the snapshot 16-byte aligned).
+// (frame widths are multiples of 16, so the pixel count is a multiple of
+// eight and every plane of the snapshot starts 16-byte aligned).
 __global__ void k_snap_frame(const float4* __restrict__ depth, const uint4* __restrict__ r,
                              const uint4* __restrict__ g, const uint4* __restrict__ b, size_t n8,
                              float4* __restrict__ F0, uint4* __restrict__ R0,

@@ -59,6 +59,13 @@ def test_water_with_shadows_and_64_steps():
     check(scenes.water_scene(W=256, H=192, shadow_dim=256, ssr_steps=64), "water_shadow_64")
 
 
+@pytest.mark.parametrize("W,H", [(208, 117), (48, 37)])
+def test_water_ragged_frame(W, H):
+    """Frames whose 32-px bins are cut at the right and bottom edges (the
+    march and the snapshot of the opaque frame run to the last pixel)."""
+    check(scenes.water_scene(W=W, H=H, ssr_steps=64), f"water_{W}x{H}")
+
+
 def test_metal_variant_step_two():
     sc = scenes.water_scene(W=192, H=160, n_inst=10)
     sc.ssr_step, sc.ssr_steps, sc.ssr_absorb, sc.ssr_fresnel = 2, 800, 0.4, False
