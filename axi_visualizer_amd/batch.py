@@ -101,6 +101,12 @@ class Batch:
         self.bin_size = bin_size
         self.cap_small, self.cap_large, self.graph = cap_small, cap_large, graph
 
+    def set_spill_capacity(self, spill: int):
+        """Entries of the spill list from the next render on (it grows by
+        itself after a render that lost entries)."""
+        check(self._L.axr_batch_set_spill_capacity(self._b, int(spill)),
+              "axr_batch_set_spill_capacity")
+
     def render(self, jobs: Sequence[Job], ids: bool = False, hdr: bool = False):
         """Enqueues the jobs and returns at once: RGBA8 frames, or winner ids
         (ids=True), or HDR u16 planes (hdr=True), each with depth."""

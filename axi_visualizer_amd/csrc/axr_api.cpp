@@ -265,22 +265,39 @@ int apply_shadow_clear(axr_ctx* c, int i) {
 
 // After a synchronising readback.  Spilled entries were still drawn (the
 // buckets grow for later passes); entries lost because the spill list was
-// full make the frame incomplete: report it and grow the list.
-int check_overflow(axr_ctx* c) {
+// full make the frame incomplete: report it and grow the list.  A loss is
+// attributed to no pass, so the frame and every shadow map a pass wrote
+// since the last check are marked incomplete: each stays so (its readbacks
+// fail with AXR_ECAP) until it is drawn again from a clear.
+// frame: the readback is of the frame (else of shadow map `map`).
+int check_overflow(axr_ctx* c, bool frame, int map = -1) {
   uint32_t h[kCounters];
   HIPCHK(hipMemcpy(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost));
   const uint32_t msp = h[AXR_CTR_BIN_SPILL], ssp = h[AXR_CTR_SH_SPILL], lost = h[AXR_CTR_LOST];
-  if (!msp && !ssp && !lost) return 0;
-  // A loss is not attributed to a kind: grow both spill lists.
-  bins_grow(c->mbins, msp, lost);
-  bins_grow(c->sbins, ssp, lost);
-  const uint32_t zero[3] = {0, 0, 0};
-  HIPCHK(hipMemcpy(c->d_counters + AXR_CTR_BIN_SPILL, zero, 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_counters + AXR_CTR_SH_SPILL, zero, 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->d_counters + AXR_CTR_LOST, zero, 4, hipMemcpyHostToDevice));
+  if (msp || ssp || lost) {
+    // A loss is not attributed to a kind: grow both spill lists.
+    bins_grow(c->mbins, msp, lost);
+    bins_grow(c->sbins, ssp, lost);
+    const uint32_t zero[3] = {0, 0, 0};
+    HIPCHK(hipMemcpy(c->d_counters + AXR_CTR_BIN_SPILL, zero, 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_counters + AXR_CTR_SH_SPILL, zero, 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_counters + AXR_CTR_LOST, zero, 4, hipMemcpyHostToDevice));
+  }
+  // (a frame drawn since the last clear may have lost them)
+  if (lost && c->state != ST_UNCLEARED && c->state != ST_CLEARED) c->frame_incomplete = true;
+  for (HostShadow& s : c->sm) {
+    if (lost && s.unchecked) s.incomplete = true;
+    s.unchecked = false;
+  }
   if (lost)
     return fail(AXR_ECAP, "%u bin entries did not fit the spill list; it has been grown, redraw",
                 lost);
+  if (frame && c->frame_incomplete)
+    return fail(AXR_ECAP, "the frame lost bin entries or samples an incomplete shadow map; "
+                          "redraw it (clearZBuffer, drawAll)");
+  if (map >= 0 && c->sm[map].incomplete)
+    return fail(AXR_ECAP, "shadow map %d lost bin entries; redraw it (clearShadowMap, shadowMap)",
+                map);
   return 0;
 }
 
@@ -333,6 +350,9 @@ int run_setup(axr_ctx* c, const AxrDrawConst* dc_main, const AxrGroupDev* table,
     P.ok = c->d_ok[k];
     P.map = s.d_map;
     nb_total += s.nb;
+    HostShadow& sw = c->sm[q.map];
+    if (!q.from_map) sw.incomplete = false;
+    sw.unchecked = true;
   }
   c->pend.clear();
   int rc;
@@ -647,6 +667,7 @@ int axr_clear_zbuffer(axr_ctx* c) {
   // The bins of the next drawAll start empty (the order pass zeroes them),
   // and the per-bin walk starts from the clear key: nothing to launch.
   c->state = ST_CLEARED;
+  c->frame_incomplete = false;
   return AXR_OK;
 }
 
@@ -682,6 +703,7 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
       if (i < 0 || i >= AXR_MAX_SHADOWS || !c->sm[i].exists || !c->sm[i].placed)
         return fail(AXR_ESTATE, "drawAll: shadow map %d missing or not placed", i);
       if ((rc = apply_shadow_clear(c, i))) return rc;
+      if (c->sm[i].incomplete) c->frame_incomplete = true;
       AxrShadowDev& sd = dc.sh[dc.nshadow++];
       sd.map = c->sm[i].d_map;
       sd.dim = c->sm[i].dim;
@@ -778,7 +800,6 @@ int axr_draw_ssr(axr_ctx* c, const int* groups, const int* refl, int n, int step
   if (c->snap_bytes < snap) {
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->d_snap) (void)hipFree(c->d_snap);
-  dfree(c->d_groups_ssr);
     c->d_snap = nullptr;
     c->snap_bytes = 0;
     HIPCHK(hipMalloc(&c->d_snap, snap));
@@ -892,6 +913,7 @@ int axr_clear_shadow_map(axr_ctx* c, int i) {
   if (bound_to_pending(c, i) && (rc = settle(c))) return rc;
   if (pass_pending(c, i) && (rc = flush_shadows(c))) return rc;
   c->sm[i].clear_pending = true;
+  c->sm[i].incomplete = false;
   return AXR_OK;
 }
 
@@ -1029,7 +1051,7 @@ int axr_get_frame(axr_ctx* c, uint16_t* r, uint16_t* g, uint16_t* b, float* dept
   if (b) HIPCHK(hipMemcpyAsync(b, c->d_b, n * 2, hipMemcpyDeviceToHost, c->stream));
   if (depth) HIPCHK(hipMemcpyAsync(depth, c->d_depth, n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  return check_overflow(c);
+  return check_overflow(c, true);
 }
 
 int axr_get_rgba8(axr_ctx* c, uint32_t* rgba) {
@@ -1041,7 +1063,7 @@ int axr_get_rgba8(axr_ctx* c, uint32_t* rgba) {
   if (c->state == ST_LDR8) {
     HIPCHK(hipMemcpyAsync(rgba, c->d_rgba, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    return check_overflow(c);
+    return check_overflow(c, true);
   }
   if (c->state == ST_LDR16) {
     std::vector<uint16_t> r(n), g(n), b(n);
@@ -1052,7 +1074,7 @@ int axr_get_rgba8(axr_ctx* c, uint32_t* rgba) {
     for (size_t i = 0; i < n; i++)
       rgba[i] = (uint32_t)(r[i] & 0xFF) | ((uint32_t)(g[i] & 0xFF) << 8) |
                 ((uint32_t)(b[i] & 0xFF) << 16) | 0xFF000000u;
-    return check_overflow(c);
+    return check_overflow(c, true);
   }
   return fail(AXR_ESTATE, "getRGBA8 needs gamma() first");
 }
@@ -1070,7 +1092,7 @@ int axr_get_ids(axr_ctx* c, uint32_t* ids) {
   if ((rc = bin_raster(c, SHADE_IDS, 1.0f))) return rc;
   HIPCHK(hipMemcpyAsync(ids, c->d_ids, n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  return check_overflow(c);
+  return check_overflow(c, true);
 }
 
 int axr_get_shadow_map(axr_ctx* c, int i, float* out) {
@@ -1088,7 +1110,7 @@ int axr_get_shadow_map(axr_ctx* c, int i, float* out) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   dfree(d);
   if (e != hipSuccess) return fail(AXR_EHIP, "getSHM: %s", hipGetErrorString(e));
-  return check_overflow(c);
+  return check_overflow(c, false, i);
 }
 
 int axr_sync(axr_ctx* c) {

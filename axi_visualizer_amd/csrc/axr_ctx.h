@@ -51,6 +51,10 @@ struct HostShadow {
   bool exists = false;
   bool placed = false;
   bool clear_pending = false;  // cleared since its last pass; map not yet written
+  // a pass ran on the map since the bin counters were last read back
+  bool unchecked = false;
+  // a pass on the map lost bin entries: incomplete until cleared
+  bool incomplete = false;
   int dim = 0;
   int nbx = 0, nb = 0;  // 64x64 texel bins
   float scale = 0.0f;
@@ -162,6 +166,9 @@ struct axr_ctx {
   int state = ST_UNCLEARED;
   AxrDrawConst last{};
   bool records_valid = false;  // setup records and bins belong to `last`
+  // the frame since the last clearZBuffer lost bin entries, or samples an
+  // incomplete shadow map: every frame readback fails until the next clear
+  bool frame_incomplete = false;
 
   bool timing = false;
   std::vector<TimingRec> trecs;

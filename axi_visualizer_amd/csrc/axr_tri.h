@@ -144,7 +144,7 @@ __device__ __forceinline__ void a_dec(Attr& a, const Attr& s) {
 #undef M
 }
 template <class T>
-__device__ __forceinline__ void swp(T& a, T& b) {
+AXR_HD void swp(T& a, T& b) {
   T t = a;
   a = b;
   b = t;
@@ -564,7 +564,7 @@ __device__ __forceinline__ void tile_walk(const Geom<float>& g, const TileBox& b
 // Pipe/coarse.c:26-236 over one large triangle.  emit(tile) per entry,
 // duplicates included.
 template <class F>
-__device__ __forceinline__ void coarse_walk(const float px[3], const float py[3], int WC, int HC,
+AXR_HD void coarse_walk(const float px[3], const float py[3], int WC, int HC,
                                             F&& emit) {
   float x1 = px[0] / 16.0f - 0.5f, y1 = py[0] / 16.0f - 0.5f;
   float x2 = px[1] / 16.0f - 0.5f, y2 = py[1] / 16.0f - 0.5f;
@@ -632,7 +632,7 @@ __device__ __forceinline__ void coarse_walk(const float px[3], const float py[3]
 // (float)tile <= xe, as coarse_walk emits them.  Together the threads emit
 // coarse_walk's entries.
 template <class R>
-__device__ __forceinline__ void coarse_rows(const float px[3], const float py[3], int WC, int HC,
+AXR_HD void coarse_rows(const float px[3], const float py[3], int WC, int HC,
                                             int lane, int nl, R&& row) {
   float x1 = px[0] / 16.0f - 0.5f, y1 = py[0] / 16.0f - 0.5f;
   float x2 = px[1] / 16.0f - 0.5f, y2 = py[1] / 16.0f - 0.5f;
@@ -697,6 +697,33 @@ __device__ __forceinline__ void coarse_rows(const float px[3], const float py[3]
       }
     }
   }
+}
+
+// Coarse binning of one large triangle by a block of nthreads threads
+// (nthreads a power of two): each tile row of coarse_rows goes to a group of
+// G threads, G the largest power of two with 2 x rows x G <= nthreads, and
+// thread `tid` takes the row's columns first + sub, first + sub + G, ...
+// (sub = tid % G).  batch(cy, ax, G, xe) receives up to eight of them at a
+// time: columns ax + j*G, j < 8, those with (float)(ax + j*G) <= xe.
+// A row's `first` can be far right of the grid: floorf of an edge position
+// past 2^31 saturates to INT_MAX, with xe <= WC - 1, where the serial walk
+// emits nothing.  first is clamped to WC before the stride is added, so
+// first + sub cannot wrap, and every emitted column lies in [0, WC - 1].
+template <class B>
+AXR_HD void coarse_block_tiles(const float px[3], const float py[3], int WC, int HC, int tid,
+                               int nthreads, B&& batch) {
+  // tile rows the walk may visit (both halves; clamped to the grid), an
+  // estimate only used to size the groups
+  const float ylo = omin(py[0], omin(py[1], py[2])) / 16.0f - 0.5f;
+  const float yhi = omax(py[0], omax(py[1], py[2])) / 16.0f - 0.5f;
+  const int r0 = imax(0, f2i(floorf(ylo))), r1 = imin(HC - 1, f2i(ceilf(yhi)));
+  const int rows = imax(1, r1 - r0 + 1);
+  int G = 1;
+  while (G < nthreads && rows * 2 * G <= nthreads) G *= 2;
+  const int sub = tid % G;
+  coarse_rows(px, py, WC, HC, tid / G, nthreads / G, [&](int cy, int first, float xe) {
+    for (int ax = imin(first, WC) + sub; (float)ax <= xe; ax += 8 * G) batch(cy, ax, G, xe);
+  });
 }
 
 // ------------------------------------------------ a11: shadow depth walk

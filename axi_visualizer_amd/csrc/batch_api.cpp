@@ -73,6 +73,9 @@ struct axr_batch {
   hipEvent_t ev_start = nullptr, ev_done = nullptr;
   hipEvent_t ev_copied = nullptr;  // the job table has left the staging buffer
   bool rendered = false;
+  // the last render lost bin entries: its frames are incomplete until the
+  // next render (every wait and readback of it fails with AXR_ECAP)
+  bool incomplete = false;
   // graph replay
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -289,6 +292,16 @@ int axr_batch_configure(axr_batch* b, int bin_size, uint32_t cap_small, uint32_t
   return AXR_OK;
 }
 
+int axr_batch_set_spill_capacity(axr_batch* b, uint32_t spill) {
+  if (!b) return fail(AXR_EINVAL, "null batch");
+  if (spill == 0) return fail(AXR_EINVAL, "spill capacity must be positive");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->bins.want_spill = spill;
+  free_graph(b);
+  return AXR_OK;
+}
+
 int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
   if (!b || (!jobs && njobs)) return fail(AXR_EINVAL, "batch render: bad arguments");
   if (njobs < 0) return fail(AXR_EINVAL, "negative job count");
@@ -386,6 +399,7 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
   std::memcpy(b->h_sg, grp_desc.data(), grp_desc.size() * sizeof(uint32_t));
   std::memcpy(b->h_sg + 4 * (size_t)njobs, grp_jobs.data(), grp_jobs.size() * sizeof(uint32_t));
   b->rendered = true;
+  b->incomplete = false;
   HIPCHK(hipEventRecord(b->ev_start, b->stream));
   if (njobs) {
     HIPCHK(hipMemcpyAsync(b->d_jobs, b->h_jobs, sizeof(AxrJobDev) * (size_t)njobs,
@@ -452,11 +466,16 @@ int axr_batch_wait(axr_batch* b, float* gpu_ms) {
     bins_grow(b->bins, h[AXR_CTR_BIN_SPILL], h[AXR_CTR_LOST]);
     HIPCHK(launch_fill_u32(b->d_counters, AXR_CTR_WORDS, 0, b->stream));
     std::memset(b->h_ctr, 0, sizeof h);
-    if (h[AXR_CTR_LOST])
+    if (h[AXR_CTR_LOST]) {
+      b->incomplete = true;
       return fail(AXR_ECAP, "%u bin entries did not fit the spill list; it has been grown, "
                             "render the batch again",
                   h[AXR_CTR_LOST]);
+    }
   }
+  if (b->incomplete)
+    return fail(AXR_ECAP, "the last render lost bin entries (the spill list has been grown): "
+                          "render the batch again");
   return AXR_OK;
 }
 

@@ -186,8 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
 
 // Coarse binning of the setup's large triangles (a8, Pipe/coarse.c:26-236):
 // a block per triangle of the list; each tile row (coarse_rows) goes to a
-// group of G threads, G the largest power of two with rows x G <= the
-// block, which share the row's tiles (column first + sub, + G, ...); the
+// group of threads which share the row's tiles (coarse_block_tiles); the
 // appends are issued eight at a time so their atomics overlap.  A wide flat
 // triangle (few rows, spans of a hundred tiles) thus keeps the block busy
 // instead of a handful of threads.  Every thread of a group walks the rows
@@ -205,37 +204,27 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse(const AxrTriRec* __re
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
     const uint32_t gi = large[i];
     const AxrTriRec r = rec[gi];
-    // tile rows the walk may visit (both halves; clamped to the grid), an
-    // estimate only used to size the groups
-    const float ylo = omin(r.py[0], omin(r.py[1], r.py[2])) / 16.0f - 0.5f;
-    const float yhi = omax(r.py[0], omax(r.py[1], r.py[2])) / 16.0f - 0.5f;
-    const int r0 = imax(0, f2i(floorf(ylo))), r1 = imin(HC - 1, f2i(ceilf(yhi)));
-    const int rows = imax(1, r1 - r0 + 1);
-    int G = 1;
-    while (G < kCoarseThreads && rows * 2 * G <= kCoarseThreads) G *= 2;
-    const int sub = (int)threadIdx.x % G;
-    coarse_rows(r.px, r.py, WC, HC, (int)threadIdx.x / G, kCoarseThreads / G,
-                [&](int cy, int first, float xe) {
-                  for (int ax = first + sub; (float)ax <= xe; ax += 8 * G) {
-                    uint32_t slot[8], pos[8];
-                    bool on[8];
+    coarse_block_tiles(r.px, r.py, WC, HC, (int)threadIdx.x, kCoarseThreads,
+                       [&](int cy, int ax, int G, float xe) {
+                         // columns in [0, WC - 1] and cy in [0, HC - 1]
+                         // (coarse_block_tiles, coarse_rows)
+                         uint32_t slot[8], pos[8];
+                         bool on[8];
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                      on[j] = (float)(ax + j * G) <= xe;
-                      const int t = WC * cy + ax + j * G;
-                      const int tx = t % WC, ty = t / WC;
-                      slot[j] = (uint32_t)MB.nb + (uint32_t)((ty >> 1) * NBX + (tx >> 1));
-                      pos[j] = on[j] ? atomicAdd(&MB.cnt[slot[j]], 1u) : 0u;
-                    }
+                         for (int j = 0; j < 8; j++) {
+                           const int tx = ax + j * G;
+                           on[j] = (float)tx <= xe;
+                           slot[j] = (uint32_t)MB.nb + (uint32_t)((cy >> 1) * NBX + (tx >> 1));
+                           pos[j] = on[j] ? atomicAdd(&MB.cnt[slot[j]], 1u) : 0u;
+                         }
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                      if (!on[j]) continue;
-                      const int tx = ax + j * G;
-                      const uint32_t sb = (uint32_t)(((cy & 1) << 1) | (tx & 1));
-                      bin_store(MB, slot[j], pos[j], (gi << 2) | sb);
-                    }
-                  }
-                });
+                         for (int j = 0; j < 8; j++) {
+                           if (!on[j]) continue;
+                           const int tx = ax + j * G;
+                           const uint32_t sb = (uint32_t)(((cy & 1) << 1) | (tx & 1));
+                           bin_store(MB, slot[j], pos[j], (gi << 2) | sb);
+                         }
+                       });
   }
 }
 

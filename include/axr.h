@@ -212,6 +212,10 @@ int axr_batch_destroy(axr_batch* b);
  * batch layout is unchanged. */
 int axr_batch_configure(axr_batch* b, int bin_size, uint32_t cap_small, uint32_t cap_large,
                         int use_graph);
+/* Entries of the batch's spill list (pairs of bin slot and entry taken when
+ * a bucket is full), from the next render on; it grows by itself when a
+ * render loses entries.  Mirrors axr_set_bin_capacity's spill argument. */
+int axr_batch_set_spill_capacity(axr_batch* b, uint32_t spill);
 /* Enqueues the jobs; returns without waiting.  output: AXR_OUT_RGBA8
  * (gamma'd colour + depth), AXR_OUT_IDS (winner ids + depth) or
  * AXR_OUT_HDR (u16 colour planes + depth). */
@@ -222,7 +226,9 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output);
 /* 1 when the last render has finished, 0 while it runs. */
 int axr_batch_query(axr_batch* b);
 /* Waits for the last render; GPU time of it in *gpu_ms (may be NULL).
- * AXR_ECAP when bin entries were lost (render again: storage has grown). */
+ * AXR_ECAP when bin entries were lost (render again: storage has grown);
+ * until the next axr_batch_render every wait and every readback of that
+ * render fails with AXR_ECAP, so an incomplete frame is never returned. */
 int axr_batch_wait(axr_batch* b, float* gpu_ms);
 int axr_batch_get_rgba8(axr_batch* b, int job, uint32_t* out);
 int axr_batch_get_depth(axr_batch* b, int job, float* out);

@@ -100,6 +100,11 @@ def test_lost_entries_are_reported_and_the_redraw_is_whole():
         r.frame()
         with pytest.raises(AxrError, match="spill list"):
             d.getRGBA8()
+        # the incomplete frame and maps stay unreadable until redrawn
+        with pytest.raises(AxrError, match="redraw"):
+            d.getIds()
+        with pytest.raises(AxrError, match="redraw"):
+            d.getFrame()
         r.frame()
         gpu = {"id": d.getIds()}
         rr, gg, bb, dep = d.getFrame()

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from axi_visualizer_amd import scenes
+from axi_visualizer_amd._native import AxrError
 from axi_visualizer_amd.batch import Batch, Job, SceneContext, autotune, job_camera
 from axi_visualizer_amd.render import Renderer
 from oracle import pyoracle
@@ -228,6 +229,37 @@ def test_spilling_batch_matches_roomy_batch(spheres_ctx):
                 assert np.array_equal(tiny.depth(j).view(np.uint32),
                                       roomy.depth(j).view(np.uint32)), f"render {rep} job {j}"
         assert held[1] > held[0], held  # the first render spilled: its buckets grew
+    finally:
+        tiny.close()
+        roomy.close()
+
+
+def test_lost_entries_fail_every_readback_until_the_next_render(spheres_ctx):
+    """Tiny buckets and a tiny spill list: the render loses bin entries.
+    The wait reports it (AXR_ECAP) and grows the list; every later wait and
+    readback of that render fails too; the next render is whole."""
+    jobs = make_jobs(spheres_ctx, SIZES, seed=8)
+    tiny, roomy = Batch(cap_small=4, cap_large=4), Batch()
+    try:
+        roomy.render(jobs)
+        roomy.wait()
+        tiny.set_spill_capacity(16)
+        tiny.render(jobs)
+        with pytest.raises(AxrError, match="spill list"):
+            tiny.wait()
+        with pytest.raises(AxrError, match="render the batch again"):
+            tiny.rgba8(0)
+        with pytest.raises(AxrError, match="render the batch again"):
+            tiny.wait()
+        for _ in range(8):  # the list doubles per loss
+            tiny.render(jobs)
+            try:
+                tiny.wait()
+                break
+            except AxrError as e:
+                assert "spill list" in str(e)
+        for j in range(len(jobs)):
+            assert np.array_equal(tiny.rgba8(j), roomy.rgba8(j)), f"job {j}"
     finally:
         tiny.close()
         roomy.close()

@@ -10,32 +10,12 @@ import numpy as np
 import pytest
 
 from axi_visualizer_amd import scenes
+from tests.edge_scenes import SLIVER_TRIS, hostile_scene, sliver_scene
 from tests.parity import check_scene
 
 pytestmark = pytest.mark.gpu
 
 NONE = 0xFFFFFFFF
-
-
-def hostile_scene(W=256, H=192):
-    """Random triangles whose first ten are replaced by hostile ones."""
-    sc = scenes.random_tris_scene(W=W, H=H, ntri=600, seed=9, name="hostile")
-    g = sc.groups[0]
-    xyz = g.xyz.reshape(-1, 3, 3).copy()
-    bad = []
-    t = xyz[0].copy(); t[1, 0] = np.nan; bad.append(t)            # NaN vertex
-    t = xyz[1].copy(); t[2, 2] = np.inf; bad.append(t)            # +inf depth
-    t = xyz[2].copy(); t[0, 1] = -np.inf; bad.append(t)           # -inf coordinate
-    t = xyz[3].copy(); t[2] = 0.5 * (t[0] + t[1]); bad.append(t)  # collinear
-    t = xyz[4].copy(); t[1] = t[0]; t[2] = t[0]; bad.append(t)    # one point
-    t = xyz[5].copy(); t[0, 2] = -3.0; bad.append(t)              # crosses the eye plane
-    t = xyz[6].copy(); t[:, 2] = -10.0; bad.append(t)             # behind the eye
-    t = xyz[7].copy() * 1e6; t[:, 2] = np.abs(t[:, 2]) + 1.0; bad.append(t)  # enormous
-    t = xyz[8].copy(); t[:, 2] = 1e7; bad.append(t)               # far away
-    t = xyz[9].copy(); c = t.mean(0); t = c + (t - c) * 1e-4; bad.append(t)  # sub-pixel
-    xyz[:len(bad)] = np.array(bad, np.float32)
-    groups = [dataclasses.replace(g, xyz=xyz.reshape(-1, 3).astype(np.float32))] + sc.groups[1:]
-    return dataclasses.replace(sc, groups=groups), len(bad)
 
 
 @pytest.mark.parametrize("W,H", [(208, 143), (48, 17), (16, 1)])
@@ -63,6 +43,21 @@ def test_hostile_geometry_with_shadows():
                              groups=[dataclasses.replace(g, flags=g.flags | scenes.SH_SHADOW)
                                      for g in sc.groups], name="hostile_shadows")
     check_scene(sc)
+
+
+def test_coarse_slivers():
+    """Large triangles two screen-y steps tall and 14,000 px wide in an
+    8192 x 48 frame: their coarse rows include ones whose first tile column
+    saturates to INT_MAX (checked on the CPU by test_coarse_host); the
+    coarse launch must bin exactly the oracle's tiles and never outside the
+    grid."""
+    sc = sliver_scene()
+    rep = check_scene(sc)
+    from tests.parity import render_gpu
+    ids = render_gpu(sc)["id"]
+    for k in range(len(SLIVER_TRIS)):
+        print(f"[parity] sliver {k}: {int((ids == k).sum())} px", flush=True)
+    assert rep["covered"] > 0.1
 
 
 def test_nothing_in_view():
