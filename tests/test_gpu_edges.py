@@ -57,7 +57,7 @@ def test_coarse_slivers():
     ids = render_gpu(sc)["id"]
     for k in range(len(SLIVER_TRIS)):
         print(f"[parity] sliver {k}: {int((ids == k).sum())} px", flush=True)
-    assert rep["covered"] > 0.1
+    assert rep["covered"] > 0.02  # the fillers; the slivers cover no pixel centre
 
 
 def test_nothing_in_view():
