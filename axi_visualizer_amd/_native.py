@@ -71,6 +71,7 @@ SIGNATURES = {
     "axr_set_timing": (_i, [_ctx, _i]),
     "axr_set_shaders": (_i, [_ctx, _p, _p, _i]),
     "axr_get_texels": (_i, [_ctx, _i, _p, _p, _p]),
+    "axr_get_vertex_light": (_i, [_ctx, _i, _p]),
     "axr_get_bytes": (_i, [_ctx, ctypes.POINTER(ctypes.c_uint64)]),
     "axr_batch_create": (_i, [ctypes.POINTER(_batch)]),
     "axr_batch_destroy": (_i, [_batch]),

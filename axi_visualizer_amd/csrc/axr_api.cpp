@@ -1156,6 +1156,27 @@ int axr_get_texels(axr_ctx* c, int tn, uint16_t* r, uint16_t* g, uint16_t* b) {
   return AXR_OK;
 }
 
+int axr_get_vertex_light(axr_ctx* c, int tn, float* out) {
+  int rc;
+  if ((rc = check_ctx(c))) return rc;
+  if (tn < 0 || tn >= (int)c->groups.size() || !out)
+    return fail(AXR_EINVAL, "getVertLight: bad arguments");
+  const HostGroup& hg = c->groups[(size_t)tn];
+  const size_t nv = 3 * (size_t)hg.ntri;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // vattr[2v + 1] = (v, Ix, Iy, Iz)
+  std::vector<float4> h(2 * nv);
+  if (nv)
+    HIPCHK(hipMemcpy(h.data(), c->d_vattr + 2 * 3 * (size_t)hg.base, h.size() * sizeof(float4),
+                     hipMemcpyDeviceToHost));
+  for (size_t v = 0; v < nv; v++) {
+    out[3 * v] = h[2 * v + 1].y;
+    out[3 * v + 1] = h[2 * v + 1].z;
+    out[3 * v + 2] = h[2 * v + 1].w;
+  }
+  return AXR_OK;
+}
+
 int axr_set_shaders(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask,
                     int ngroups) {
   int rc;

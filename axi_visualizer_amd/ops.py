@@ -80,6 +80,10 @@ class CLDraw:
         self._texels = []
         self.SHADOWMAP = {}
         self.ambLight = np.float32(0.2)
+        # point and spot lights of the last vertLight that set them (a call
+        # that passes None keeps them, as Ops.py:564-600 keeps its buffers)
+        self._points = (0, [None, None])
+        self._spots = (0, [None, None, None])
         self._flags_buf = None
         self._mask_buf = None
         self._sid_buf = np.zeros(2, np.int32)
@@ -160,20 +164,27 @@ class CLDraw:
     def vertLight(self, mask, dirI, dirD, pointI=None, pointP=None,
                   spotI=None, spotD=None, spotP=None):
         """Ops.py:552-612 -> Pipe/vert.c vertL.  mask[tn] True lights group
-        tn.  pointI/spotI may be None or 1 (no lights), as the reference
-        accepts."""
+        tn.  pointI (spotI) = 1 removes the point (spot) lights; None keeps
+        those of the previous call (Ops.py:564-600: the light buffers
+        persist, and Compute.py:429 relies on it); an array sets them."""
         m = np.ascontiguousarray(np.asarray(mask, dtype=bool).astype(np.uint8))
         di = _f32(dirI, (-1, 3))
         dd = _f32(dirD, (-1, 3))
 
-        def lights(i, *rest):
-            if i is None or (isinstance(i, int) and i == 1):
+        def lights(kept, i, *rest):
+            if i is None:
+                return kept
+            if isinstance(i, int) and i == 1:
                 return 0, [None] * (1 + len(rest))
             a = [_f32(i, (-1, 3))] + [_f32(x, (-1, 3)) for x in rest]
+            if any(x.shape != a[0].shape for x in a):
+                raise ValueError("light arrays must have the same number of lights")
             return a[0].shape[0], a
 
-        npt, (pI, pP) = lights(pointI, pointP)
-        nsp, (sI, sD, sP) = lights(spotI, spotD, spotP)
+        self._points = lights(self._points, pointI, pointP)
+        self._spots = lights(self._spots, spotI, spotD, spotP)
+        npt, (pI, pP) = self._points
+        nsp, (sI, sD, sP) = self._spots
         self._chk(self._L.axr_vert_light(
             self._ctx, ptr(m), m.shape[0], ptr(di), ptr(dd), di.shape[0],
             ptr(pI), ptr(pP), npt, ptr(sI), ptr(sD), ptr(sP), nsp,
@@ -371,6 +382,13 @@ class CLDraw:
         self._chk(self._L.axr_get_texels(self._ctx, int(tn), *[ptr(o) for o in out]),
                   "getTexels")
         return tuple(out)
+
+    def getVertLight(self, tn):
+        """Group tn's per-vertex light I (the LI buffer vertL writes,
+        Ops.py:605-611) as stored on the device: (3T, 3) float32."""
+        out = np.empty((int(self.gSize[tn]), 3), np.float32)
+        self._chk(self._L.axr_get_vertex_light(self._ctx, int(tn), ptr(out)), "getVertLight")
+        return out
 
     def deviceBytes(self) -> int:
         n = ctypes.c_uint64()

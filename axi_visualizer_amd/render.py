@@ -27,8 +27,12 @@ def load_scene(d: CLDraw, sc: Scene):
         d.setGroupMaterial(tn, g.mip_bias, g.spec_pow)
     d.ambLight = np.float32(sc.amb)
     d.setPrimaryLight(sc.light_int[None], sc.light_dir[None])
-    # one directional light: vertL leaves I at the ambient floor
-    d.vertLight([True] * len(sc.groups), sc.light_int[None], sc.light_dir[None])
+    # light 0 is shaded per pixel; the others light the vertices (with the
+    # primary light alone vertL leaves I at the ambient floor)
+    L = sc.vert_lights()
+    d.vertLight([True] * len(sc.groups), L["dirI"], L["dirD"],
+                1 if L["pointI"] is None else L["pointI"], L["pointP"],
+                1 if L["spotI"] is None else L["spotI"], L["spotD"], L["spotP"])
     for i, c in enumerate(sc.cascades):
         d.addShadowMap(i, c.dim, c.cam.scale, sc.amb)
         d.placeShadowMap(i, c.cam.pos, c.cam.vm, sc.amb)

@@ -169,10 +169,32 @@ class Scene:
     ssr_step: int = 1        # REFL_STEP
     ssr_absorb: float = 0.3  # ABSORB
     ssr_fresnel: bool = True
+    # vertex-lit lights of vertL (Pipe/vert.c:38-88) beside the primary
+    # light, which is light 0 and shaded per pixel: directional lights 1..
+    # as (I, D), point lights as (I, P), spot lights as (I, D, P), each
+    # array (k, 3) float32
+    extra_dirs: Optional[tuple] = None
+    points: Optional[tuple] = None
+    spots: Optional[tuple] = None
 
     @property
     def ntri(self) -> int:
         return sum(g.ntri for g in self.groups)
+
+    def vert_lights(self) -> dict:
+        """vertLight's arguments (Ops.py:552): dirI/dirD with the primary
+        light first, then the point and spot lights (None when absent)."""
+        dI = [np.asarray(self.light_int, np.float32).reshape(1, 3)]
+        dD = [np.asarray(self.light_dir, np.float32).reshape(1, 3)]
+        if self.extra_dirs is not None:
+            dI.append(np.asarray(self.extra_dirs[0], np.float32).reshape(-1, 3))
+            dD.append(np.asarray(self.extra_dirs[1], np.float32).reshape(-1, 3))
+        out = {"dirI": np.concatenate(dI), "dirD": np.concatenate(dD)}
+        f = lambda a: np.asarray(a, np.float32).reshape(-1, 3)  # noqa: E731
+        out["pointI"], out["pointP"] = (None, None) if self.points is None else map(f, self.points)
+        out["spotI"], out["spotD"], out["spotP"] = ((None, None, None) if self.spots is None
+                                                    else map(f, self.spots))
+        return out
 
 
 def make_camera(W: int, H: int, pos, a: float, b: float, fovx: float = 80.0) -> Camera:

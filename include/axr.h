@@ -176,6 +176,10 @@ int axr_get_timing(axr_ctx* ctx, char* names, int name_len, double* total_ms, in
  * texture or of its whole smallest-first mip chain). */
 int axr_get_texels(axr_ctx* ctx, int tn, uint16_t* r, uint16_t* g, uint16_t* b);
 
+/* Per-vertex light of group tn (the LI buffer Pipe/vert.c:vertL writes,
+ * Ops.py:605-611): out[3T][3] f32 as stored on the device. */
+int axr_get_vertex_light(axr_ctx* ctx, int tn, float* out);
+
 /* Shader binding of drawAll (flags per group, mask = groups not drawn)
  * without drawing: prepares a context used as a resident scene by batches. */
 int axr_set_shaders(axr_ctx* ctx, const uint32_t* shader_flags, const uint8_t* mask,

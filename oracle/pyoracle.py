@@ -125,6 +125,22 @@ def vertlight(xyz, vn, dirI, dirD, amb, pointI=None, pointP=None, spotI=None, sp
     return out
 
 
+def scene_vertlight(scene, xyz, vn):
+    """vertL over a group's vertices with the scene's lights: the primary
+    light first (skipped by vert.c), then its extra directional, point and
+    spot lights when the scene has them."""
+    dI = [np.asarray(scene.light_int, np.float32).reshape(1, 3)]
+    dD = [np.asarray(scene.light_dir, np.float32).reshape(1, 3)]
+    extra = getattr(scene, "extra_dirs", None)
+    if extra is not None:
+        dI.append(np.asarray(extra[0], np.float32).reshape(-1, 3))
+        dD.append(np.asarray(extra[1], np.float32).reshape(-1, 3))
+    pts = getattr(scene, "points", None) or (None, None)
+    sps = getattr(scene, "spots", None) or (None, None, None)
+    return vertlight(xyz, vn, np.concatenate(dI), np.concatenate(dD), scene.amb,
+                     pointI=pts[0], pointP=pts[1], spotI=sps[0], spotD=sps[1], spotP=sps[2])
+
+
 def render(scene, nthreads=0, with_shadows_out=False):
     """Renders a scenes.Scene: shadow cascades, clear, drawAll, gamma.
     Returns dict(depth, id, r, g, b, stats[, shadow])."""
@@ -134,7 +150,7 @@ def render(scene, nthreads=0, with_shadows_out=False):
         xyz = _c(g.xyz, np.float32)
         vn = _c(g.vn, np.float32)
         uv = _c(g.uv, np.float32)
-        li = vertlight(xyz, vn, scene.light_int[None], scene.light_dir[None], scene.amb)
+        li = scene_vertlight(scene, xyz, vn)
         tr = _c(g.tex[:, 0], np.uint16)
         tg = _c(g.tex[:, 1], np.uint16)
         tb = _c(g.tex[:, 2], np.uint16)

@@ -293,6 +293,87 @@ def test_vertlight_closed_forms():
     np.testing.assert_array_equal(two[1], np.full(3, np.float32(0.2)))
 
 
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+def test_vertlight_point_light_closed_form():
+    """vert.c:61-66: a point light at P adds n.normalize(pl)/(1 + |pl|^2)*I,
+    pl = x - P, when n.pl > 0 (the light behind the surface as the
+    reference orients it), nothing otherwise."""
+    n = _unit([0.0, 0.6, 0.8])
+    P = np.array([1.0, -2.0, 0.5])
+    # vertices at offsets pl from the light: |pl| = 2, 5, 0.5, and two
+    # with n.pl <= 0
+    pls = np.array([[0, 0, 2], [3, 0, 4], [0.3, 0.4, 0], [0, -0.8, -0.6], [0, 0.8, -0.6]],
+                   np.float64)
+    xyz = (P + pls).astype(np.float32)
+    vn = np.tile(n, (len(pls), 1)).astype(np.float32)
+    I = np.array([[2.0, 1.0, 0.5]])
+    # an ambient floor far below zero: the gate, not the floor, must zero
+    # the lights behind the vertices
+    out = po.vertlight(xyz, vn, [[9, 9, 9]], [[0, 1, 0]], -10.0, pointI=I, pointP=[P])
+    pl = xyz.astype(np.float64) - P
+    L = np.linalg.norm(pl, axis=1)
+    s = np.where(pl @ n > 0, (pl @ n) / L / (1.0 + L * L), 0.0)
+    np.testing.assert_allclose(out, s[:, None] * I, rtol=2e-6, atol=1e-7)
+    assert np.all(out[3:] == 0.0)  # n.pl <= 0: no contribution
+    # the hand values of the first two: 0.8/5 and (0.6*0 + 0.8*0.8)/26
+    np.testing.assert_allclose(out[0, 0], 2.0 * 0.8 / 5.0, rtol=2e-6)
+    np.testing.assert_allclose(out[1, 0], 2.0 * (0.64 / 26.0), rtol=2e-6)
+
+
+def test_vertlight_spot_light_closed_form():
+    """vert.c:68-73: a spot light adds n.npl * npl.SDir / (1.5 + |pl|^2) * I
+    when SDir.pl > 0 and n.pl > 0 (npl = normalize(pl))."""
+    n = _unit([0.0, 0.0, 1.0])
+    P = np.array([0.0, 0.0, 0.0])
+    sd = _unit([0.0, 0.6, 0.8]).astype(np.float32)
+    pls = np.array([[3, 0, 4], [0, 0, 2], [0, 3, 4], [0, -4, 3], [1, 1, -1], [0, -4, 1]],
+                   np.float64)
+    xyz = (P + pls).astype(np.float32)
+    vn = np.tile(n, (len(pls), 1)).astype(np.float32)
+    I = np.array([[1.0, 0.5, 0.25]])
+    out = po.vertlight(xyz, vn, [[9, 9, 9]], [[0, 1, 0]], -10.0, spotI=I, spotD=[sd],
+                       spotP=[P])
+    pl = xyz.astype(np.float64) - P
+    L = np.linalg.norm(pl, axis=1)
+    npl = pl / L[:, None]
+    on = (pl @ sd.astype(np.float64) > 0) & (pl @ n > 0)
+    s = np.where(on, (npl @ n) * (npl @ sd.astype(np.float64)) / (1.5 + L * L), 0.0)
+    np.testing.assert_allclose(out, s[:, None] * I, rtol=2e-6, atol=1e-7)
+    # by hand: pl = (3,0,4): n.npl = 0.8, npl.SDir = 0.64 -> 0.512/26.5
+    np.testing.assert_allclose(out[0, 0], 0.8 * 0.64 / 26.5, rtol=2e-6)
+    # pl = (0,-4,3): SDir.pl = 0, and (0,-4,1): SDir.pl < 0 -> gated off
+    # although n.pl > 0; (1,1,-1): n.pl < 0
+    assert np.all(out[3:] == 0.0)
+
+
+def test_vertlight_lights_sum_and_floor_at_ambient():
+    """Directional lights 1.. (light 0 ignored), points and spots add per
+    channel; the sum is floored per channel at the ambient level."""
+    n = _unit([0.0, 1.0, 0.0])
+    xyz = np.array([[0.0, 2.0, 0.0]], np.float32)
+    vn = n[None].astype(np.float32)
+    dI = np.array([[50, 50, 50], [0.4, 0.0, 0.1], [0.1, 0.2, 0.0]])
+    dD = np.array([[0, 1, 0], [0, 1, 0], [0, -1, 0]], np.float32)
+    pI, pP = np.array([[0.0, 1.0, 0.0]]), np.array([[0.0, 0.0, 0.0]])
+    sI, sD, sP = (np.array([[0.0, 0.0, 1.0]]), np.array([[0.0, 1.0, 0.0]], np.float32),
+                  np.array([[0.0, 1.0, 0.0]]))
+    out = po.vertlight(xyz, vn, dI, dD, 0.15, pointI=pI, pointP=pP, spotI=sI, spotD=sD,
+                       spotP=sP)[0]
+    d1 = max(0.0, 1.0 + 0.5) * 0.66 * dI[1]        # (0.396, 0, 0.099)
+    d2 = max(0.0, -1.0 + 0.5) * 0.66 * dI[2]       # 0
+    pt = 1.0 / (1.0 + 4.0) * pI[0]                 # |pl| = 2
+    sp = 1.0 * 1.0 / (1.5 + 1.0) * sI[0]           # |pl| = 1
+    want = np.maximum(d1 + d2 + pt + sp, 0.15)
+    np.testing.assert_allclose(out, want, rtol=2e-6)
+    # the green channel is the point light's 0.2 alone, the blue channel
+    # the directional 0.099 plus the spot's 0.4, red the directional alone
+    np.testing.assert_allclose(out, [0.396, 0.2, 0.499], rtol=2e-6)
+
+
 # ----------------------------------------------- depth-winner semantics
 def _quad_scene(z_a, z_b, W=64, H=64):
     """Two screen-filling triangles at depths z_a (group 0) and z_b (group 1)."""
