@@ -82,6 +82,26 @@ def job_camera(width: int, height: int, pos, target, fovx: float = 80.0) -> Came
     return make_camera(width, height, pos, a, b, fovx)
 
 
+MULTIVIEW_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
+
+
+def multiview_jobs(ctx: SceneContext, n_jobs: int = 64, seed: int = 1,
+                   sizes=MULTIVIEW_SIZES) -> List[Job]:
+    """The batched multi-view workload's views of a spheres scene: seeded
+    camera poses around and inside the 200^3 box, each looking at a point
+    near the centre, resolutions drawn uniformly from `sizes`."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n_jobs):
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        ang = rng.uniform(0, 2 * np.pi)
+        dist = rng.uniform(60.0, 330.0)
+        pos = np.array([dist * np.cos(ang), rng.uniform(-60.0, 120.0), dist * np.sin(ang)])
+        target = rng.uniform(-60.0, 60.0, 3)
+        jobs.append(Job(ctx, job_camera(w, h, pos, target), w, h))
+    return jobs
+
+
 class Batch:
     """One batch slot: its own stream and buffers (several may be in flight)."""
 

@@ -29,6 +29,9 @@ from .scenes import Camera, make_camera
 # Default bucket capacities of a 32-px bin (entries) used for admission estimates.
 _CAP_SMALL, _CAP_LARGE = 4096, 1024
 
+# Resolutions of the serving workload's requests: 480p, 720p, 1080p.
+SERVING_SIZES = [(848, 480), (1280, 720), (1920, 1080)]
+
 
 @dataclasses.dataclass
 class Request:

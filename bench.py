@@ -121,38 +121,19 @@ def build_scene(name):
     raise SystemExit(f"unknown config {name}")
 
 
-C4_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
-
-
-def c4_jobs(ctx, n_jobs=64, seed=1):
-    """64 views of the scene: seeded camera poses around and inside the
-    200^3 box, resolutions drawn uniformly from C4_SIZES."""
-    from axi_visualizer_amd.batch import Job, job_camera
-
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for _ in range(n_jobs):
-        w, h = C4_SIZES[int(rng.integers(len(C4_SIZES)))]
-        ang = rng.uniform(0, 2 * np.pi)
-        dist = rng.uniform(60.0, 330.0)
-        pos = np.array([dist * np.cos(ang), rng.uniform(-60.0, 120.0), dist * np.sin(ang)])
-        target = rng.uniform(-60.0, 60.0, 3)
-        jobs.append(Job(ctx, job_camera(w, h, pos, target), w, h))
-    return jobs
-
-
 def bench_c4(args):
     """Batched multi-view: 64 jobs of a 500k-triangle scene per batch,
     autotuned bin size and capacity, graph-replayed; against the same 64
     views drawn one at a time through CLDraw (the plain path)."""
     from axi_visualizer_amd import scenes
-    from axi_visualizer_amd.batch import Batch, SceneContext, autotune, settle
+    from axi_visualizer_amd.batch import (MULTIVIEW_SIZES, Batch, SceneContext, autotune,
+                                          multiview_jobs, settle)
     from axi_visualizer_amd.ops import CLDraw
     from axi_visualizer_amd.render import load_scene
 
     sc = scenes.spheres_scene(W=320, H=240, n_inst=1000, shadows=False, name="c4_spheres")
     ctx = SceneContext(sc)
-    jobs = c4_jobs(ctx)
+    jobs = multiview_jobs(ctx)
     tune = autotune(jobs, sizes=(8, 16, 32), cap_scales=(1, 4), reps=3, graph=True)
     best = tune["best"]
     print(f"[bench] c4 autotune: {tune}", file=sys.stderr, flush=True)
@@ -176,7 +157,7 @@ def bench_c4(args):
         lat.append((time.perf_counter() - t1) * 1e3)
     # plain path: the same views one at a time, one CLDraw per resolution
     draws = {}
-    for w, h in C4_SIZES:
+    for w, h in MULTIVIEW_SIZES:
         d = CLDraw(1, len(sc.groups), w, h, 1)
         load_scene(d, sc)
         draws[w, h] = d
@@ -235,9 +216,6 @@ def bench_c4(args):
     print(json.dumps(line))
 
 
-C5_SIZES = [(848, 480), (1280, 720), (1920, 1080)]
-
-
 def bench_c5(args):
     """Serving: 16 resident scenes (50k-1M triangles, terrain and spheres,
     1-4 textures, two shadow cascades each), Poisson arrivals at 2,000 req/s,
@@ -247,6 +225,7 @@ def bench_c5(args):
     and the same requests rendered one at a time for comparison."""
     from axi_visualizer_amd import scenes
     from axi_visualizer_amd.batch import Batch, Job
+    from axi_visualizer_amd.server import SERVING_SIZES as C5_SIZES
     from axi_visualizer_amd.server import RenderServer, poisson_requests
 
     mix = scenes.serving_scenes()

@@ -11,7 +11,7 @@ from axi_visualizer_amd._native import AxrError
 from axi_visualizer_amd.batch import Batch, Job, SceneContext, autotune, job_camera
 from axi_visualizer_amd.render import Renderer
 from oracle import pyoracle
-from tests.parity import RGB_MIN_FRAC, RGB_TOL
+from tests.parity import RGB_MIN_FRAC, RGB_TOL, job_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -31,16 +31,7 @@ def scene_view(base, w, h):
     return scenes.make_camera(w, h, base.view.pos, np.pi / 4, np.pi / 6)
 
 
-def view_scene(base, job):
-    """The job's view as a plain Scene (for CLDraw and the oracle)."""
-    groups = base.groups
-    cascades = base.cascades
-    if not job.shadows:
-        groups = [dataclasses.replace(g, flags=g.flags & ~scenes.SH_SHADOW) for g in groups]
-        cascades = []
-    return dataclasses.replace(base, W=job.width, H=job.height, view=job.camera, groups=groups,
-                               cascades=cascades, pcf_taps=job.pcf_taps,
-                               exposure=job.exposure, name=f"view_{job.width}x{job.height}")
+view_scene = job_scene
 
 
 def single(view):
