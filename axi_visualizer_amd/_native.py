@@ -77,6 +77,8 @@ SIGNATURES = {
     "axr_batch_destroy": (_i, [_batch]),
     "axr_batch_configure": (_i, [_batch, _i, _u32, _u32, _i]),
     "axr_batch_set_spill_capacity": (_i, [_batch, _u32]),
+    "axr_batch_set_timing": (_i, [_batch, _i]),
+    "axr_batch_get_timing": (_i, [_batch, _p, _p]),
     "axr_batch_render": (_i, [_batch, ctypes.POINTER(AxrJob), _i, _i]),
     "axr_batch_query": (_i, [_batch]),
     "axr_batch_wait": (_i, [_batch, ctypes.POINTER(ctypes.c_float)]),

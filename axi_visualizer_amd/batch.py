@@ -82,6 +82,22 @@ def job_camera(width: int, height: int, pos, target, fovx: float = 80.0) -> Came
     return make_camera(width, height, pos, a, b, fovx)
 
 
+def job_view_scene(job: Job) -> Scene:
+    """A job's view of its scene as a plain Scene (what CLDraw draws for
+    it): the job's camera, resolution, shadow switch, PCF and exposure."""
+    from .scenes import SH_SHADOW
+
+    base = job.scene.scene
+    groups = base.groups
+    cascades = base.cascades
+    if not job.shadows:
+        groups = [dataclasses.replace(g, flags=g.flags & ~SH_SHADOW) for g in groups]
+        cascades = []
+    return dataclasses.replace(base, W=job.width, H=job.height, view=job.camera, groups=groups,
+                               cascades=cascades, pcf_taps=job.pcf_taps,
+                               exposure=job.exposure, name=f"view_{job.width}x{job.height}")
+
+
 MULTIVIEW_SIZES = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
 
 
@@ -120,6 +136,20 @@ class Batch:
                                           1 if graph else 0), "axr_batch_configure")
         self.bin_size = bin_size
         self.cap_small, self.cap_large, self.graph = cap_small, cap_large, graph
+
+    def set_timing(self, enable: bool = True):
+        """Per-kernel HIP-event timing of the following renders (which then
+        run without graph replay)."""
+        check(self._L.axr_batch_set_timing(self._b, 1 if enable else 0), "axr_batch_set_timing")
+
+    def kernel_timing(self) -> dict:
+        """setup / order / raster milliseconds of the last (timed) render and
+        the bin entries its setup appended."""
+        ms = (ctypes.c_float * 3)()
+        n = ctypes.c_uint32()
+        check(self._L.axr_batch_get_timing(self._b, ms, ctypes.byref(n)), "axr_batch_get_timing")
+        return {"setup_ms": ms[0], "order_ms": ms[1], "raster_ms": ms[2],
+                "bin_entries": int(n.value)}
 
     def set_spill_capacity(self, spill: int):
         """Entries of the spill list from the next render on (it grows by

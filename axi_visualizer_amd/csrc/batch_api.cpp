@@ -76,6 +76,11 @@ struct axr_batch {
   // the last render lost bin entries: its frames are incomplete until the
   // next render (every wait and readback of it fails with AXR_ECAP)
   bool incomplete = false;
+  // per-kernel timing (axr_batch_set_timing): events around setup, order
+  // and raster of the last render, which then runs without its graph
+  bool timing = false;
+  hipEvent_t ev_k[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool timed = false;  // the last render recorded ev_k
   // graph replay
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -199,10 +204,15 @@ int enqueue(axr_batch* b) {
   if (!occ) occ = batch_raster_blocks_per_cu(b->bin_shift, b->mode);
   const int grid_max = b->cu_count * occ;
   const int grid = b->nb_total < grid_max ? b->nb_total : grid_max;
+  if (b->timing) HIPCHK(hipEventRecord(b->ev_k[0], b->stream));
   HIPCHK(launch_batch_setup(b->bin_shift, b->d_jobs, (const uint4*)b->d_sg, b->nsg,
                             b->d_sg + 4 * (size_t)b->njobs, b->setup_blocks, B, b->stream));
+  if (b->timing) HIPCHK(hipEventRecord(b->ev_k[1], b->stream));
   HIPCHK(launch_bin_order(&B, nullptr, b->d_counters, nullptr, nullptr, b->stream));
+  if (b->timing) HIPCHK(hipEventRecord(b->ev_k[2], b->stream));
   HIPCHK(launch_batch_raster(b->bin_shift, b->mode, b->d_jobs, b->njobs, B, grid, b->stream));
+  if (b->timing) HIPCHK(hipEventRecord(b->ev_k[3], b->stream));
+  b->timed = b->timing;
   return 0;
 }
 
@@ -265,6 +275,8 @@ int axr_batch_destroy(axr_batch* b) {
   if (b->ev_start) (void)hipEventDestroy(b->ev_start);
   if (b->ev_done) (void)hipEventDestroy(b->ev_done);
   if (b->ev_copied) (void)hipEventDestroy(b->ev_copied);
+  for (hipEvent_t e : b->ev_k)
+    if (e) (void)hipEventDestroy(e);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return AXR_OK;
@@ -289,6 +301,25 @@ int axr_batch_configure(axr_batch* b, int bin_size, uint32_t cap_small, uint32_t
   b->bins.want1 = cap_large ? cap_large : default_cap(shift, true);
   b->use_graph = use_graph != 0;
   free_graph(b);
+  return AXR_OK;
+}
+
+int axr_batch_set_timing(axr_batch* b, int enable) {
+  if (!b) return fail(AXR_EINVAL, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  for (hipEvent_t& e : b->ev_k)
+    if (!e) HIPCHK(hipEventCreate(&e));
+  b->timing = enable != 0;
+  return AXR_OK;
+}
+
+int axr_batch_get_timing(axr_batch* b, float ms[3], uint32_t* bin_entries) {
+  if (!b || !ms) return fail(AXR_EINVAL, "batch timing: bad arguments");
+  int rc;
+  if ((rc = axr_batch_wait(b, nullptr))) return rc;
+  if (!b->timed) return fail(AXR_ESTATE, "the last render was not timed (axr_batch_set_timing)");
+  for (int k = 0; k < 3; k++) HIPCHK(hipEventElapsedTime(&ms[k], b->ev_k[k], b->ev_k[k + 1]));
+  if (bin_entries) *bin_entries = b->h_ctr[AXR_CTR_BIN_TOTAL];
   return AXR_OK;
 }
 
@@ -407,7 +438,8 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
     HIPCHK(hipMemcpyAsync(b->d_sg, b->h_sg, sizeof(uint32_t) * 5 * (size_t)njobs,
                           hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipEventRecord(b->ev_copied, b->stream));
-    if (b->use_graph) {
+    b->timed = false;
+    if (b->use_graph && !b->timing) {
       // The graph holds the three launches; its arguments depend on this key.
       std::vector<uint64_t> key = {(uint64_t)njobs, (uint64_t)b->nsg, blocks, slots,
                                    (uint64_t)b->bin_shift,

@@ -452,6 +452,21 @@ def spheres_scene(W=1920, H=1080, n_inst=2000, n_tex=8, tex_side=1024, seed=1,
                  cascades=cascades, pcf_taps=pcf_taps)
 
 
+def spheres_closeup_camera(W=1920, H=1080, n_inst=2000, seed=1, gap=4.0, yaw=0.3, pitch=0.25,
+                           box=200.0, smin=0.5, smax=3.0) -> Camera:
+    """A camera inside the spheres scene's box: `gap` units in front of
+    the largest instance within 60 units of the centre, looking at it along
+    (yaw, pitch), with the rest of the box behind it.  Its near triangles
+    span hundreds of pixels (the tiled path), the far ones a few."""
+    scale = smin + (smax - smin) * rand_uniform(seed, 2003, n_inst)
+    pos = (rand_uniform(seed, 2004, n_inst * 3).reshape(n_inst, 3) - 0.5) * np.asarray(box)
+    near = np.linalg.norm(pos, axis=1) < 60.0
+    k = int(np.argmax(np.where(near, scale, -1.0)))
+    fwd = view_vec(yaw, pitch)
+    eye = pos[k] - (scale[k] * 1.05 + gap) * fwd
+    return make_camera(W, H, eye, yaw, pitch)
+
+
 # -------------------------------------------------------------- water
 def water_plane(x0: float, x1: float, z0: float, z1: float, y: float, n: int = 8,
                 uv_repeat: float = 4.0):

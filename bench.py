@@ -1,21 +1,33 @@
 """Benchmark of the MI355X rendering hot path.
 
-Default workload (the plain path at scale): 1920x1080, 1,000,000 triangles
-(2,000 seeded 500-triangle icospheres in a 200^3 box), 8 mipmapped 1024^2
-textures, two 2048^2 shadow cascades with 3x3 PCF, trilinear + Blinn-Phong.
-A step is one full frame: clearShadowMap + shadowMap per cascade,
-clearZBuffer, drawAll, gamma -- the drawAll sequence of the reference's
-engine (Compute.py:308-339) with its shadow pass (Ops.py:1089-1141).
-Scene data is resident in HBM before timing starts; readback is excluded
-and timed separately.
+Default run (one JSON line):
+* the plain path at scale (the headline `value`): 1920x1080, 1,000,000
+  triangles (2,000 seeded 500-triangle icospheres in a 200^3 box), 8
+  mipmapped 1024^2 textures, two 2048^2 shadow cascades with 3x3 PCF,
+  trilinear + Blinn-Phong.  A step is one full frame: clearShadowMap +
+  shadowMap per cascade, clearZBuffer, drawAll, gamma -- the drawAll
+  sequence of the reference's engine (Compute.py:308-339) with its shadow
+  pass (Ops.py:1089-1141).  Scene data is resident in HBM before timing
+  starts; readback is excluded and timed separately.  A second camera pose
+  close to a sphere inside the box (large triangles: the tiled path) is
+  timed beside it as extra fields;
+* `batched_multiview`: 64 views of a 500k-triangle scene at 320x240 to
+  1920x1080 per batch (setup, order, persistent raster; autotuned bins,
+  graph replay) against the same views through CLDraw back to back;
+* `serving`: 16 resident scenes, Poisson requests at 2,000 req/s and a
+  saturated burst, against the same requests through CLDraw back to back
+  and one at a time.
+Each carries its frames/s, p50/p99, the roofline of its dominant kernel
+(HIP events on the stream the kernel runs on) and the oracle's frames/s on
+a bounded sample of the same workload on the host CPU.
 
-Prints one JSON line with frames/s, Mtris/s, Mpix/s, per-frame p50/p99
-latency, the roofline of the dominant kernel (HIP events on the render
-stream) and the oracle's CPU frames/s on a bounded sample.
+`--config c1|c2|c3|c4|c5` prints one workload's line alone (c2 without the
+sub-objects).
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -27,19 +39,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# threads the CPU baseline may use: the host cores a GPU is allotted
+CPU_SHARE_MAX = 16
 
 # HBM bytes per launch of the frame kernels, measured with two rocprofv3
 # counter runs (FETCH_SIZE, then WRITE_SIZE) over
-# `bench.py --steps 5 --warmup 2 --latency-frames 2 --no-cpu-baseline
-# --calibrate-copy 256`, each scaled by what the same counter reads for the
-# plain 256 MiB device copy (fetch x 1.9997, write x 1.0000); setup_bin is
-# k_setup + k_coarse.  Valid for the
-# kernels as they are; re-measure after changing one.
+# `bench.py --config c2 --steps 5 --warmup 2 --latency-frames 2
+# --no-cpu-baseline --calibrate-copy 256`, each scaled by what the same
+# counter reads for the plain 256 MiB device copy (fetch x 1.9997, write
+# x 1.0000); setup_bin is k_setup + k_coarse.  Valid for the kernels as
+# they are; re-measure after changing one.
 MEASURED_TRAFFIC = {
     "c2": {"shadow_raster": 92224574, "raster_shade": 134691073, "setup_bin": 134339832},
 }
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------ roofline
 def algorithmic_bytes(sc, st):
     """Compulsory HBM bytes per frame and per kernel launch.
 
@@ -56,8 +75,8 @@ def algorithmic_bytes(sc, st):
         frame += T * 36 + 2 * c.dim * c.dim * 4
     visible = st["small"] + st["large"]
     entries = st["bin_entries"]
-    casters = st["cascade_casters"]
-    sh_entries = st["cascade_entries"]
+    casters = st.get("cascade_casters", [])
+    sh_entries = st.get("cascade_entries", [])
     per_kernel = {
         # bin entries and the setup records they name; RGBA8 + depth out
         # (8 B/px); the winning triangles' vertex data (132 B each, bounded
@@ -74,6 +93,13 @@ def algorithmic_bytes(sc, st):
         + sum(c.dim * c.dim * 4 for c in sc.cascades),
     }
     return frame, per_kernel
+
+
+def roofline(kernel, algo_bytes, avg_us, traffic=None):
+    ach = algo_bytes / (avg_us * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kernel,
+            "algorithmic_bytes": int(algo_bytes), "avg_us": round(avg_us, 2)}
 
 
 def calibrate_copy(mb):
@@ -94,23 +120,55 @@ def calibrate_copy(mb):
     hip.hipFree(b)
 
 
-def cpu_baseline(sc, budget_s=12.0):
+# -------------------------------------------------------- CPU baseline
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count() or 1
+    return {"cpu_model": model, "host_cpus": os.cpu_count() or 1, "cpu_share": share}
+
+
+def cpu_baseline(frames, what, budget_s=10.0, one_thread_s=5.0):
+    """The C++ oracle (the reference's OpenCL kernels restated) rendering the
+    scenes in `frames` (cycled) on the host: with one worker per allotted
+    core for about budget_s, then on one thread for about one_thread_s (at
+    least one frame)."""
     from oracle import pyoracle
 
-    cores = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        pyoracle.render(sc, nthreads=cores)
-        n += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": f"{n} full frames of the same workload (shadow cascades, drawAll, "
-                      f"gamma) by the C++ oracle, {dt:.1f} s"}
+    info = cpu_info()
+    threads = max(1, min(CPU_SHARE_MAX, info["cpu_share"]))
+
+    def rate(nthreads, budget):
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            pyoracle.render(frames[n % len(frames)], nthreads=nthreads)
+            n += 1
+            if time.perf_counter() - t0 >= budget:
+                break
+        return n, time.perf_counter() - t0
+
+    n, dt = rate(threads, budget_s)
+    n1, dt1 = rate(1, one_thread_s)
+    out = {"value": round(n / dt, 3), "unit": "frames/s", "cores": threads, "kind": "port",
+           "sample": f"{n} frames of {what} by the C++ oracle on {threads} threads in "
+                     f"{dt:.1f} s; {n1} frames on 1 thread in {dt1:.1f} s",
+           "one_thread_value": round(n1 / dt1, 4)}
+    out.update(info)
+    return out
 
 
+# ------------------------------------------------------- plain path (C2)
 def build_scene(name):
     from axi_visualizer_amd import scenes
 
@@ -121,292 +179,36 @@ def build_scene(name):
     raise SystemExit(f"unknown config {name}")
 
 
-def bench_c4(args):
-    """Batched multi-view: 64 jobs of a 500k-triangle scene per batch,
-    autotuned bin size and capacity, graph-replayed; against the same 64
-    views drawn one at a time through CLDraw (the plain path)."""
-    from axi_visualizer_amd import scenes
-    from axi_visualizer_amd.batch import (MULTIVIEW_SIZES, Batch, SceneContext, autotune,
-                                          multiview_jobs, settle)
-    from axi_visualizer_amd.ops import CLDraw
-    from axi_visualizer_amd.render import load_scene
-
-    sc = scenes.spheres_scene(W=320, H=240, n_inst=1000, shadows=False, name="c4_spheres")
-    ctx = SceneContext(sc)
-    jobs = multiview_jobs(ctx)
-    tune = autotune(jobs, sizes=(8, 16, 32), cap_scales=(1, 4), reps=3, graph=True)
-    best = tune["best"]
-    print(f"[bench] c4 autotune: {tune}", file=sys.stderr, flush=True)
-    b = Batch(best["bin_size"], best["cap_small"], best["cap_large"], graph=True)
-    settle(b, jobs)
-    for _ in range(max(2, args.warmup)):
-        b.render(jobs)
-    b.wait()
-    steps = max(1, args.steps // 20)
+def time_plain(r, steps, latency_frames, timing_frames):
+    """Frames/s (steps back to back), per-frame latency (each submitted and
+    waited for) and per-kernel HIP-event durations of a Renderer."""
+    d = r.draw
     t0 = time.perf_counter()
     for _ in range(steps):
-        b.render(jobs)
-    b.wait()
-    dt = time.perf_counter() - t0
-    fps = steps * len(jobs) / dt
-    lat, gpu = [], []
-    for _ in range(max(5, args.latency_frames // 10)):
-        t1 = time.perf_counter()
-        b.render(jobs)
-        gpu.append(b.wait())
-        lat.append((time.perf_counter() - t1) * 1e3)
-    # plain path: the same views one at a time, one CLDraw per resolution
-    draws = {}
-    for w, h in MULTIVIEW_SIZES:
-        d = CLDraw(1, len(sc.groups), w, h, 1)
-        load_scene(d, sc)
-        draws[w, h] = d
-    shaders = [g.flags for g in sc.groups]
-
-    def plain_pass():
-        for j in jobs:
-            d = draws[j.width, j.height]
-            d.setScaleCull(j.camera.scale, j.camera.cax, j.camera.cay)
-            d.setPos(j.camera.pos)
-            d.setVM(j.camera.vm)
-            d.clearZBuffer()
-            d.drawAll(shaders, shadowIds=[])
-            d.gamma(1.0)
-
-    plain_pass()
-    for d in draws.values():
-        d.sync()
-    t0 = time.perf_counter()
-    reps = max(1, steps // 2)
-    for _ in range(reps):
-        plain_pass()
-    for d in draws.values():
-        d.sync()
-    plain_fps = reps * len(jobs) / (time.perf_counter() - t0)
-    px = sum(j.width * j.height for j in jobs)
-    algo = sum(sc.ntri * 36 + j.width * j.height * 8 for j in jobs)
-    gpu_ms = float(np.median(gpu))
-    ach = algo / (gpu_ms * 1e-3) / 1e9
-    line = {
-        "metric": "frames/s (batched, 64 views per launch sequence)", "value": round(fps, 2),
-        "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-        "ms_per_step": round(dt / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generators)",
-        "config": {"workload": "batched_64_views_500k_tris_mixed_320x240_to_1080p",
-                   "jobs": len(jobs), "triangles": sc.ntri, "pixels_per_batch": px,
-                   "bin_size": best["bin_size"], "cap_small": best["cap_small"],
-                   "cap_large": best["cap_large"], "graph": True},
-        "plain_fps_same_views": round(plain_fps, 2),
-        "speedup_vs_plain": round(fps / plain_fps, 3),
-        "mpix_per_s": round(fps * px / len(jobs) / 1e6, 2),
-        "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
-                             "p99": round(float(np.percentile(lat, 99)), 3)},
-        "batch_gpu_ms": round(gpu_ms, 3),
-        "autotune": tune["all"],
-        "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "batch (setup + order + raster)", "algorithmic_bytes": int(algo),
-                     "avg_us": round(gpu_ms * 1e3, 2)},
-        "cpu_baseline": None,
-    }
-    b.close()
-    for d in draws.values():
-        d.close()
-    ctx.close()
-    print(json.dumps(line))
-
-
-def bench_c5(args):
-    """Serving: 16 resident scenes (50k-1M triangles, terrain and spheres,
-    1-4 textures, two shadow cascades each), Poisson arrivals at 2,000 req/s,
-    random scene / camera / resolution (480p, 720p, 1080p), shadows on half
-    the requests, 200 GB HBM admission budget, batches formed every <= 2 ms.
-    Reports latency at the offered load, the sustained (saturated) frames/s,
-    and the same requests rendered one at a time for comparison."""
-    from axi_visualizer_amd import scenes
-    from axi_visualizer_amd.batch import Batch, Job
-    from axi_visualizer_amd.server import SERVING_SIZES as C5_SIZES
-    from axi_visualizer_amd.server import RenderServer, poisson_requests
-
-    mix = scenes.serving_scenes()
-    kinds = [k for _, k in mix]
-    print(f"[bench] c5: {len(mix)} scenes, {sum(s.ntri for s, _ in mix)} triangles",
-          file=sys.stderr, flush=True)
-    srv = RenderServer([s for s, _ in mix], hbm_budget=200e9, slots=3, max_batch=64,
-                       window_ms=2.0)
-
-    def cam(s, w, h, rng):
-        return scenes.serving_camera(kinds[s], w, h, rng.random(4))
-
-    # warm-up: every slot's buffers at their largest, then one short burst
-    srv.reserve(*C5_SIZES[-1])
-    warm = poisson_requests(len(mix), 1e6, 200e-6, C5_SIZES, seed=99, camera_for=cam)
-    srv.run(warm)
-    srv.completed.clear()
-    offered = poisson_requests(len(mix), 2000.0, args.serve_seconds, C5_SIZES, seed=1,
-                               camera_for=cam)
-    at_load = srv.run(offered)
-    print(f"[bench] c5 at 2000 req/s: {at_load}", file=sys.stderr, flush=True)
-    srv.completed.clear()
-    burst = poisson_requests(len(mix), 1e6, 2e-3, C5_SIZES, seed=2, camera_for=cam)
-    for r in burst:
-        r.arrival = 0.0
-    sat = srv.run(burst)
-    print(f"[bench] c5 saturated: {sat}", file=sys.stderr, flush=True)
-    # the same kind of requests one at a time, each waited for
-    one = Batch()
-    sample = burst[:min(len(burst), 300)]
-    t0 = time.perf_counter()
-    for r in sample:
-        one.render([Job(srv.contexts[r.scene], r.camera, r.width, r.height, shadows=r.shadows,
-                        pcf_taps=r.pcf_taps)])
-        one.wait()
-    single_fps = len(sample) / (time.perf_counter() - t0)
-    one.close()
-    line = {
-        "metric": "frames/s (serving, sustained)", "value": round(sat["fps"], 2),
-        "unit": "frames/s", "n_gpus": 1, "steps": sat["completed"], "warmup": len(warm),
-        "ms_per_step": round(1e3 / sat["fps"], 4) if sat["fps"] else None,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded scene generators)",
-        "config": {"workload": "serving_16_scenes_poisson_2000rps_480p_720p_1080p",
-                   "scenes": len(mix), "triangles": [s.ntri for s, _ in mix],
-                   "hbm_budget_bytes": 200e9, "slots": 3, "max_batch": 64, "window_ms": 2.0},
-        "offered_rate": 2000.0,
-        "at_offered_load": {"fps": round(at_load["fps"], 2), "p50_ms": round(at_load["p50_ms"], 3),
-                            "p99_ms": round(at_load["p99_ms"], 3),
-                            "mean_batch": round(at_load["mean_batch"], 2),
-                            "completed": at_load["completed"]},
-        "saturated": {"p50_ms": round(sat["p50_ms"], 3), "p99_ms": round(sat["p99_ms"], 3),
-                      "mean_batch": round(sat["mean_batch"], 2),
-                      "max_concurrent_batches": sat["max_concurrent_batches"]},
-        "one_at_a_time_fps": round(single_fps, 2),
-        "speedup_vs_one_at_a_time": round(sat["fps"] / single_fps, 3) if single_fps else None,
-        "resident_bytes": sat["resident_bytes"],
-        "roofline": None, "cpu_baseline": None,
-    }
-    srv.close()
-    print(json.dumps(line))
-
-
-def bench_c3(args):
-    """Full frame at 2560x1440: the config-1 terrain upscaled to 388^2
-    (299,538 triangles) plus 400 config-2 spheres (200,000 triangles),
-    two 2048^2 shadow cascades with 3x3 PCF, the cube map pass (6 x 512^2
-    HDR faces rendered from the scene), the SSR water plane (about 30% of
-    the screen, 64 march steps) over the opaque frame, 5-level fp16 bloom,
-    gamma.  A step is one whole frame."""
-    from axi_visualizer_amd import scenes
-    from axi_visualizer_amd.render import FullFrame
-
-    sc = scenes.full_frame_scene()
-    print(f"[bench] scene {sc.name}: {sc.ntri} tris {sc.W}x{sc.H}", file=sys.stderr, flush=True)
-    ff = FullFrame(sc, scenes.full_frame_probe(sc), cube_size=512)
-    d = ff.draw
-    for _ in range(args.warmup):
-        ff.frame()
-    d.sync()
-    ff.cube.wait()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ff.frame()
-    d.sync()
-    dt = time.perf_counter() - t0
-    fps = args.steps / dt
-    print(f"[bench] {args.steps} frames in {dt:.3f} s", file=sys.stderr, flush=True)
-    lat = []
-    for _ in range(args.latency_frames):
-        t1 = time.perf_counter()
-        ff.frame()
-        d.sync()
-        lat.append((time.perf_counter() - t1) * 1e3)
-    lat = np.array(lat)
-    # per-kernel durations: HIP events on the scene's stream, the cube pass
-    # by the batch's own events (waited for after each frame)
-    d.setTiming(True)
-    nt = max(10, args.steps // 4)
-    cube_ms = []
-    for _ in range(nt):
-        ff.frame()
-        cube_ms.append(ff.cube.wait())
-    tm = d.timing()
-    d.setTiming(False)
-    kern = {k: {"avg_us": v[0] / v[1] * 1e3, "launches": v[1]} for k, v in tm.items()}
-    kern["cube_pass"] = {"avg_us": float(np.mean(cube_ms)) * 1e3, "launches": nt}
-    tot = sum(v["avg_us"] * v["launches"] for v in kern.values())
-    for v in kern.values():
-        v["share"] = v["avg_us"] * v["launches"] / tot if tot else 0.0
-    st = d.stats()
-    water = sum(g.ntri for g in sc.groups if g.ssr >= 0)
-    W, H = sc.W, sc.H
-    # algorithmic bytes of the kernels a roofline is stated for: bloom reads
-    # and writes the frame's planes (12 B/px) and writes each fp16 level and
-    # its upsampled sum once and reads them back (5 x 6 B per level texel,
-    # 1/3 texel per frame pixel over the levels)
-    per_kernel = {"bloom": W * H * 12 + int(5 * 6 * W * H / 3)}
-    dom = max((k for k in kern if k in per_kernel), key=lambda k: kern[k]["avg_us"])
-    ach = per_kernel[dom] / (kern[dom]["avg_us"] * 1e-6) / 1e9
-    roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
-            "algorithmic_bytes": int(per_kernel[dom]), "avg_us": round(kern[dom]["avg_us"], 2)}
-    line = {
-        "metric": "frames/s (full frame)", "value": round(fps, 2), "unit": "frames/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 / fps, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generators)",
-        "config": {"workload": "full_frame_1440p_500k_tris_shadow_cube6x512_ssr64_bloom5",
-                   "W": W, "H": H, "triangles": sc.ntri, "water_triangles": water,
-                   "shadow_cascades": len(sc.cascades), "shadow_dim": sc.cascades[0].dim,
-                   "cube_face": 512, "ssr_steps": sc.ssr_steps, "bloom_levels": 5},
-        "mtris_per_s": round(sc.ntri * fps / 1e6, 2),
-        "mpix_per_s": round(W * H * fps / 1e6, 2),
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
-                       "p99": round(float(np.percentile(lat, 99)), 4)},
-        "kernels": {k: {"avg_us": round(v["avg_us"], 2), "share": round(v["share"], 4)}
-                    for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["share"])},
-        "visible": st,
-        "roofline": roof, "cpu_baseline": None,
-    }
-    ff.close()
-    print(json.dumps(line))
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c2")
-    ap.add_argument("--latency-frames", type=int, default=100)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=3.0,
-                    help="serving: seconds of Poisson arrivals")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="also run one plain device copy of MIB MiB (counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
-        sys.exit(2)
-
-    if args.config == "c3":
-        return bench_c3(args)
-    if args.config == "c4":
-        return bench_c4(args)
-    if args.config == "c5":
-        return bench_c5(args)
-
-    from axi_visualizer_amd.render import Renderer
-
-    sc, workload = build_scene(args.config)
-    print(f"[bench] scene {sc.name}: {sc.ntri} tris {sc.W}x{sc.H}", file=sys.stderr, flush=True)
-    r = Renderer(sc)
-    d = r.draw
-    for _ in range(args.warmup):
         r.frame()
     d.sync()
-    # one probe frame for the per-pass counters (casters and shadow-bin
-    # entries of each cascade, screen-bin entries of the draw)
+    dt = time.perf_counter() - t0
+    lat = []
+    for _ in range(latency_frames):
+        t1 = time.perf_counter()
+        r.frame()
+        d.sync()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    d.setTiming(True)
+    for _ in range(timing_frames):
+        r.frame()
+    tm = d.timing()
+    d.setTiming(False)
+    tot = sum(v[0] for v in tm.values())
+    kern = {k: {"avg_us": v[0] / v[1] * 1e3, "launches": v[1],
+                "share": v[0] / tot if tot else 0.0} for k, v in tm.items()}
+    return steps / dt, dt, np.array(lat if lat else [0.0]), kern
+
+
+def probe_stats(r):
+    """One frame's counters: visible triangles, bin entries, and per cascade
+    its casters and shadow-bin entries."""
+    d = r.draw
     per_cascade = []
     r.frame(after_cascade=lambda i: per_cascade.append(d.stats()))
     st = d.stats()
@@ -414,59 +216,46 @@ def main():
     st["cascade_entries"] = [p["shadow_bin_entries"] for p in per_cascade]
     if st["bin_overflow"]:
         raise SystemExit(f"bin capacity exceeded: {st}")
+    ids = d.getIds()
+    st["coverage"] = round(float((ids != 0xFFFFFFFF).mean()), 5)
+    return st
 
-    # throughput: K frames back to back
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+
+def kernels_json(kern):
+    return {k: {"avg_us": round(v["avg_us"], 2), "share": round(v["share"], 4)}
+            for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["share"])}
+
+
+def run_plain(args, config):
+    from axi_visualizer_amd import scenes
+    from axi_visualizer_amd.render import Renderer
+
+    sc, workload = build_scene(config)
+    log(f"scene {sc.name}: {sc.ntri} tris {sc.W}x{sc.H}")
+    r = Renderer(sc)
+    d = r.draw
+    for _ in range(args.warmup):
         r.frame()
     d.sync()
-    dt = time.perf_counter() - t0
-    fps = args.steps / dt
-    print(f"[bench] {args.steps} frames in {dt:.3f} s", file=sys.stderr, flush=True)
+    st = probe_stats(r)
+    fps, dt, lat, kern = time_plain(r, args.steps, args.latency_frames,
+                                    max(20, args.steps // 4))
+    log(f"{args.steps} frames in {dt:.3f} s")
+    dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
 
-    # latency: each frame submitted alone and waited for
-    lat = []
-    for _ in range(args.latency_frames):
-        t1 = time.perf_counter()
-        r.frame()
-        d.sync()
-        lat.append((time.perf_counter() - t1) * 1e3)
-    lat = np.array(lat)
-
-    # per-kernel durations with HIP events on the render stream
-    d.setTiming(True)
-    nt = max(20, args.steps // 4)
-    for _ in range(nt):
-        r.frame()
-    tm = d.timing()
-    d.setTiming(False)
-    kern = {k: {"avg_us": v[0] / v[1] * 1e3, "launches": v[1], "share": 0.0}
-            for k, v in tm.items()}
-    tot = sum(v[0] for v in tm.values())
-    for k, v in tm.items():
-        kern[k]["share"] = v[0] / tot if tot else 0.0
-    dom = max(tm, key=lambda k: tm[k][0])
-
-    # readback, timed separately
+    # readback, timed separately (PCIe-inclusive; never part of value)
     t2 = time.perf_counter()
     d.getRGBA8()
     readback_ms = (time.perf_counter() - t2) * 1e3
+    t2 = time.perf_counter()
+    d.getFrame()
+    getframe_ms = (time.perf_counter() - t2) * 1e3
 
     frame_bytes, per_kernel = algorithmic_bytes(sc, st)
-    dom_us = kern[dom]["avg_us"]
-    dom_bytes = per_kernel.get(dom)
     roof = None
-    if dom_bytes:
-        ach = dom_bytes / (dom_us * 1e-6) / 1e9
-        traffic = MEASURED_TRAFFIC.get(args.config, {}).get(dom)
-        roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
-                "algorithmic_bytes": int(dom_bytes), "avg_us": round(dom_us, 2)}
-    frame_gbs = frame_bytes * fps / 1e9
-
-    if args.calibrate_copy:
-        calibrate_copy(args.calibrate_copy)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(sc)
+    if dom in per_kernel:
+        roof = roofline(dom, per_kernel[dom], kern[dom]["avg_us"],
+                        MEASURED_TRAFFIC.get(config, {}).get(dom))
     line = {
         "metric": "frames/s", "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 / fps, 4),
@@ -480,17 +269,395 @@ def main():
         "mpix_per_s": round(sc.W * sc.H * fps / 1e6, 2),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                        "p99": round(float(np.percentile(lat, 99)), 4)},
-        "readback_ms": round(readback_ms, 3),
+        "readback_ms": {"rgba8": round(readback_ms, 3), "getFrame": round(getframe_ms, 3)},
         "frame_algorithmic_bytes": int(frame_bytes),
-        "frame_roofline_frac": round(frame_gbs / HBM_PEAK_GBS, 5),
+        "frame_roofline_frac": round(frame_bytes * fps / 1e9 / HBM_PEAK_GBS, 5),
         "visible": st,
-        "kernels": {k: {"avg_us": round(v["avg_us"], 2), "share": round(v["share"], 4)}
-                    for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["share"])},
+        "kernels": kernels_json(kern),
         "roofline": roof,
-        "cpu_baseline": cpu,
     }
+    cpu_frames = [sc]
+    if config == "c2":
+        # the same scene from a camera inside the box, 2 units in front of a
+        # large sphere: hundreds of large triangles on the tiled path and
+        # most of the frame shaded
+        sc2 = dataclasses.replace(sc, view=scenes.spheres_closeup_camera(sc.W, sc.H, gap=2.0),
+                                  name="c2_closeup")
+        r2 = Renderer(sc2)
+        for _ in range(args.warmup):
+            r2.frame()
+        r2.draw.sync()
+        st2 = probe_stats(r2)
+        fps2, _, lat2, kern2 = time_plain(r2, max(20, args.steps // 2),
+                                          max(10, args.latency_frames // 2),
+                                          max(20, args.steps // 4))
+        _, pk2 = algorithmic_bytes(sc2, st2)
+        line["closeup_pose"] = {
+            "fps": round(fps2, 2), "mtris_per_s": round(sc2.ntri * fps2 / 1e6, 2),
+            "latency_ms": {"p50": round(float(np.percentile(lat2, 50)), 4),
+                           "p99": round(float(np.percentile(lat2, 99)), 4)},
+            "visible": st2, "kernels": kernels_json(kern2),
+            "roofline_raster": roofline("raster_shade", pk2["raster_shade"],
+                                        kern2["raster_shade"]["avg_us"]),
+        }
+        r2.close()
+        cpu_frames.append(sc2)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline([sc], f"the {workload} workload (shadow cascades, "
+                                            "drawAll, gamma)")
+        if config == "c2":
+            from oracle import pyoracle
+            ref = pyoracle.render(cpu_frames[-1], nthreads=CPU_SHARE_MAX)
+            covered = max(1, int((ref["id"] != 0xFFFFFFFF).sum()))
+            line["closeup_pose"]["fragments_per_covered_px"] = round(ref["stats"][2] / covered, 3)
+    else:
+        line["cpu_baseline"] = None
     r.close()
-    print(json.dumps(line))
+    return line
+
+
+# ------------------------------------------------ batched multi-view (C4)
+def plain_views(jobs, shaders_of, reps):
+    """The jobs' views through CLDraw, one CLDraw per (scene, resolution),
+    back to back without host syncs (the engine's per-frame calls:
+    setScaleCull, setPos, setVM, clearZBuffer, drawAll, gamma); the scene's
+    shadow maps rendered once, as the batch keeps them resident."""
+    from axi_visualizer_amd.ops import CLDraw
+    from axi_visualizer_amd.render import load_scene
+
+    draws = {}
+    for j in jobs:
+        key = (id(j.scene), j.width, j.height)
+        if key in draws:
+            continue
+        sc = j.scene.scene
+        d = CLDraw(1, len(sc.groups), j.width, j.height, 1)
+        load_scene(d, sc)
+        cast = [g.cast for g in sc.groups]
+        for i, c in enumerate(sc.cascades):
+            d.clearShadowMap(i)
+            d.shadowMap(i, cast, shaders_of(sc, True), c.bias)
+        draws[key] = d
+
+    def one_pass():
+        for j in jobs:
+            d = draws[(id(j.scene), j.width, j.height)]
+            c = j.camera
+            d.setScaleCull(c.scale, c.cax, c.cay)
+            d.setPos(c.pos)
+            d.setVM(c.vm)
+            d.clearZBuffer()
+            sc = j.scene.scene
+            d.drawAll(shaders_of(sc, j.shadows), shadowIds=list(range(len(sc.cascades)))
+                      if j.shadows else [], pcf=j.pcf_taps if j.shadows else 5)
+            d.gamma(j.exposure)
+
+    one_pass()
+    for d in draws.values():
+        d.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one_pass()
+    for d in draws.values():
+        d.sync()
+    fps = reps * len(jobs) / (time.perf_counter() - t0)
+    for d in draws.values():
+        d.close()
+    return fps
+
+
+def shader_flags_of(sc, shadows):
+    from axi_visualizer_amd import scenes
+    return [g.flags if shadows else g.flags & ~scenes.SH_SHADOW for g in sc.groups]
+
+
+def batch_kernel_roofline(b, jobs, reps=5):
+    """Per-kernel HIP-event times of the batch (eager launches) and the
+    roofline of its dominant kernel, the persistent raster: 40 B per bin
+    entry (4 B entry + 36 B record) and 8 B per pixel written (the 132 B per
+    visible triangle of the single-frame figure are left out: the batch
+    does not count its visible triangles)."""
+    b.set_timing(True)
+    ts = []
+    for _ in range(reps):
+        b.render(jobs)
+        ts.append(b.kernel_timing())
+    b.set_timing(False)
+    med = {k: float(np.median([t[k] for t in ts])) for k in ("setup_ms", "order_ms", "raster_ms")}
+    entries = ts[-1]["bin_entries"]
+    px = sum(j.width * j.height for j in jobs)
+    algo = entries * 40 + px * 8
+    return med, roofline("batch_raster", algo, med["raster_ms"] * 1e3)
+
+
+def run_multiview(args, cpu=True):
+    from axi_visualizer_amd import scenes
+    from axi_visualizer_amd.batch import (Batch, SceneContext, autotune, job_view_scene,
+                                          multiview_jobs, settle)
+
+    sc = scenes.spheres_scene(W=320, H=240, n_inst=1000, shadows=False, name="c4_spheres")
+    ctx = SceneContext(sc)
+    jobs = multiview_jobs(ctx)
+    tune = autotune(jobs, sizes=(8, 16, 32), cap_scales=(1, 4), reps=3, graph=True)
+    best = tune["best"]
+    log(f"c4 autotune: {best}")
+    b = Batch(best["bin_size"], best["cap_small"], best["cap_large"], graph=True)
+    settle(b, jobs)
+    for _ in range(max(2, args.warmup)):
+        b.render(jobs)
+    b.wait()
+    steps = max(5, args.steps // 10)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.render(jobs)
+    b.wait()
+    dt = time.perf_counter() - t0
+    fps = steps * len(jobs) / dt
+    lat = []
+    for _ in range(max(5, args.latency_frames // 10)):
+        t1 = time.perf_counter()
+        b.render(jobs)
+        b.wait()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    kms, roof = batch_kernel_roofline(b, jobs)
+    b.close()
+    plain = plain_views(jobs, shader_flags_of, reps=max(1, steps // 2))
+    px = sum(j.width * j.height for j in jobs)
+    out = {
+        "workload": "batched_64_views_500k_tris_mixed_320x240_to_1080p",
+        "jobs": len(jobs), "triangles": sc.ntri, "pixels_per_batch": px,
+        "bin_size": best["bin_size"], "graph": True,
+        "fps": round(fps, 2), "steps": steps, "ms_per_batch": round(dt / steps * 1e3, 4),
+        "mpix_per_s": round(fps * px / len(jobs) / 1e6, 2),
+        "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                             "p99": round(float(np.percentile(lat, 99)), 3)},
+        "plain_fps_same_views": round(plain, 2),
+        "speedup_vs_plain_same_views": round(fps / plain, 3),
+        "kernels_ms": {k: round(v, 4) for k, v in kms.items()},
+        "roofline": roof,
+    }
+    if cpu and not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline([job_view_scene(j) for j in jobs[:16]],
+                                           "the batch's views (its first 16 jobs)",
+                                           budget_s=8.0, one_thread_s=4.0)
+    else:
+        out["cpu_baseline"] = None
+    ctx.close()
+    return out
+
+
+# ------------------------------------------------------------ serving (C5)
+def run_serving(args, cpu=True):
+    from axi_visualizer_amd import scenes
+    from axi_visualizer_amd.batch import Batch, Job, job_view_scene
+    from axi_visualizer_amd.server import SERVING_SIZES, RenderServer, poisson_requests
+
+    mix = scenes.serving_scenes()
+    kinds = [k for _, k in mix]
+    log(f"c5: {len(mix)} scenes, {sum(s.ntri for s, _ in mix)} triangles")
+    srv = RenderServer([s for s, _ in mix], hbm_budget=200e9, slots=3, max_batch=64,
+                       window_ms=2.0)
+
+    def cam(s, w, h, rng):
+        return scenes.serving_camera(kinds[s], w, h, rng.random(4))
+
+    # warm-up: every slot's buffers at their largest, then one short burst
+    srv.reserve(*SERVING_SIZES[-1])
+    warm = poisson_requests(len(mix), 1e6, 200e-6, SERVING_SIZES, seed=99, camera_for=cam)
+    srv.run(warm)
+    srv.completed.clear()
+    offered = poisson_requests(len(mix), 2000.0, args.serve_seconds, SERVING_SIZES, seed=1,
+                               camera_for=cam)
+    at_load = srv.run(offered)
+    log(f"c5 at 2000 req/s: {at_load}")
+    srv.completed.clear()
+    burst = poisson_requests(len(mix), 1e6, 2e-3, SERVING_SIZES, seed=2, camera_for=cam)
+    for r in burst:
+        r.arrival = 0.0
+    sat = srv.run(burst)
+    log(f"c5 saturated: {sat}")
+    # the dominant kernel of a full batch of the burst's requests
+    jobs64 = [Job(srv.contexts[r.scene], r.camera, r.width, r.height, shadows=r.shadows,
+                  pcf_taps=r.pcf_taps) for r in burst[:64]]
+    kb = Batch()
+    kb.render(jobs64)
+    kb.wait()
+    kms, roof = batch_kernel_roofline(kb, jobs64)
+    # the same requests one at a time, each waited for
+    sample = burst[:min(len(burst), 300)]
+    t0 = time.perf_counter()
+    for r in sample:
+        kb.render([Job(srv.contexts[r.scene], r.camera, r.width, r.height, shadows=r.shadows,
+                       pcf_taps=r.pcf_taps)])
+        kb.wait()
+    single_fps = len(sample) / (time.perf_counter() - t0)
+    kb.close()
+    sample_jobs = [Job(srv.contexts[r.scene], r.camera, r.width, r.height, shadows=r.shadows,
+                       pcf_taps=r.pcf_taps) for r in sample]
+    plain = plain_views(sample_jobs, shader_flags_of, reps=2)
+    out = {
+        "workload": "serving_16_scenes_poisson_2000rps_480p_720p_1080p",
+        "scenes": len(mix), "triangles": [s.ntri for s, _ in mix],
+        "hbm_budget_bytes": 200e9, "slots": 3, "max_batch": 64, "window_ms": 2.0,
+        "fps": round(sat["fps"], 2), "requests": sat["completed"],
+        "at_offered_load": {"offered_rate": 2000.0, "fps": round(at_load["fps"], 2),
+                            "p50_ms": round(at_load["p50_ms"], 3),
+                            "p99_ms": round(at_load["p99_ms"], 3),
+                            "mean_batch": round(at_load["mean_batch"], 2),
+                            "completed": at_load["completed"]},
+        "saturated": {"p50_ms": round(sat["p50_ms"], 3), "p99_ms": round(sat["p99_ms"], 3),
+                      "mean_batch": round(sat["mean_batch"], 2),
+                      "max_concurrent_batches": sat["max_concurrent_batches"]},
+        "plain_fps_same_requests": round(plain, 2),
+        "speedup_vs_plain_same_requests": round(sat["fps"] / plain, 3),
+        "one_at_a_time_fps": round(single_fps, 2),
+        "speedup_vs_one_at_a_time": round(sat["fps"] / single_fps, 3),
+        "resident_bytes": sat["resident_bytes"],
+        "kernels_ms_64_request_batch": {k: round(v, 4) for k, v in kms.items()},
+        "roofline": roof,
+    }
+    if cpu and not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline([job_view_scene(j) for j in sample_jobs[:24]],
+                                           "the burst's requests (its first 24; the oracle "
+                                           "renders a request's shadow maps with it)",
+                                           budget_s=8.0, one_thread_s=4.0)
+    else:
+        out["cpu_baseline"] = None
+    srv.close()
+    return out
+
+
+# --------------------------------------------------------- full frame (C3)
+def run_full_frame(args):
+    """Full frame at 2560x1440: the config-1 terrain upscaled to 388^2
+    (299,538 triangles) plus 400 config-2 spheres (200,000 triangles),
+    two 2048^2 shadow cascades with 3x3 PCF, the cube map pass (6 x 512^2
+    HDR faces rendered from the scene), the SSR water plane (about 30% of
+    the screen, 64 march steps) over the opaque frame, 5-level fp16 bloom,
+    gamma.  A step is one whole frame."""
+    from axi_visualizer_amd import scenes
+    from axi_visualizer_amd.render import FullFrame
+
+    sc = scenes.full_frame_scene()
+    log(f"scene {sc.name}: {sc.ntri} tris {sc.W}x{sc.H}")
+    ff = FullFrame(sc, scenes.full_frame_probe(sc), cube_size=512)
+    d = ff.draw
+    for _ in range(args.warmup):
+        ff.frame()
+    d.sync()
+    ff.cube.wait()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ff.frame()
+    d.sync()
+    dt = time.perf_counter() - t0
+    fps = args.steps / dt
+    log(f"{args.steps} frames in {dt:.3f} s")
+    lat = []
+    for _ in range(args.latency_frames):
+        t1 = time.perf_counter()
+        ff.frame()
+        d.sync()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    lat = np.array(lat if lat else [0.0])
+    # the opaque drawAll's counters (bin entries, visible triangles) for the
+    # raster's algorithmic bytes: one drawAll of the frame without its
+    # water groups (the SSR pass sets up and draws them on its own)
+    water_mask = [m or isinstance(s, dict) for m, s in zip(ff.mask, ff.shaders)]
+    d.setPos(sc.view.pos)
+    d.setVM(sc.view.vm)
+    d.clearZBuffer()
+    d.drawAll([0 if isinstance(s, dict) else s for s in ff.shaders], mask=water_mask,
+              shadowIds=list(range(len(sc.cascades))), pcf=sc.pcf_taps)
+    st = d.stats()
+    d.sync()
+    d.setTiming(True)
+    nt = max(10, args.steps // 4)
+    cube_ms = []
+    for _ in range(nt):
+        ff.frame()
+        cube_ms.append(ff.cube.wait())
+    tm = d.timing()
+    d.setTiming(False)
+    kern = {k: {"avg_us": v[0] / v[1] * 1e3, "launches": v[1]} for k, v in tm.items()}
+    kern["cube_pass"] = {"avg_us": float(np.mean(cube_ms)) * 1e3, "launches": nt}
+    tot = sum(v["avg_us"] * v["launches"] for v in kern.values())
+    for v in kern.values():
+        v["share"] = v["avg_us"] * v["launches"] / tot if tot else 0.0
+    water = sum(g.ntri for g in sc.groups if g.ssr >= 0)
+    W, H = sc.W, sc.H
+    raster = next((k for k in kern if k.startswith("raster") and "ssr" not in k), None)
+    roof = None
+    if raster:
+        # HDR raster: bin entries (4 B) and their records (36 B), the visible
+        # triangles' vertex data (132 B), depth and three u16 planes out
+        # (10 B/px); counters of the frame's last setup
+        algo = st["bin_entries"] * 40 + (st["small"] + st["large"]) * 132 + W * H * 10
+        roof = roofline(raster, algo, kern[raster]["avg_us"])
+    line = {
+        "metric": "frames/s (full frame)", "value": round(fps, 2), "unit": "frames/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 / fps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generators)",
+        "config": {"workload": "full_frame_1440p_500k_tris_shadow_cube6x512_ssr64_bloom5",
+                   "W": W, "H": H, "triangles": sc.ntri, "water_triangles": water,
+                   "shadow_cascades": len(sc.cascades), "shadow_dim": sc.cascades[0].dim,
+                   "cube_face": 512, "ssr_steps": sc.ssr_steps, "bloom_levels": 5},
+        "mtris_per_s": round(sc.ntri * fps / 1e6, 2),
+        "mpix_per_s": round(W * H * fps / 1e6, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                       "p99": round(float(np.percentile(lat, 99)), 4)},
+        "kernels": kernels_json(kern),
+        "visible": st,
+        "roofline": roof, "cpu_baseline": None,
+    }
+    ff.close()
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="all",
+                    help="all (default: c2 with the batched and serving sub-objects), "
+                         "c1, c2, c3, c4 or c5 alone")
+    ap.add_argument("--latency-frames", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="serving: seconds of Poisson arrivals")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    if args.config == "c3":
+        line = run_full_frame(args)
+    elif args.config == "c4":
+        line = run_multiview(args)
+    elif args.config == "c5":
+        line = run_serving(args)
+    elif args.config in ("c1", "c2"):
+        line = run_plain(args, args.config)
+    elif args.config == "all":
+        line = run_plain(args, "c2")
+        line["metric"] = ("frames/s at 1080p / 1M tris (plain path); batched-serving vs "
+                          "plain fps in batched_multiview and serving")
+        mv = run_multiview(args)
+        sv = run_serving(args)
+        for sub in (mv, sv):
+            sub["speedup_vs_plain_headline"] = round(sub["fps"] / line["value"], 3)
+        line["batched_multiview"] = mv
+        line["serving"] = sv
+    else:
+        raise SystemExit(f"unknown config {args.config}")
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -220,6 +220,12 @@ int axr_batch_configure(axr_batch* b, int bin_size, uint32_t cap_small, uint32_t
  * a bucket is full), from the next render on; it grows by itself when a
  * render loses entries.  Mirrors axr_set_bin_capacity's spill argument. */
 int axr_batch_set_spill_capacity(axr_batch* b, uint32_t spill);
+/* Per-kernel timing of the batch (HIP events on its stream; a timed render
+ * runs its three launches without the graph).  axr_batch_get_timing waits
+ * for the last render and gives the setup, bin-order and raster
+ * milliseconds, and the bin entries the setup appended (may be NULL). */
+int axr_batch_set_timing(axr_batch* b, int enable);
+int axr_batch_get_timing(axr_batch* b, float ms[3], uint32_t* bin_entries);
 /* Enqueues the jobs; returns without waiting.  output: AXR_OUT_RGBA8
  * (gamma'd colour + depth), AXR_OUT_IDS (winner ids + depth) or
  * AXR_OUT_HDR (u16 colour planes + depth). */
