@@ -60,6 +60,7 @@ SIGNATURES = {
     "axr_get_refl_tex": (_i, [_ctx, _i, _p, _p, _p]),
     "axr_draw_ssr": (_i, [_ctx, _p, _p, _i, _i, _i, _f, _i]),
     "axr_bloom": (_i, [_ctx, _i, _f, _f, _f]),
+    "axr_bloom_planes_noalloc": (_i, [_p, _p, _p, _i, _i, _i, _f, _f, _f, _p]),
     "axr_get_frame": (_i, [_ctx, _p, _p, _p, _p]),
     "axr_get_rgba8": (_i, [_ctx, _p]),
     "axr_get_ids": (_i, [_ctx, _p]),

@@ -262,3 +262,69 @@ hipError_t launch_bloom(uint16_t* R, uint16_t* G, uint16_t* B, int W, int H, int
 }
 
 }  // namespace axr
+
+// ------------------------------------------------ bloom without allocation
+// axr_bloom_planes_noalloc: the bloom passes on caller-owned device planes,
+// on the caller's stream, with the level pyramid in a file-scope device
+// array instead of memory the library allocates.  Each GPU holds its own
+// copy of the array (until the process ends); its address on a device is
+// looked up once, with that device current, and kept per device: host
+// code passes that address to the kernels, never the array's name.  Every
+// level is written before it is read, so nothing needs clearing between
+// calls; calls that overlap on one GPU share the array and race.
+#include <mutex>
+
+#include "axr_ctx.h"
+
+namespace {
+
+// 32 MiB: the fp16 levels and upward chain of a 3840x2160 frame with four
+// levels (33,048,000 bytes) fit.
+constexpr size_t kNoallocScratchBytes = (size_t)32 << 20;
+__device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+constexpr int kNoallocMaxDevices = 64;
+void* g_scratch_ptr[kNoallocMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+}  // namespace
+
+extern "C" int axr_bloom_planes_noalloc(uint16_t* r, uint16_t* g, uint16_t* b, int w, int h,
+                                        int levels, float threshold, float white,
+                                        float strength, void* stream) {
+  using namespace axri;
+  if (!r || !g || !b) return fail(AXR_EINVAL, "bloom_noalloc: null plane");
+  if (w <= 0 || h <= 0) return fail(AXR_EINVAL, "bloom_noalloc: frame %dx%d", w, h);
+  if (levels < 1 || levels > 12)
+    return fail(AXR_EINVAL, "bloom_noalloc: levels must be 1..12, got %d", levels);
+  if ((w % (1 << levels)) || (h % (1 << levels)))
+    return fail(AXR_EINVAL, "bloom_noalloc: %dx%d is not a multiple of 2^%d", w, h, levels);
+  if (!(white > 0.0f) || !(strength >= 0.0f) || threshold != threshold)
+    return fail(AXR_EINVAL, "bloom_noalloc: white must be > 0 and strength >= 0");
+  const size_t need = 2 * bloom_halves(w, h, levels);
+  if (need > kNoallocScratchBytes)
+    return fail(AXR_EINVAL,
+                "bloom_noalloc: %dx%d with %d levels needs %zu bytes of scratch; the "
+                "no-allocation entry point holds %zu (up to 3840x2160 with 4 levels)",
+                w, h, levels, need, kNoallocScratchBytes);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoallocMaxDevices)
+    return fail(AXR_EINVAL, "bloom_noalloc: device %d beyond %d", dev, kNoallocMaxDevices);
+  void* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void* q = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !q)
+        return fail(AXR_EHIP, "bloom_noalloc: scratch address on device %d: %s", dev,
+                    hipGetErrorString(e));
+      g_scratch_ptr[dev] = q;
+    }
+    p = g_scratch_ptr[dev];
+  }
+  HIPCHK(launch_bloom(r, g, b, w, h, levels, threshold, white, strength, p,
+                      (hipStream_t)stream));
+  return AXR_OK;
+}

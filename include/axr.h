@@ -134,6 +134,17 @@ int axr_blur(axr_ctx* ctx);
  * weight white * strength.  W and H must be multiples of 2^levels. */
 int axr_bloom(axr_ctx* ctx, int levels, float threshold, float white, float strength);
 
+/* No counterpart in the reference.  axr_bloom's passes (same kernels, same
+ * result) on caller-owned device planes r, g, b (u16, w*h each, updated in
+ * place) on the caller's HIP stream (NULL: the null stream), allocating
+ * nothing: for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops.  The level pyramid lives in one
+ * 32 MiB scratch array per GPU, enough for frames up to 3840x2160 with 4
+ * levels; larger requests fail with AXR_EINVAL.  Calls that overlap on one
+ * GPU share that array and race. */
+int axr_bloom_planes_noalloc(uint16_t* r, uint16_t* g, uint16_t* b, int w, int h, int levels,
+                             float threshold, float white, float strength, void* stream);
+
 /* CLDraw.getFrame(), Ops.py:1224-1230: R, G, B u16 planes and f32 depth,
  * each [h][w].  Any pointer may be NULL. */
 int axr_get_frame(axr_ctx* ctx, uint16_t* r, uint16_t* g, uint16_t* b, float* depth);
