@@ -95,8 +95,16 @@ SIGNATURES = {
 _lib = None
 
 
+# error codes of include/axr.h
+AXR_EINVAL, AXR_EHIP, AXR_ESTATE, AXR_ECAP = -1, -2, -3, -4
+
+
 class AxrError(RuntimeError):
-    """A libaxr call returned an error code."""
+    """A libaxr call returned an error code (``code``: an AXR_E* value)."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 def lib() -> ctypes.CDLL:
@@ -117,7 +125,7 @@ def lib() -> ctypes.CDLL:
 def check(rc: int, what: str) -> int:
     if rc < 0:
         msg = lib().axr_last_error()
-        raise AxrError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+        raise AxrError(f"{what} failed ({rc}): {msg.decode() if msg else ''}", rc)
     return rc
 
 

@@ -18,7 +18,8 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from ._native import OUT_HDR, OUT_IDS, OUT_RGBA8, AxrError, AxrJob, check, lib, ptr
+from ._native import (AXR_ECAP, OUT_HDR, OUT_IDS, OUT_RGBA8, AxrError, AxrJob, check, lib,
+                      ptr)
 from .ops import CLDraw
 from .render import load_scene
 from .scenes import Camera, Scene, make_camera
@@ -241,9 +242,9 @@ def settle(b: "Batch", jobs: Sequence[Job], tries: int = 6) -> float:
         try:
             return b.wait()
         except AxrError as e:
-            if "spill list" not in str(e):
+            if e.code != AXR_ECAP:
                 raise
-    raise AxrError("bin storage did not settle")
+    raise AxrError("bin storage did not settle", AXR_ECAP)
 
 
 def autotune(jobs: Sequence[Job], sizes=(8, 16, 32), cap_scales=(1, 4), reps: int = 3,

@@ -11,6 +11,14 @@ within the HBM budget, and launches it without waiting.  Finished slots are
 polled, their requests completed and their memory released, so requests
 that arrive while others render are taken as slots and memory free up.
 
+The queue is bounded: a request that arrives to a full queue is dropped and
+counted, as the reference engine counts the frames its full output queue
+refuses (Compute.py:590-593).  A request can be cancelled while queued (it
+is removed) or in flight (its result is discarded when the batch ends).  A
+batch whose bin storage overflowed (AXR_ECAP: the storage has grown) goes
+back to the head of the queue and renders again; any other error of a
+batch propagates.
+
 A request's pixels do not depend on what else is in its batch: every job is
 bit-identical to the same view rendered alone.
 """
@@ -19,10 +27,11 @@ from __future__ import annotations
 import collections
 import dataclasses
 import time
-from typing import Dict, List, Optional, Sequence
+from typing import Callable, Dict, List, Optional, Sequence
 
 import numpy as np
 
+from ._native import AXR_ECAP, AxrError
 from .batch import Batch, Job, SceneContext, settle
 from .scenes import Camera, make_camera
 
@@ -31,6 +40,10 @@ _CAP_SMALL, _CAP_LARGE = 4096, 1024
 
 # Resolutions of the serving workload's requests: 480p, 720p, 1080p.
 SERVING_SIZES = [(848, 480), (1280, 720), (1920, 1080)]
+
+# Request states
+QUEUED, RUNNING, DONE, DROPPED, REJECTED, CANCELLED = (
+    "queued", "running", "done", "dropped", "rejected", "cancelled")
 
 
 @dataclasses.dataclass
@@ -47,6 +60,8 @@ class Request:
     done: float = 0.0          # completion observed
     batch_size: int = 0
     rgba: Optional[np.ndarray] = None
+    state: str = QUEUED
+    attempts: int = 0          # launches (more than one after a requeue)
 
     @property
     def latency_ms(self) -> float:
@@ -61,8 +76,8 @@ def request_bytes(ntri: int, w: int, h: int, bin_size: int = 32) -> int:
 
 
 class _Slot:
-    def __init__(self, bin_size: int, graph: bool):
-        self.batch = Batch(bin_size=bin_size, graph=graph)
+    def __init__(self, batch):
+        self.batch = batch
         self.reqs: List[Request] = []
         self.bytes = 0
 
@@ -71,29 +86,47 @@ class _Slot:
         return bool(self.reqs)
 
 
+def _default_batch(bin_size: int, graph: bool):
+    return Batch(bin_size=bin_size, graph=graph)
+
+
 class RenderServer:
-    """Continuous-batching render loop over resident scenes."""
+    """Continuous-batching render loop over resident scenes.
+
+    context_factory(scene) makes a scene resident (default SceneContext);
+    batch_factory(bin_size, graph) makes a batch slot (default Batch);
+    clock() gives seconds (default time.perf_counter)."""
 
     def __init__(self, scenes: Sequence, hbm_budget: float = 200e9, slots: int = 3,
                  max_batch: int = 64, window_ms: float = 2.0, bin_size: int = 32,
-                 readback: bool = False, graph: bool = False):
-        self.contexts: List[SceneContext] = [SceneContext(s) for s in scenes]
+                 readback: bool = False, graph: bool = False, max_queue: int = 8192,
+                 context_factory: Optional[Callable] = None,
+                 batch_factory: Optional[Callable] = None,
+                 clock: Callable[[], float] = time.perf_counter):
+        make_ctx = context_factory or SceneContext
+        make_batch = batch_factory or _default_batch
+        self.contexts = [make_ctx(s) for s in scenes]
         self.ntri = [c.scene.ntri for c in self.contexts]
         self.resident = sum(c.device_bytes() for c in self.contexts)
         self.budget = float(hbm_budget)
         if self.resident > self.budget:
             raise ValueError(f"resident scenes need {self.resident} B, budget {self.budget:.0f}")
-        self.slots = [_Slot(bin_size, graph) for _ in range(slots)]
+        self.slots = [_Slot(make_batch(bin_size, graph)) for _ in range(slots)]
         self.max_batch = max_batch
+        self.max_queue = max_queue
         self.window = window_ms * 1e-3
         self.bin_size = bin_size
         self.readback = readback
+        self.clock = clock
         self.queue: collections.deque = collections.deque()
         self.completed: List[Request] = []
         self.rejected: List[Request] = []
+        self.dropped: List[Request] = []
+        self.cancelled: List[Request] = []
+        self.requeued = 0          # batches rendered again after a capacity loss
         self.max_in_flight = 0     # bytes, for the admission check
         self.max_busy = 0          # batches on the device at once
-        self.t0 = time.perf_counter()
+        self.t0 = self.clock()
 
     def reserve(self, width: int, height: int, shadows: bool = True):
         """Grows every slot's device pools to a full batch of the largest
@@ -111,17 +144,45 @@ class RenderServer:
 
     # -- clock
     def now(self) -> float:
-        return time.perf_counter() - self.t0
+        return self.clock() - self.t0
 
     def in_flight_bytes(self) -> int:
         return sum(s.bytes for s in self.slots)
 
-    def submit(self, req: Request):
-        need = request_bytes(self.ntri[req.scene], req.width, req.height, self.bin_size)
-        if self.resident + need > self.budget:
-            self.rejected.append(req)   # can never fit
-            return
+    def _need(self, r: Request) -> int:
+        return request_bytes(self.ntri[r.scene], r.width, r.height, self.bin_size)
+
+    def submit(self, req: Request) -> bool:
+        """Queues a request; False if it is rejected (it can never fit the
+        budget) or dropped (the queue is full)."""
+        if self.resident + self._need(req) > self.budget:
+            req.state = REJECTED
+            self.rejected.append(req)
+            return False
+        if len(self.queue) >= self.max_queue:
+            req.state = DROPPED
+            self.dropped.append(req)
+            return False
+        req.state = QUEUED
         self.queue.append(req)
+        return True
+
+    def cancel(self, rid: int) -> bool:
+        """Cancels request rid: removed from the queue, or, if its batch is
+        on the device, its result is discarded when the batch ends.  False
+        if no queued or running request has that id."""
+        for r in self.queue:
+            if r.rid == rid:
+                self.queue.remove(r)
+                r.state = CANCELLED
+                self.cancelled.append(r)
+                return True
+        for s in self.slots:
+            for r in s.reqs:
+                if r.rid == rid and r.state == RUNNING:
+                    r.state = CANCELLED
+                    return True
+        return False
 
     # -- loop pieces
     def _poll(self):
@@ -130,19 +191,32 @@ class RenderServer:
                 self._finish(s)
 
     def _finish(self, s: _Slot):
+        reqs, s.reqs, s.bytes = s.reqs, [], 0
         try:
             s.batch.wait()
-        except Exception:
-            # bin storage grew after a loss: render the same jobs again
-            settle(s.batch, s.batch.jobs)
+        except AxrError as e:
+            if e.code != AXR_ECAP:
+                raise
+            # bin entries were lost and the storage has grown: the batch
+            # goes back to the head of the queue, oldest first
+            self.requeued += 1
+            for r in reversed(reqs):
+                if r.state == CANCELLED:
+                    self.cancelled.append(r)
+                    continue
+                r.state = QUEUED
+                self.queue.appendleft(r)
+            return
         t = self.now()
-        for k, r in enumerate(s.reqs):
+        for k, r in enumerate(reqs):
+            if r.state == CANCELLED:
+                self.cancelled.append(r)
+                continue
             if self.readback:
                 r.rgba = s.batch.rgba8(k)
             r.done = t
+            r.state = DONE
             self.completed.append(r)
-        s.reqs = []
-        s.bytes = 0
 
     def _launch(self, s: _Slot) -> bool:
         """Forms a batch from the queue head under the budget; False if none."""
@@ -155,8 +229,7 @@ class RenderServer:
         free = self.budget - self.resident - self.in_flight_bytes()
         take, used = [], 0
         while self.queue and len(take) < self.max_batch:
-            r = self.queue[0]
-            need = request_bytes(self.ntri[r.scene], r.width, r.height, self.bin_size)
+            need = self._need(self.queue[0])
             if used + need > free:
                 break
             take.append(self.queue.popleft())
@@ -169,6 +242,8 @@ class RenderServer:
         for r in take:
             r.start = t
             r.batch_size = len(take)
+            r.state = RUNNING
+            r.attempts += 1
         s.batch.render(jobs)
         s.reqs, s.bytes = take, used
         self.max_in_flight = max(self.max_in_flight, self.in_flight_bytes())
@@ -187,10 +262,10 @@ class RenderServer:
 
     def run(self, requests: Sequence[Request]):
         """Serves requests whose .arrival are seconds from now; returns when
-        every request is complete."""
+        every request is complete (or dropped, rejected or cancelled)."""
         pending = sorted(requests, key=lambda r: r.arrival)
         i = 0
-        self.t0 = time.perf_counter()
+        self.t0 = self.clock()
         while i < len(pending) or self.queue or any(s.busy for s in self.slots):
             t = self.now()
             while i < len(pending) and pending[i].arrival <= t:
@@ -205,6 +280,8 @@ class RenderServer:
             min((r.arrival for r in self.completed), default=0.0)
         sizes = [r.batch_size for r in self.completed]
         return {"completed": len(self.completed), "rejected": len(self.rejected),
+                "dropped": len(self.dropped), "cancelled": len(self.cancelled),
+                "requeued_batches": self.requeued,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                 "mean_batch": float(np.mean(sizes)) if sizes else 0.0,
                 "fps": len(self.completed) / span if span > 0 else 0.0,
