@@ -69,8 +69,27 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
   const uint32_t T = G.w;
   const uint32_t gi = (blockIdx.x - G.x) * kSetupBlock + threadIdx.x;
   const bool live = gi < T;
-  // the wave's 64 triangles are one chunk: a job skips it when out of view
-  const float4 sph = J0.chunks[imin((int)(gi >> 6), (int)((T - 1) >> 6))];
+  // the wave's 64 triangles are one chunk: a job skips it when out of view.
+  // The first wave tests the block's chunks against every job of the group
+  // (a lane per (job, chunk) pair) so the other waves issue none of the
+  // tests; groups of more jobs than the table holds test per wave.
+  constexpr int kWaves = kSetupBlock / 64;
+  constexpr uint32_t kVisJobs = 512;
+  __shared__ uint8_t jvis[kVisJobs * kWaves];
+  const int last_chunk = (int)((T - 1) >> 6);
+  const float4 sph = J0.chunks[imin((int)(gi >> 6), last_chunk)];
+  const bool pre = G.z <= kVisJobs;
+  if (pre) {
+    if (threadIdx.x < 64) {
+      const int c0 = (int)((blockIdx.x - G.x) * kWaves);
+      for (uint32_t k = threadIdx.x; k < kWaves * G.z; k += 64) {
+        const uint32_t q = k / kWaves;
+        const float4 cs = J0.chunks[imin(c0 + (int)(k % kWaves), last_chunk)];
+        jvis[k] = chunk_outside_view(cs, jobs[jidx[G.y + q]].dc.view) ? 0 : 1;
+      }
+    }
+    __syncthreads();
+  }
   float4 vp[3];
   uint32_t f = 0;
   if (live) {
@@ -86,7 +105,9 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
     const AxrDrawConst& dc = J.dc;
     int cls = 0;
     AxrTriRec r;
-    if (live && (f & AXR_GRP_DRAW) && !chunk_outside_view(sph, dc.view)) {
+    const bool in_view = pre ? jvis[q * kWaves + (threadIdx.x >> 6)] != 0
+                             : !chunk_outside_view(sph, dc.view);
+    if (live && (f & AXR_GRP_DRAW) && in_view) {
       cls = tri_setup(vp, J.vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
                       r.z);
       if (cls) J.rec[gi] = r;  // read by the raster through the bin entries only

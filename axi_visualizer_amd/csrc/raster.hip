@@ -401,19 +401,16 @@ __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, A
   __syncthreads();
   for_bin_entries(B, slot, n, (int)threadIdx.x, (int)blockDim.x, [&](uint32_t gi) {
     const AxrShRec r = rec[gi];
-    // texels lie in [x0, xe) x [y0, ye), inside the bin; the index clamp
-    // only keeps a stray one from leaving the LDS tile
-    constexpr unsigned kLast = AXR_SHADOW_BIN * AXR_SHADOW_BIN - 1;
+    // texels lie in [x0, xe) x [y0, ye): shadow_walk clips every row to
+    // the bin, so the tile index stays inside the LDS tile
     shadow_walk(
         r, dim, x0, y0, xe, ye,
         [&](int ax, int cy, float tz) {
           if (tz != tz) return;
-          const unsigned lc = (unsigned)(((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0));
-          atomicMin(&lk[umin(lc, kLast)], ordf(tz == 0.0f ? 0.0f : tz));
+          atomicMin(&lk[((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0)], ordf(tz == 0.0f ? 0.0f : tz));
         },
         [&](int ax, int cy, uint32_t key) {
-          const unsigned lc = (unsigned)(((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0));
-          atomicMin(&lk[umin(lc, kLast)], key);
+          atomicMin(&lk[((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0)], key);
         });
   });
   __syncthreads();
