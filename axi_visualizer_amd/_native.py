@@ -8,6 +8,9 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
+
+import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libaxr.so")
 
@@ -66,6 +69,8 @@ SIGNATURES = {
     "axr_get_ids": (_i, [_ctx, _p]),
     "axr_get_shadow_map": (_i, [_ctx, _i, _p]),
     "axr_sync": (_i, [_ctx]),
+    "axr_host_alloc": (_i, [ctypes.c_uint64, ctypes.POINTER(_p)]),
+    "axr_host_free": (_i, [_p]),
     "axr_get_stats": (_i, [_ctx, _p]),
     "axr_set_bin_capacity": (_i, [_ctx, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_uint32]),
@@ -127,6 +132,23 @@ def check(rc: int, what: str) -> int:
         msg = lib().axr_last_error()
         raise AxrError(f"{what} failed ({rc}): {msg.decode() if msg else ''}", rc)
     return rc
+
+
+def pinned_array(shape, dtype) -> np.ndarray:
+    """A numpy array in page-locked host memory (axr_host_alloc); the memory
+    is released when the array and every view of it are gone."""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    p = ctypes.c_void_p()
+    check(lib().axr_host_alloc(max(n, 1), ctypes.byref(p)), "axr_host_alloc")
+    buf = (ctypes.c_uint8 * max(n, 1)).from_address(p.value)
+    weakref.finalize(buf, _host_free, p.value)
+    return np.frombuffer(buf, dt, count=int(np.prod(shape))).reshape(shape)
+
+
+def _host_free(addr: int):
+    if _lib is not None:
+        _lib.axr_host_free(ctypes.c_void_p(addr))
 
 
 def ptr(a) -> int:

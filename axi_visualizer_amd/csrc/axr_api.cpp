@@ -1066,14 +1066,11 @@ int axr_get_rgba8(axr_ctx* c, uint32_t* rgba) {
     return check_overflow(c, true);
   }
   if (c->state == ST_LDR16) {
-    std::vector<uint16_t> r(n), g(n), b(n);
-    HIPCHK(hipMemcpyAsync(r.data(), c->d_r, n * 2, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(g.data(), c->d_g, n * 2, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(b.data(), c->d_b, n * 2, hipMemcpyDeviceToHost, c->stream));
+    // the planes hold the gamma'd colour (post passes after gamma): pack
+    // them on the device, so only the packed frame crosses the link
+    HIPCHK(launch_pack_rgba8(c->d_r, c->d_g, c->d_b, c->d_rgba, n, c->stream));
+    HIPCHK(hipMemcpyAsync(rgba, c->d_rgba, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < n; i++)
-      rgba[i] = (uint32_t)(r[i] & 0xFF) | ((uint32_t)(g[i] & 0xFF) << 8) |
-                ((uint32_t)(b[i] & 0xFF) << 16) | 0xFF000000u;
     return check_overflow(c, true);
   }
   return fail(AXR_ESTATE, "getRGBA8 needs gamma() first");
@@ -1111,6 +1108,19 @@ int axr_get_shadow_map(axr_ctx* c, int i, float* out) {
   dfree(d);
   if (e != hipSuccess) return fail(AXR_EHIP, "getSHM: %s", hipGetErrorString(e));
   return check_overflow(c, false, i);
+}
+
+int axr_host_alloc(uint64_t bytes, void** out) {
+  if (!out) return fail(AXR_EINVAL, "null out pointer");
+  *out = nullptr;
+  if (bytes == 0) return fail(AXR_EINVAL, "host_alloc: zero bytes");
+  HIPCHK(hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault));
+  return AXR_OK;
+}
+
+int axr_host_free(void* p) {
+  if (p) HIPCHK(hipHostFree(p));
+  return AXR_OK;
 }
 
 int axr_sync(axr_ctx* c) {

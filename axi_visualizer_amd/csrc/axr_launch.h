@@ -60,6 +60,8 @@ hipError_t launch_gamma_planes(uint16_t* r, uint16_t* g, uint16_t* b, size_t n, 
                                hipStream_t s);
 hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, uint16_t* b,
                                size_t n, hipStream_t s);
+hipError_t launch_pack_rgba8(const uint16_t* r, const uint16_t* g, const uint16_t* b,
+                             uint32_t* rgba, size_t n, hipStream_t s);
 // Smallest-first mip chain of `levels` levels whose last (largest) level
 // is filled: builds the others on the device.
 hipError_t launch_create_mips(ushort4* chain, int levels, hipStream_t s);

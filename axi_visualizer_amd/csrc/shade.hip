@@ -110,6 +110,16 @@ __global__ void k_expand_rgba8(const uint32_t* __restrict__ rgba, uint16_t* __re
   }
 }
 
+// Gamma'd u16 planes (values 0..255) packed into RGBA8 for readback.
+__global__ void k_pack_rgba8(const uint16_t* __restrict__ r, const uint16_t* __restrict__ g,
+                             const uint16_t* __restrict__ b, uint32_t* __restrict__ rgba,
+                             size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    rgba[i] = (uint32_t)(r[i] & 0xFF) | ((uint32_t)(g[i] & 0xFF) << 8) |
+              ((uint32_t)(b[i] & 0xFF) << 16) | 0xFF000000u;
+}
+
 __global__ void k_decode_shadow(const uint32_t* __restrict__ map, float* __restrict__ out,
                                 size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -227,6 +237,13 @@ hipError_t launch_expand_rgba8(const uint32_t* rgba, uint16_t* r, uint16_t* g, u
                                size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_expand_rgba8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rgba, r, g,
                      b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_rgba8(const uint16_t* r, const uint16_t* g, const uint16_t* b,
+                             uint32_t* rgba, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_rgba8, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, r, g, b, rgba,
+                     n);
   return hipGetLastError();
 }
 

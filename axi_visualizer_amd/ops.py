@@ -20,7 +20,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._native import AxrError, check, lib, ptr
+from ._native import AxrError, check, lib, pinned_array, ptr
 from .scenes import SH_CULL, SH_MIP, SH_PHONG, SH_SHADOW, view_mat
 
 _UNSUPPORTED = ("alpha", "multi", "sky", "SSR", "refl", "sub", "emissive", "normal",
@@ -85,6 +85,8 @@ class CLDraw:
         self._points = (0, [None, None])
         self._spots = (0, [None, None, None])
         self._flags_buf = None
+        self._hframe = None   # getFrame's arrays (page-locked, reused)
+        self._hrgba = None    # getRGBA8's array
         self._mask_buf = None
         self._sid_buf = np.zeros(2, np.int32)
         self.reflTexSize = {}
@@ -290,21 +292,25 @@ class CLDraw:
                                     float(strength)), "bloom")
 
     def getFrame(self):
-        """Ops.py:1224-1230: (R, G, B) u16 and depth f32, each (H, W)."""
-        H, W = int(self.H), int(self.W)
-        r = np.empty((H, W), np.uint16)
-        g = np.empty((H, W), np.uint16)
-        b = np.empty((H, W), np.uint16)
-        d = np.empty((H, W), np.float32)
+        """Ops.py:1224-1230: (R, G, B) u16 and depth f32, each (H, W).  As in
+        the reference (Ops.py:216-219), the same four arrays come back from
+        every call, overwritten by the next one; they live in page-locked
+        memory, so the copy runs at the link's DMA rate."""
+        if self._hframe is None:
+            H, W = int(self.H), int(self.W)
+            self._hframe = tuple(pinned_array((H, W), t)
+                                 for t in (np.uint16, np.uint16, np.uint16, np.float32))
+        r, g, b, d = self._hframe
         self._chk(self._L.axr_get_frame(self._ctx, ptr(r), ptr(g), ptr(b), ptr(d)), "getFrame")
-        return (r, g, b, d)
+        return self._hframe
 
     def getRGBA8(self):
-        """The gamma'd frame as (H, W, 4) uint8."""
-        H, W = int(self.H), int(self.W)
-        out = np.empty((H, W), np.uint32)
-        self._chk(self._L.axr_get_rgba8(self._ctx, ptr(out)), "getRGBA8")
-        return out.view(np.uint8).reshape(H, W, 4)
+        """The gamma'd frame as (H, W, 4) uint8: one page-locked array,
+        overwritten by the next call (as getFrame's are)."""
+        if self._hrgba is None:
+            self._hrgba = pinned_array((int(self.H), int(self.W)), np.uint32)
+        self._chk(self._L.axr_get_rgba8(self._ctx, ptr(self._hrgba)), "getRGBA8")
+        return self._hrgba.view(np.uint8).reshape(int(self.H), int(self.W), 4)
 
     def getIds(self):
         """Depth-winning global triangle index per pixel (0xFFFFFFFF: none)."""

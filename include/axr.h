@@ -150,6 +150,14 @@ int axr_bloom_planes_noalloc(uint16_t* r, uint16_t* g, uint16_t* b, int w, int h
 int axr_get_frame(axr_ctx* ctx, uint16_t* r, uint16_t* g, uint16_t* b, float* depth);
 /* The frame as packed RGBA8 (after gamma), [h][w] u32. */
 int axr_get_rgba8(axr_ctx* ctx, uint32_t* rgba);
+
+/* Page-locked host memory (hipHostMalloc) for readback destinations:
+ * device-to-host copies into it run at the link's DMA rate, where pageable
+ * destinations are staged and fault their pages in.  No counterpart in the
+ * reference (its readback arrays, Ops.py:216-219, are pageable numpy
+ * arrays allocated once); CLDraw keeps its getFrame/getRGBA8 arrays here. */
+int axr_host_alloc(uint64_t bytes, void** out);
+int axr_host_free(void* p);
 /* Depth-winning global triangle index per pixel (0xFFFFFFFF: none);
  * global index = sum of earlier groups' ntri + index within the group. */
 int axr_get_ids(axr_ctx* ctx, uint32_t* ids);

@@ -192,3 +192,22 @@ def test_vertlight_none_keeps_the_point_and_spot_lights():
         assert np.array_equal(d.getVertLight(0), only_dirs)
     finally:
         r.close()
+
+
+def test_readback_arrays_are_reused_and_page_locked():
+    """getFrame returns the same four arrays every call, overwritten by the
+    next one (Ops.py:216-219, 1224-1230), and getRGBA8 one array; both stay
+    valid after the CLDraw is closed."""
+    sc = small_spheres()
+    r = Renderer(sc)
+    try:
+        r.frame()
+        a = r.draw.getFrame()
+        b = r.draw.getFrame()
+        assert all(x is y for x, y in zip(a, b))
+        rgba = r.draw.getRGBA8()
+        assert np.shares_memory(rgba, r.draw.getRGBA8())
+        keep = rgba.copy()
+    finally:
+        r.close()
+    assert np.array_equal(rgba, keep)  # memory outlives the context

@@ -27,7 +27,7 @@ def test_full_frame_at_1440p_matches_oracle():
     ff = FullFrame(sc, probe, cube_size=512)
     try:
         ff.frame(gamma=True)
-        got = ff.draw.getFrame()
+        got = [p.copy() for p in ff.draw.getFrame()]
         maps = [ff.draw.getSHM(i) for i in range(len(sc.cascades))]
         cube_gpu = ff.draw.getReflTex(0)
         ff.frame(gamma=True)

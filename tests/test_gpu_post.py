@@ -28,7 +28,7 @@ def test_blur_on_pending_hdr_frame():
     r = Renderer(spheres())
     try:
         r.frame(gamma=False)
-        before = r.draw.getFrame()[:3]
+        before = [p.copy() for p in r.draw.getFrame()[:3]]
         r.frame(gamma=False)  # same frame again, now blurred straight from the draw
         r.draw.blur()
         after = r.draw.getFrame()[:3]
@@ -42,9 +42,9 @@ def test_blur_after_gamma_and_twice():
     r = Renderer(spheres())
     try:
         r.frame(gamma=True)
-        p0 = r.draw.getFrame()[:3]
+        p0 = [p.copy() for p in r.draw.getFrame()[:3]]
         r.draw.blur()
-        p1 = r.draw.getFrame()[:3]
+        p1 = [p.copy() for p in r.draw.getFrame()[:3]]
         same(p1, pyoracle.blur(*p0), "blur after gamma")
         r.draw.blur()
         same(r.draw.getFrame()[:3], pyoracle.blur(*p1), "second blur")
@@ -57,7 +57,7 @@ def test_bloom_on_rendered_frame(levels, threshold, strength):
     r = Renderer(spheres())
     try:
         r.frame(gamma=False)
-        before = r.draw.getFrame()[:3]
+        before = [p.copy() for p in r.draw.getFrame()[:3]]
         r.draw.bloom(levels=levels, threshold=threshold, white=9835.0, strength=strength)
         after = r.draw.getFrame()[:3]
         want = pyoracle.bloom(*before, levels=levels, threshold=threshold, white=9835.0,
@@ -72,7 +72,7 @@ def test_blur_cleared_frame_edge_sizes(W, H):
     d = CLDraw(1, 1, W, H, 0)
     try:
         d.clearZBuffer()
-        before = d.getFrame()[:3]
+        before = [p.copy() for p in d.getFrame()[:3]]
         d.clearZBuffer()
         d.blur()
         same(d.getFrame()[:3], pyoracle.blur(*before), f"blur {W}x{H}")

@@ -205,7 +205,7 @@ def test_full_frame_pipeline_matches_oracle_composition():
     ff = FullFrame(sc, probe, cube_size=32)
     try:
         ff.frame(gamma=True)
-        got = ff.draw.getFrame()
+        got = [p.copy() for p in ff.draw.getFrame()]
         ff.frame(gamma=True)  # a second frame: same pixels
         again = ff.draw.getFrame()
     finally:
