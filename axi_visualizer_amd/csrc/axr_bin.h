@@ -337,7 +337,7 @@ __device__ __forceinline__ bool behind_bin(const float z[3], float zfar) {
 // on their reference tile (a9).  FIRST: the key's low word is seq, so the
 // first fragment in draw order wins a depth tie (a strict test); otherwise
 // ~seq, the last one wins (the reference's F >= tz).
-template <int SHIFT, bool FIRST>
+template <int SHIFT, bool FIRST, bool DEAL = false>
 __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec* __restrict__ rec,
                                          const AxrBins& B, uint32_t s_small, uint32_t s_large,
                                          int bx, int by, const AxrDrawConst& dc, int tid,
@@ -404,7 +404,11 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
     const unsigned lc = (unsigned)(((cy - y0) << SHIFT) + (ax - x0));
     if (lc < (unsigned)NPX) atomicMin(&lk[lc], key(tz, seq));
   };
-  for_bin_entries_waves(B, s_large, B.num[s_large], tid, nthr, [&](uint32_t v, bool valid) {
+  // a bin holds few large-triangle tiles, each walked by the whole wave
+  // (coop_tile_walk): DEAL spreads them over the waves so they walk in
+  // parallel -- shorter bins when the device has idle waves (a single
+  // frame), more instructions when it has none (the persistent batch)
+  for_bin_entries_waves<DEAL>(B, s_large, B.num[s_large], tid, nthr, [&](uint32_t v, bool valid) {
     Geom<float> g{};
     TileBox box{};
     uint32_t order = 0;
@@ -443,7 +447,7 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
 // threads tid of nthr work on it (a block may hold two bins, each with half
 // of its threads: every group passes the same barriers, an inactive group
 // with no work).
-template <int SHIFT, int MODE>
+template <int SHIFT, int MODE, bool DEAL = false>
 __device__ __forceinline__ void raster_bin(
     unsigned long long* lk, uint16_t* list, uint32_t* ncov, const AxrTriRec* __restrict__ rec,
     const AxrBins& B, uint32_t s_small, uint32_t s_large, int empty, int bx, int by,
@@ -464,7 +468,7 @@ __device__ __forceinline__ void raster_bin(
   for (int i = tid; i < NPX; i += nthr) lk[i] = key_clear();
   if (tid == 0) *ncov = 0;
   __syncthreads();
-  if (!empty) walk_bin<SHIFT, false>(lk, rec, B, s_small, s_large, bx, by, dc, tid, nthr);
+  if (!empty) walk_bin<SHIFT, false, DEAL>(lk, rec, B, s_small, s_large, bx, by, dc, tid, nthr);
   __syncthreads();
   // nthr is a multiple of the wave size, so every lane of a wave runs the
   // same rounds and the covered pixels are appended with one LDS atomic a

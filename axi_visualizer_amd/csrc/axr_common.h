@@ -193,6 +193,16 @@ __device__ __forceinline__ void bin_store(const AxrBins& B, uint32_t s, uint32_t
   }
 }
 
+// The position of thread tid within a round of nthr entries (nthr a
+// multiple of the wave size) when the entries are dealt to the waves in
+// turn: a bin with fewer entries than threads then spreads them over every
+// wave (each walking a few) instead of filling the first waves while the
+// others wait at the next barrier.
+__device__ __forceinline__ uint32_t dealt_slot(int tid, int nthr) {
+  const int nw = nthr >> 6;
+  return (uint32_t)((tid & 63) * nw + (tid >> 6));
+}
+
 // f(entry) for the n entries of slot s, strided over the block's threads:
 // the bucket, then -- only when it overflowed -- the slot's spilled entries.
 template <class F>
@@ -201,10 +211,11 @@ __device__ __forceinline__ void for_bin_entries(const AxrBins& B, uint32_t s, ui
   const uint32_t cap = s < (uint32_t)B.nb ? B.cap0 : B.cap1;
   const uint32_t* __restrict__ ent = B.ent + bin_base(B, s);
   const uint32_t nb = n < cap ? n : cap;
-  for (uint32_t e = (uint32_t)tid; e < nb; e += (uint32_t)nthr) f(ent[e]);
+  const uint32_t k = (uint32_t)tid;
+  for (uint32_t e = k; e < nb; e += (uint32_t)nthr) f(ent[e]);
   if (n > cap) {
     const uint32_t ns = B.spill_ctr[1];
-    for (uint32_t q = (uint32_t)tid; q < ns; q += (uint32_t)nthr) {
+    for (uint32_t q = k; q < ns; q += (uint32_t)nthr) {
       const uint2 p = B.spill[q];
       if (p.x == s) f(p.y);
     }
@@ -213,21 +224,24 @@ __device__ __forceinline__ void for_bin_entries(const AxrBins& B, uint32_t s, ui
 
 // f(entry, valid) for the n entries of slot s in rounds of the block's
 // threads, every lane of every wave calling f in every round (valid marks
-// lanes holding an entry), so f may use wave-wide operations.
-template <class F>
+// lanes holding an entry), so f may use wave-wide operations.  DEAL: the
+// entries of a round go to the waves in turn (dealt_slot), else each wave
+// takes 64 consecutive ones.
+template <bool DEAL = false, class F>
 __device__ __forceinline__ void for_bin_entries_waves(const AxrBins& B, uint32_t s, uint32_t n,
                                                       int tid, int nthr, F&& f) {
   const uint32_t cap = s < (uint32_t)B.nb ? B.cap0 : B.cap1;
   const uint32_t* __restrict__ ent = B.ent + bin_base(B, s);
   const uint32_t nb = n < cap ? n : cap;
+  const uint32_t k = DEAL ? dealt_slot(tid, nthr) : (uint32_t)tid;
   for (uint32_t e0 = 0; e0 < nb; e0 += (uint32_t)nthr) {
-    const uint32_t e = e0 + (uint32_t)tid;
+    const uint32_t e = e0 + k;
     f(e < nb ? ent[e] : 0u, e < nb);
   }
   if (n > cap) {
     const uint32_t ns = B.spill_ctr[1];
     for (uint32_t q0 = 0; q0 < ns; q0 += (uint32_t)nthr) {
-      const uint32_t q = q0 + (uint32_t)tid;
+      const uint32_t q = q0 + k;
       bool ok = false;
       uint32_t v = 0;
       if (q < ns) {

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
       nthr = QUARTER;
     }
   }
-  raster_bin<AXR_BIN_SHIFT, MODE>(lk[h], list[h], &ncov[h], rec, B, (uint32_t)b,
+  raster_bin<AXR_BIN_SHIFT, MODE, true>(lk[h], list[h], &ncov[h], rec, B, (uint32_t)b,
                                   (uint32_t)(B.nb + b), empty, b % NBX, b / NBX, vpos, vattr,
                                   groups, dc, exposure, depth, ro, go, bo, rgba, ids, t, nthr,
                                   active);

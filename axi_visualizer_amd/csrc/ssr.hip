@@ -201,7 +201,7 @@ __global__ __launch_bounds__(AXR_SSR_THREADS) void k_ssr_raster(
   __syncthreads();
   if ((tid & 63) == 0) atomicMax(&bin_far, __float_as_uint(zfar));  // non-negative: bits order
   __syncthreads();
-  walk_bin<AXR_BIN_SHIFT, true>(lk, rec, B, (uint32_t)b, (uint32_t)(B.nb + b), bx, by, dc, tid,
+  walk_bin<AXR_BIN_SHIFT, true, true>(lk, rec, B, (uint32_t)b, (uint32_t)(B.nb + b), bx, by, dc, tid,
                                 nthr, __uint_as_float(bin_far));
   __syncthreads();
   for (int i = tid; i < NPX; i += nthr) {
