@@ -98,9 +98,16 @@ hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipSt
 // Batched jobs (batch.hip).  bin_shift 3/4/5 = 8/16/32-px bins.
 // groups: per scene of the batch (first setup block, offset of its job
 // indices in jidx, job count, triangles), ascending in first block.
+// large: (job, triangle) pairs of the large triangles, counted in
+// large_ctl[0] (reset by the order pass); pairs past large_cap add to
+// ctr[AXR_CTR_LOST].
 hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, const uint4* groups,
                               int ngroups, const uint32_t* jidx, uint32_t blocks,
-                              const AxrBins& B, hipStream_t s);
+                              const AxrBins& B, uint2* large, uint32_t* large_ctl,
+                              uint32_t large_cap, uint32_t* ctr, hipStream_t s);
+hipError_t launch_batch_coarse(int bin_shift, const AxrJobDev* jobs, const uint2* large,
+                               const uint32_t* large_ctl, uint32_t large_cap, const AxrBins& B,
+                               int grid, hipStream_t s);
 // mode: a ShadeMode (RGBA8 with gamma, winner ids, or HDR planes).
 hipError_t launch_batch_raster(int bin_shift, int mode, const AxrJobDev* jobs, int njobs,
                                const AxrBins& B, int grid, hipStream_t s);

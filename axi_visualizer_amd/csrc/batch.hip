@@ -63,7 +63,10 @@ template <int SHIFT>
 __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __restrict__ jobs,
                                                              const uint4* __restrict__ sg, int nsg,
                                                              const uint32_t* __restrict__ jidx,
-                                                             AxrBins B) {
+                                                             AxrBins B, uint2* __restrict__ large,
+                                                             uint32_t* __restrict__ large_ctl,
+                                                             uint32_t large_cap,
+                                                             uint32_t* __restrict__ ctr) {
   const uint4 G = sg[group_of_block(sg, nsg, blockIdx.x)];
   const AxrJobDev& J0 = jobs[jidx[G.y]];
   const uint32_t T = G.w;
@@ -127,11 +130,47 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
       if (act) bin_store(B, slot, pos, gi);
       bc.next(br);
     }
+    // large triangles are binned by k_batch_coarse, a block per (job,
+    // triangle); past the list's capacity they count as lost entries (the
+    // wait reports them and the list grows)
+    const uint32_t at = wave_bin_add(large_ctl, cls == 2, 0u);
     if (cls == 2) {
-      coarse_walk(r.px, r.py, dc.WC, dc.HC, [&](int t) {
-        append_tile<SHIFT>(B, (uint32_t)B.nb + J.slot0, t, dc.WC, J.nbx, gi);
-      });
+      if (at < large_cap)
+        large[at] = make_uint2(jidx[G.y + q], gi);
+      else
+        atomicAdd(&ctr[AXR_CTR_LOST], 1u);
     }
+  }
+}
+
+// Coarse binning of the batch's large triangles (Pipe/coarse.c), a block
+// per (job, triangle) of the list, each tile row to a group of threads
+// (coarse_block_tiles), as the single-frame k_coarse does; the tiles go to
+// the job's large-tile bins at this batch's bin size (append_tile).  The
+// order pass that follows resets the list.
+constexpr int kCoarseThreads = 256;
+template <int SHIFT>
+__global__ __launch_bounds__(kCoarseThreads) void k_batch_coarse(
+    const AxrJobDev* __restrict__ jobs, const uint2* __restrict__ large,
+    const uint32_t* __restrict__ large_ctl, uint32_t large_cap, AxrBins B) {
+  const uint32_t n = umin(large_ctl[0], large_cap);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint2 e = large[i];
+    const AxrJobDev& J = jobs[e.x];
+    const uint32_t gi = e.y;
+    const AxrTriRec r = J.rec[gi];
+    const int WC = J.dc.WC;
+    coarse_block_tiles(r.px, r.py, WC, J.dc.HC, (int)threadIdx.x, kCoarseThreads,
+                       [&](int cy, int ax, int G, float xe) {
+                         // columns in [0, WC - 1], cy in [0, HC - 1]
+#pragma unroll
+                         for (int j = 0; j < 8; j++) {
+                           const int tx = ax + j * G;
+                           if ((float)tx <= xe)
+                             append_tile<SHIFT>(B, (uint32_t)B.nb + J.slot0, cy * WC + tx, WC,
+                                                J.nbx, gi);
+                         }
+                       });
   }
 }
 
@@ -184,20 +223,40 @@ namespace axr {
 
 hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, const uint4* groups,
                               int ngroups, const uint32_t* jidx, uint32_t blocks,
-                              const AxrBins& B, hipStream_t s) {
+                              const AxrBins& B, uint2* large, uint32_t* large_ctl,
+                              uint32_t large_cap, uint32_t* ctr, hipStream_t s) {
   if (ngroups == 0 || blocks == 0) return hipSuccess;
   switch (bin_shift) {
     case 3:
       hipLaunchKernelGGL(k_batch_setup<3>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, groups,
-                         ngroups, jidx, B);
+                         ngroups, jidx, B, large, large_ctl, large_cap, ctr);
       break;
     case 4:
       hipLaunchKernelGGL(k_batch_setup<4>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, groups,
-                         ngroups, jidx, B);
+                         ngroups, jidx, B, large, large_ctl, large_cap, ctr);
       break;
     default:
       hipLaunchKernelGGL(k_batch_setup<5>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, groups,
-                         ngroups, jidx, B);
+                         ngroups, jidx, B, large, large_ctl, large_cap, ctr);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_coarse(int bin_shift, const AxrJobDev* jobs, const uint2* large,
+                               const uint32_t* large_ctl, uint32_t large_cap, const AxrBins& B,
+                               int grid, hipStream_t s) {
+  switch (bin_shift) {
+    case 3:
+      hipLaunchKernelGGL(k_batch_coarse<3>, dim3(grid), dim3(kCoarseThreads), 0, s, jobs, large,
+                         large_ctl, large_cap, B);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_batch_coarse<4>, dim3(grid), dim3(kCoarseThreads), 0, s, jobs, large,
+                         large_ctl, large_cap, B);
+      break;
+    default:
+      hipLaunchKernelGGL(k_batch_coarse<5>, dim3(grid), dim3(kCoarseThreads), 0, s, jobs, large,
+                         large_ctl, large_cap, B);
   }
   return hipGetLastError();
 }

@@ -16,6 +16,9 @@
 namespace {
 
 constexpr uint32_t kBatchSpill = 1u << 20;
+// (job, triangle) pairs of large triangles a render can list before its
+// coarse pass; grown like the spill list when a render loses some
+constexpr uint32_t kBatchLarge = 1u << 16;
 
 struct JobInfo {
   axr_ctx* scene;
@@ -42,6 +45,9 @@ struct axr_batch {
   int occupancy[3][3] = {};  // raster blocks per CU by (bin shift - 3, shade mode)
   BinStore bins;
   uint32_t* d_head = nullptr;
+  uint2* d_large = nullptr;        // large triangles of the render: (job, triangle)
+  uint32_t* d_large_ctl = nullptr;  // [0]: pairs listed (reset by the order pass)
+  uint32_t large_cap = 0, want_large = kBatchLarge;
   uint32_t* d_counters = nullptr;
   // pinned copy of d_counters, written on the stream after every render so
   // that a wait never issues a copy of its own: a synchronous copy would
@@ -206,9 +212,12 @@ int enqueue(axr_batch* b) {
   const int grid = b->nb_total < grid_max ? b->nb_total : grid_max;
   if (b->timing) HIPCHK(hipEventRecord(b->ev_k[0], b->stream));
   HIPCHK(launch_batch_setup(b->bin_shift, b->d_jobs, (const uint4*)b->d_sg, b->nsg,
-                            b->d_sg + 4 * (size_t)b->njobs, b->setup_blocks, B, b->stream));
+                            b->d_sg + 4 * (size_t)b->njobs, b->setup_blocks, B, b->d_large,
+                            b->d_large_ctl, b->large_cap, b->d_counters, b->stream));
+  HIPCHK(launch_batch_coarse(b->bin_shift, b->d_jobs, b->d_large, b->d_large_ctl, b->large_cap,
+                             B, b->cu_count * 2, b->stream));
   if (b->timing) HIPCHK(hipEventRecord(b->ev_k[1], b->stream));
-  HIPCHK(launch_bin_order(&B, nullptr, b->d_counters, nullptr, nullptr, b->stream));
+  HIPCHK(launch_bin_order(&B, nullptr, b->d_counters, b->d_large_ctl, nullptr, b->stream));
   if (b->timing) HIPCHK(hipEventRecord(b->ev_k[2], b->stream));
   HIPCHK(launch_batch_raster(b->bin_shift, b->mode, b->d_jobs, b->njobs, B, grid, b->stream));
   if (b->timing) HIPCHK(hipEventRecord(b->ev_k[3], b->stream));
@@ -237,12 +246,15 @@ int axr_batch_create(axr_batch** out) {
       hipEventCreateWithFlags(&b->ev_copied, hipEventDisableTiming) != hipSuccess)
     return bail(fail(AXR_EHIP, "stream/event creation failed"));
   int rc;
-  if ((rc = dalloc(&b->d_head, 1)) || (rc = dalloc(&b->d_counters, AXR_CTR_WORDS))) return bail(rc);
+  if ((rc = dalloc(&b->d_head, 1)) || (rc = dalloc(&b->d_counters, AXR_CTR_WORDS)) ||
+      (rc = dalloc(&b->d_large_ctl, 2)))
+    return bail(rc);
   if (hipHostMalloc((void**)&b->h_ctr, sizeof(uint32_t) * AXR_CTR_WORDS) != hipSuccess)
     return bail(fail(AXR_EHIP, "pinned counter allocation failed"));
   std::memset(b->h_ctr, 0, sizeof(uint32_t) * AXR_CTR_WORDS);
   if (launch_fill_u32(b->d_counters, AXR_CTR_WORDS, 0, b->stream) != hipSuccess ||
       launch_fill_u32(b->d_head, 1, 0, b->stream) != hipSuccess ||
+      launch_fill_u32(b->d_large_ctl, 2, 0, b->stream) != hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess)
     return bail(fail(AXR_EHIP, "initial fill failed"));
   b->bins.nseg = 2;
@@ -260,6 +272,8 @@ int axr_batch_destroy(axr_batch* b) {
   free_graph(b);
   bins_free(b->bins);
   dfree(b->d_head);
+  dfree(b->d_large);
+  dfree(b->d_large_ctl);
   dfree(b->d_counters);
   dfree(b->d_rec);
   dfree(b->d_depth);
@@ -416,6 +430,14 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
   if ((rc = grow_pools(b, tris ? tris : 1, px ? px : 1, njobs ? njobs : 1,
                        b->mode == SHADE_HDR)))
     return rc;
+  if (b->large_cap < b->want_large) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    dfree(b->d_large);
+    b->large_cap = 0;
+    if ((rc = dalloc(&b->d_large, b->want_large))) return rc;
+    b->large_cap = b->want_large;
+    free_graph(b);
+  }
   const AxrBins before = b->bins.B;
   if ((rc = bins_prepare(b->bins, slots ? (int)slots : 1, b->stream))) return rc;
   b->bins.B.nb = (int)slots;
@@ -496,6 +518,12 @@ int axr_batch_wait(axr_batch* b, float* gpu_ms) {
   std::memcpy(h, (const void*)b->h_ctr, sizeof h);
   if (h[AXR_CTR_BIN_SPILL] || h[AXR_CTR_LOST]) {
     bins_grow(b->bins, h[AXR_CTR_BIN_SPILL], h[AXR_CTR_LOST]);
+    // a loss may come from the large-triangle list too: grow it as well
+    if (h[AXR_CTR_LOST]) {
+      const uint64_t want = std::max<uint64_t>(2ull * b->large_cap,
+                                               (uint64_t)b->large_cap + h[AXR_CTR_LOST]);
+      b->want_large = (uint32_t)std::min<uint64_t>(want, 1ull << 30);
+    }
     HIPCHK(launch_fill_u32(b->d_counters, AXR_CTR_WORDS, 0, b->stream));
     std::memset(b->h_ctr, 0, sizeof h);
     if (h[AXR_CTR_LOST]) {
