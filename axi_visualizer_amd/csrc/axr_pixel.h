@@ -19,16 +19,6 @@ __device__ __forceinline__ int find_group(const AxrGroupDev* __restrict__ groups
   return lo;
 }
 
-// Vertex k of three, by selects (a dynamic index would place the array in
-// scratch memory).
-__device__ __forceinline__ Attr pick3(const Attr& a0, const Attr& a1, const Attr& a2, int k) {
-  Attr r;
-#define M(f) r.f = k == 0 ? a0.f : (k == 1 ? a1.f : a2.f);
-  AXR_ATTR_FIELDS(M)
-#undef M
-  return r;
-}
-
 // A rebuilt fragment: its group, mip level and the span interpolants.
 struct FragState {
   AxrGroupDev grp;
@@ -56,17 +46,17 @@ __device__ __forceinline__ void replay_frag(uint32_t seq, int x, int y,
   const float* px = tr.px;
   const float* py = tr.py;
   const float* z = tr.z;
-  const Attr A0 = load_attr(vpos, vattr, 3 * gi, z[0]);
-  const Attr A1 = load_attr(vpos, vattr, 3 * gi + 1, z[1]);
-  const Attr A2 = load_attr(vpos, vattr, 3 * gi + 2, z[2]);
+  // the vertices are loaded in the walk's sorted order (k1, k2, k3 with
+  // their sorted reciprocal depths): no per-attribute selects
 
   float cx1, cx2, cz1, cz2;
   Attr c1, c2;
   MipInfo mi = {0, 0.0f, 0, 0, 0, 0};
   if (!large) {
     const Geom<int> g = small_geom(px, py, z);
-    Attr a1 = pick3(A0, A1, A2, g.k1), a2 = pick3(A0, A1, A2, g.k2),
-         a3 = pick3(A0, A1, A2, g.k3);
+    Attr a1 = load_attr(vpos, vattr, 3 * gi + (uint32_t)g.k1, g.z1);
+    Attr a2 = load_attr(vpos, vattr, 3 * gi + (uint32_t)g.k2, g.z2);
+    Attr a3 = load_attr(vpos, vattr, 3 * gi + (uint32_t)g.k3, g.z3);
     const Attr a4 = a_lerp4(a1, a3, g.yd);
     if (grp.flags & AXR_SH_MIP) mi = mip_setup(a1, a2, a3, g, grp.tex_size, grp.mip_bias);
     const Edges e = small_edges(g, half);
@@ -95,8 +85,9 @@ __device__ __forceinline__ void replay_frag(uint32_t seq, int x, int y,
     }
   } else {
     const Geom<float> g = tile_geom(px, py, z);
-    Attr a1 = pick3(A0, A1, A2, g.k1), a2 = pick3(A0, A1, A2, g.k2),
-         a3 = pick3(A0, A1, A2, g.k3);
+    Attr a1 = load_attr(vpos, vattr, 3 * gi + (uint32_t)g.k1, g.z1);
+    Attr a2 = load_attr(vpos, vattr, 3 * gi + (uint32_t)g.k2, g.z2);
+    Attr a3 = load_attr(vpos, vattr, 3 * gi + (uint32_t)g.k3, g.z3);
     const Attr a4 = a_lerp4(a1, a3, g.yd);
     if (grp.flags & AXR_SH_MIP) mi = mip_setup(a1, a2, a3, g, grp.tex_size, grp.mip_bias);
     const TileBox box = tile_box((y >> 4) * dc.WC + (x >> 4), dc.W);
