@@ -85,6 +85,8 @@ SIGNATURES = {
     "axr_batch_set_spill_capacity": (_i, [_batch, _u32]),
     "axr_batch_set_timing": (_i, [_batch, _i]),
     "axr_batch_get_timing": (_i, [_batch, _p, _p]),
+    "axr_batch_set_stats": (_i, [_batch, _i]),
+    "axr_batch_get_visible": (_i, [_batch, _p, _i]),
     "axr_batch_render": (_i, [_batch, ctypes.POINTER(AxrJob), _i, _i]),
     "axr_batch_query": (_i, [_batch]),
     "axr_batch_wait": (_i, [_batch, ctypes.POINTER(ctypes.c_float)]),

@@ -152,6 +152,19 @@ class Batch:
         return {"setup_ms": ms[0], "order_ms": ms[1], "raster_ms": ms[2],
                 "bin_entries": int(n.value)}
 
+    def visible_triangles(self, jobs) -> np.ndarray:
+        """Renders the jobs once as a statistics render and returns the
+        triangles each job's setup accepted."""
+        check(self._L.axr_batch_set_stats(self._b, 1), "axr_batch_set_stats")
+        try:
+            self.render(jobs)
+            out = np.zeros(len(self.jobs), np.uint32)
+            check(self._L.axr_batch_get_visible(self._b, ptr(out), len(self.jobs)),
+                  "axr_batch_get_visible")
+        finally:
+            check(self._L.axr_batch_set_stats(self._b, 0), "axr_batch_set_stats")
+        return out
+
     def set_spill_capacity(self, spill: int):
         """Entries of the spill list from the next render on (it grows by
         itself after a render that lost entries)."""

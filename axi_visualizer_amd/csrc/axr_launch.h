@@ -100,11 +100,13 @@ hipError_t launch_decode_shadow(const uint32_t* map, float* out, size_t n, hipSt
 // indices in jidx, job count, triangles), ascending in first block.
 // large: (job, triangle) pairs of the large triangles, counted in
 // large_ctl[0] (reset by the order pass); pairs past large_cap add to
-// ctr[AXR_CTR_LOST].
+// ctr[AXR_CTR_LOST].  visible (may be null): per-job counts of the
+// triangles the setup accepts, added to.
 hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, const uint4* groups,
                               int ngroups, const uint32_t* jidx, uint32_t blocks,
                               const AxrBins& B, uint2* large, uint32_t* large_ctl,
-                              uint32_t large_cap, uint32_t* ctr, hipStream_t s);
+                              uint32_t large_cap, uint32_t* ctr, uint32_t* visible,
+                              hipStream_t s);
 hipError_t launch_batch_coarse(int bin_shift, const AxrJobDev* jobs, const uint2* large,
                                const uint32_t* large_ctl, uint32_t large_cap, const AxrBins& B,
                                int grid, hipStream_t s);

@@ -66,7 +66,8 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
                                                              AxrBins B, uint2* __restrict__ large,
                                                              uint32_t* __restrict__ large_ctl,
                                                              uint32_t large_cap,
-                                                             uint32_t* __restrict__ ctr) {
+                                                             uint32_t* __restrict__ ctr,
+                                                             uint32_t* __restrict__ visible) {
   const uint4 G = sg[group_of_block(sg, nsg, blockIdx.x)];
   const AxrJobDev& J0 = jobs[jidx[G.y]];
   const uint32_t T = G.w;
@@ -133,6 +134,10 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
     // large triangles are binned by k_batch_coarse, a block per (job,
     // triangle); past the list's capacity they count as lost entries (the
     // wait reports them and the list grows)
+    if (visible) {  // statistics renders only: visible triangles per job
+      const uint64_t vm = __ballot(cls != 0);
+      if (vm && __lane_id() == 0) atomicAdd(&visible[jidx[G.y + q]], (uint32_t)__popcll(vm));
+    }
     const uint32_t at = wave_bin_add(large_ctl, cls == 2, 0u);
     if (cls == 2) {
       if (at < large_cap)
@@ -224,20 +229,21 @@ namespace axr {
 hipError_t launch_batch_setup(int bin_shift, const AxrJobDev* jobs, const uint4* groups,
                               int ngroups, const uint32_t* jidx, uint32_t blocks,
                               const AxrBins& B, uint2* large, uint32_t* large_ctl,
-                              uint32_t large_cap, uint32_t* ctr, hipStream_t s) {
+                              uint32_t large_cap, uint32_t* ctr, uint32_t* visible,
+                              hipStream_t s) {
   if (ngroups == 0 || blocks == 0) return hipSuccess;
   switch (bin_shift) {
     case 3:
       hipLaunchKernelGGL(k_batch_setup<3>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, groups,
-                         ngroups, jidx, B, large, large_ctl, large_cap, ctr);
+                         ngroups, jidx, B, large, large_ctl, large_cap, ctr, visible);
       break;
     case 4:
       hipLaunchKernelGGL(k_batch_setup<4>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, groups,
-                         ngroups, jidx, B, large, large_ctl, large_cap, ctr);
+                         ngroups, jidx, B, large, large_ctl, large_cap, ctr, visible);
       break;
     default:
       hipLaunchKernelGGL(k_batch_setup<5>, dim3(blocks), dim3(kSetupBlock), 0, s, jobs, groups,
-                         ngroups, jidx, B, large, large_ctl, large_cap, ctr);
+                         ngroups, jidx, B, large, large_ctl, large_cap, ctr, visible);
   }
   return hipGetLastError();
 }

@@ -87,6 +87,10 @@ struct axr_batch {
   bool timing = false;
   hipEvent_t ev_k[4] = {nullptr, nullptr, nullptr, nullptr};
   bool timed = false;  // the last render recorded ev_k
+  // statistics renders (axr_batch_set_stats): visible triangles per job
+  bool stats = false;
+  uint32_t* d_visible = nullptr;
+  int visible_cap = 0;
   // graph replay
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -213,7 +217,8 @@ int enqueue(axr_batch* b) {
   if (b->timing) HIPCHK(hipEventRecord(b->ev_k[0], b->stream));
   HIPCHK(launch_batch_setup(b->bin_shift, b->d_jobs, (const uint4*)b->d_sg, b->nsg,
                             b->d_sg + 4 * (size_t)b->njobs, b->setup_blocks, B, b->d_large,
-                            b->d_large_ctl, b->large_cap, b->d_counters, b->stream));
+                            b->d_large_ctl, b->large_cap, b->d_counters,
+                            b->stats ? b->d_visible : nullptr, b->stream));
   HIPCHK(launch_batch_coarse(b->bin_shift, b->d_jobs, b->d_large, b->d_large_ctl, b->large_cap,
                              B, b->cu_count * 2, b->stream));
   if (b->timing) HIPCHK(hipEventRecord(b->ev_k[1], b->stream));
@@ -274,6 +279,7 @@ int axr_batch_destroy(axr_batch* b) {
   dfree(b->d_head);
   dfree(b->d_large);
   dfree(b->d_large_ctl);
+  dfree(b->d_visible);
   dfree(b->d_counters);
   dfree(b->d_rec);
   dfree(b->d_depth);
@@ -334,6 +340,23 @@ int axr_batch_get_timing(axr_batch* b, float ms[3], uint32_t* bin_entries) {
   if (!b->timed) return fail(AXR_ESTATE, "the last render was not timed (axr_batch_set_timing)");
   for (int k = 0; k < 3; k++) HIPCHK(hipEventElapsedTime(&ms[k], b->ev_k[k], b->ev_k[k + 1]));
   if (bin_entries) *bin_entries = b->h_ctr[AXR_CTR_BIN_TOTAL];
+  return AXR_OK;
+}
+
+int axr_batch_set_stats(axr_batch* b, int enable) {
+  if (!b) return fail(AXR_EINVAL, "null batch");
+  b->stats = enable != 0;
+  return AXR_OK;
+}
+
+int axr_batch_get_visible(axr_batch* b, uint32_t* out, int njobs) {
+  if (!b || !out) return fail(AXR_EINVAL, "batch visible: bad arguments");
+  if (!b->stats || !b->d_visible || njobs != b->njobs)
+    return fail(AXR_ESTATE, "batch visible: the last render was not a statistics render of %d jobs",
+                njobs);
+  int rc;
+  if ((rc = axr_batch_wait(b, nullptr))) return rc;
+  HIPCHK(hipMemcpy(out, b->d_visible, sizeof(uint32_t) * (size_t)njobs, hipMemcpyDeviceToHost));
   return AXR_OK;
 }
 
@@ -430,6 +453,14 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
   if ((rc = grow_pools(b, tris ? tris : 1, px ? px : 1, njobs ? njobs : 1,
                        b->mode == SHADE_HDR)))
     return rc;
+  if (b->stats && njobs > b->visible_cap) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    dfree(b->d_visible);
+    b->visible_cap = 0;
+    if ((rc = dalloc(&b->d_visible, (size_t)njobs))) return rc;
+    b->visible_cap = njobs;
+  }
+  if (b->stats) HIPCHK(launch_fill_u32(b->d_visible, (size_t)njobs, 0, b->stream));
   if (b->large_cap < b->want_large) {
     HIPCHK(hipStreamSynchronize(b->stream));
     dfree(b->d_large);
@@ -461,7 +492,7 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
                           hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipEventRecord(b->ev_copied, b->stream));
     b->timed = false;
-    if (b->use_graph && !b->timing) {
+    if (b->use_graph && !b->timing && !b->stats) {
       // The graph holds the three launches; its arguments depend on this key.
       std::vector<uint64_t> key = {(uint64_t)njobs, (uint64_t)b->nsg, blocks, slots,
                                    (uint64_t)b->bin_shift,

@@ -50,7 +50,13 @@ CPU_SHARE_MAX = 16
 # x 1.0000); setup_bin is k_setup + k_coarse.  Valid for the kernels as
 # they are; re-measure after changing one.
 MEASURED_TRAFFIC = {
-    "c2": {"shadow_raster": 92224574, "raster_shade": 134691073, "setup_bin": 134339832},
+    "c2": {"shadow_raster": 95281021},
+    # the batch raster of the bench's own batch (the 64 views; the first 64
+    # requests of the saturated burst), same calibration, from
+    # `bench.py --config c4|c5 --steps 5 --warmup 2 --latency-frames 2
+    # --serve-seconds 0.3 --no-cpu-baseline --calibrate-copy 256`
+    "c4": {"batch_raster": 2257594956},
+    "c5": {"batch_raster": 5253218519},
 }
 
 
@@ -387,12 +393,13 @@ def shader_flags_of(sc, shadows):
     return [g.flags if shadows else g.flags & ~scenes.SH_SHADOW for g in sc.groups]
 
 
-def batch_kernel_roofline(b, jobs, reps=5):
+def batch_kernel_roofline(b, jobs, traffic=None, reps=5):
     """Per-kernel HIP-event times of the batch (eager launches) and the
-    roofline of its dominant kernel, the persistent raster: 40 B per bin
-    entry (4 B entry + 36 B record) and 8 B per pixel written (the 132 B per
-    visible triangle of the single-frame figure are left out: the batch
-    does not count its visible triangles)."""
+    roofline of its dominant kernel, the persistent raster, by the
+    single-frame raster's per-unit figures: 40 B per bin entry (4 B entry +
+    36 B record), 8 B per pixel written, 132 B per visible triangle (counted
+    by one statistics render before the timed ones)."""
+    visible = int(b.visible_triangles(jobs).sum())
     b.set_timing(True)
     ts = []
     for _ in range(reps):
@@ -402,8 +409,10 @@ def batch_kernel_roofline(b, jobs, reps=5):
     med = {k: float(np.median([t[k] for t in ts])) for k in ("setup_ms", "order_ms", "raster_ms")}
     entries = ts[-1]["bin_entries"]
     px = sum(j.width * j.height for j in jobs)
-    algo = entries * 40 + px * 8
-    return med, roofline("batch_raster", algo, med["raster_ms"] * 1e3)
+    algo = entries * 40 + px * 8 + visible * 132
+    roof = roofline("batch_raster", algo, med["raster_ms"] * 1e3, traffic)
+    roof.update(bin_entries=entries, pixels=px, visible_triangles=visible)
+    return med, roof
 
 
 def run_multiview(args, cpu=True):
@@ -435,7 +444,7 @@ def run_multiview(args, cpu=True):
         b.render(jobs)
         b.wait()
         lat.append((time.perf_counter() - t1) * 1e3)
-    kms, roof = batch_kernel_roofline(b, jobs)
+    kms, roof = batch_kernel_roofline(b, jobs, MEASURED_TRAFFIC["c4"]["batch_raster"])
     b.close()
     plain = plain_views(jobs, shader_flags_of, reps=max(1, steps // 2))
     px = sum(j.width * j.height for j in jobs)
@@ -498,7 +507,7 @@ def run_serving(args, cpu=True):
     kb = Batch()
     kb.render(jobs64)
     kb.wait()
-    kms, roof = batch_kernel_roofline(kb, jobs64)
+    kms, roof = batch_kernel_roofline(kb, jobs64, MEASURED_TRAFFIC["c5"]["batch_raster"])
     # the same requests one at a time, each waited for
     sample = burst[:min(len(burst), 300)]
     t0 = time.perf_counter()

@@ -245,6 +245,11 @@ int axr_batch_set_spill_capacity(axr_batch* b, uint32_t spill);
  * milliseconds, and the bin entries the setup appended (may be NULL). */
 int axr_batch_set_timing(axr_batch* b, int enable);
 int axr_batch_get_timing(axr_batch* b, float ms[3], uint32_t* bin_entries);
+/* Statistics renders: while enabled, renders (without graph replay) count
+ * the triangles each job's setup accepts; axr_batch_get_visible waits for
+ * the last render and copies the njobs counts. */
+int axr_batch_set_stats(axr_batch* b, int enable);
+int axr_batch_get_visible(axr_batch* b, uint32_t* out, int njobs);
 /* Enqueues the jobs; returns without waiting.  output: AXR_OUT_RGBA8
  * (gamma'd colour + depth), AXR_OUT_IDS (winner ids + depth) or
  * AXR_OUT_HDR (u16 colour planes + depth). */
