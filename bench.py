@@ -57,6 +57,10 @@ MEASURED_TRAFFIC = {
     # --serve-seconds 0.3 --no-cpu-baseline --calibrate-copy 256`
     "c4": {"batch_raster": 2257594956},
     "c5": {"batch_raster": 5253218519},
+    # the opaque HDR raster of the 1440p full frame, from
+    # `bench.py --config c3 --steps 5 --warmup 2 --latency-frames 2
+    # --no-cpu-baseline --calibrate-copy 256`
+    "c3": {"raster_hdr": 157939038},
 }
 
 
@@ -618,7 +622,8 @@ def run_full_frame(args):
         # triangles' vertex data (132 B), depth and three u16 planes out
         # (10 B/px); counters of the frame's last setup
         algo = st["bin_entries"] * 40 + (st["small"] + st["large"]) * 132 + W * H * 10
-        roof = roofline(raster, algo, kern[raster]["avg_us"])
+        roof = roofline(raster, algo, kern[raster]["avg_us"],
+                        MEASURED_TRAFFIC["c3"].get(raster))
     line = {
         "metric": "frames/s (full frame)", "value": round(fps, 2), "unit": "frames/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
