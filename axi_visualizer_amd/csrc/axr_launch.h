@@ -44,6 +44,18 @@ constexpr uint32_t kHeavyWeight = AXR_HEAVY_WEIGHT;
 constexpr uint32_t kMediumWeight = AXR_MEDIUM_WEIGHT;
 hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins,
                             uint32_t* ctr, uint32_t* clear, uint32_t* tiers, hipStream_t s);
+// Device scratch of the many-block order pass (zeroed once at allocation;
+// each pass leaves it zeroed again).
+struct OrderScratch {
+  uint32_t hist[64];  // bins per weight class
+  uint32_t cur[64];   // positions taken per class
+  uint32_t total, ticket;
+};
+// The same order pass for one large two-segment screen store (a batch's
+// bins, no tiers) over many blocks in two launches: the same counts,
+// counters and class order.
+hipError_t launch_bin_order_wide(const AxrBins& B, OrderScratch* sc, uint32_t* ctr,
+                                 uint32_t* clear, hipStream_t s);
 // Depth walk of every pending map's bins, one launch.
 hipError_t launch_shadow_bins(const AxrBins& B, const AxrPassSet& ps, hipStream_t s);
 hipError_t launch_stats(const uint8_t* cls, const uint8_t* ok, uint32_t T, uint32_t* out,

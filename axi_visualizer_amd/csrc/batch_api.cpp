@@ -48,6 +48,7 @@ struct axr_batch {
   uint2* d_large = nullptr;        // large triangles of the render: (job, triangle)
   uint32_t* d_large_ctl = nullptr;  // [0]: pairs listed (reset by the order pass)
   uint32_t large_cap = 0, want_large = kBatchLarge;
+  OrderScratch* d_order = nullptr;  // the many-block order pass's histogram and cursors
   uint32_t* d_counters = nullptr;
   // pinned copy of d_counters, written on the stream after every render so
   // that a wait never issues a copy of its own: a synchronous copy would
@@ -222,7 +223,7 @@ int enqueue(axr_batch* b) {
   HIPCHK(launch_batch_coarse(b->bin_shift, b->d_jobs, b->d_large, b->d_large_ctl, b->large_cap,
                              B, b->cu_count * 2, b->stream));
   if (b->timing) HIPCHK(hipEventRecord(b->ev_k[1], b->stream));
-  HIPCHK(launch_bin_order(&B, nullptr, b->d_counters, b->d_large_ctl, nullptr, b->stream));
+  HIPCHK(launch_bin_order_wide(B, b->d_order, b->d_counters, b->d_large_ctl, b->stream));
   if (b->timing) HIPCHK(hipEventRecord(b->ev_k[2], b->stream));
   HIPCHK(launch_batch_raster(b->bin_shift, b->mode, b->d_jobs, b->njobs, B, grid, b->stream));
   if (b->timing) HIPCHK(hipEventRecord(b->ev_k[3], b->stream));
@@ -252,7 +253,7 @@ int axr_batch_create(axr_batch** out) {
     return bail(fail(AXR_EHIP, "stream/event creation failed"));
   int rc;
   if ((rc = dalloc(&b->d_head, 1)) || (rc = dalloc(&b->d_counters, AXR_CTR_WORDS)) ||
-      (rc = dalloc(&b->d_large_ctl, 2)))
+      (rc = dalloc(&b->d_large_ctl, 2)) || (rc = dalloc(&b->d_order, 1)))
     return bail(rc);
   if (hipHostMalloc((void**)&b->h_ctr, sizeof(uint32_t) * AXR_CTR_WORDS) != hipSuccess)
     return bail(fail(AXR_EHIP, "pinned counter allocation failed"));
@@ -260,6 +261,8 @@ int axr_batch_create(axr_batch** out) {
   if (launch_fill_u32(b->d_counters, AXR_CTR_WORDS, 0, b->stream) != hipSuccess ||
       launch_fill_u32(b->d_head, 1, 0, b->stream) != hipSuccess ||
       launch_fill_u32(b->d_large_ctl, 2, 0, b->stream) != hipSuccess ||
+      launch_fill_u32((uint32_t*)b->d_order, sizeof(OrderScratch) / sizeof(uint32_t), 0,
+                      b->stream) != hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess)
     return bail(fail(AXR_EHIP, "initial fill failed"));
   b->bins.nseg = 2;
@@ -279,6 +282,7 @@ int axr_batch_destroy(axr_batch* b) {
   dfree(b->d_head);
   dfree(b->d_large);
   dfree(b->d_large_ctl);
+  dfree(b->d_order);
   dfree(b->d_visible);
   dfree(b->d_counters);
   dfree(b->d_rec);

@@ -10,7 +10,9 @@
 //                pending shadow pass (a11, Pipe/trisetupOrtho.c) appended to
 //                that map's 64x64 texel bins
 //   k_bin_order  entry counts of the pass, heavy-first bin order; resets the
-//                append cursors for the next pass (one block per bin store)
+//                append cursors for the next pass (one block per bin store;
+//                a batch's store of every job's bins takes k_order_count and
+//                k_order_scatter over many blocks instead)
 //   k_shadow_bins  each pending map's bins walked with the bin's texels in
 //                LDS (Shaders/drawmin.c) and written once
 // The screen bins' depth walk and shading run later in one launch
@@ -365,6 +367,130 @@ __global__ __launch_bounds__(1024) void k_bin_order(OrderJob j0, OrderJob j1,
   }
 }
 
+// The order pass of a large store (a batch: every job's bins) over many
+// blocks, in two launches.  k_order_count: num = cnt, cnt = 0, each block's
+// weight-class histogram and entry total added into OrderScratch; block 0
+// also settles the spill list, resets the work-queue head and zeroes
+// *clear.  k_order_scatter: every block derives the heavy-first class
+// offsets from the histogram, reserves each class's positions for its bins
+// with one atomic per class and writes them; the last block to read the
+// histogram clears it for the next pass (the count launch clears the
+// cursors), so the scratch needs no memset between passes.
+constexpr int kWideThreads = 256;
+constexpr int kWidePer = 4;  // bins per thread
+constexpr int kWideBins = kWideThreads * kWidePer;
+
+__global__ __launch_bounds__(kWideThreads) void k_order_count(AxrBins B, OrderScratch* sc,
+                                                              uint32_t* __restrict__ ctr,
+                                                              int ctr_spill,
+                                                              uint32_t* __restrict__ clear) {
+  __shared__ uint32_t hist[kOrderClasses];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int nb = B.nb;
+  if (tid < kOrderClasses) hist[tid] = 0;
+  if (blockIdx.x == 0) {
+    if (tid < kOrderClasses) sc->cur[tid] = 0;
+    if (tid == 0) {
+      if (clear) *clear = 0;  // the large list, read by now
+      const uint32_t used = B.spill_ctr[0];
+      B.spill_ctr[1] = used < B.spill_cap ? used : B.spill_cap;
+      B.spill_ctr[0] = 0;
+      if (B.head) *B.head = 0;
+      if (used) atomicMax(&ctr[ctr_spill], used);
+      if (used > B.spill_cap) atomicAdd(&ctr[AXR_CTR_LOST], used - B.spill_cap);
+    }
+  }
+  __syncthreads();
+  uint32_t tot = 0;
+  const int b0 = blockIdx.x * kWideBins;
+  uint32_t c0[kWidePer], c1[kWidePer];
+#pragma unroll
+  for (int u = 0; u < kWidePer; u++) {
+    const int b = b0 + u * kWideThreads + tid;
+    c0[u] = b < nb ? B.cnt[b] : 0u;
+    c1[u] = b < nb ? B.cnt[nb + b] : 0u;
+  }
+#pragma unroll
+  for (int u = 0; u < kWidePer; u++) {
+    const int b = b0 + u * kWideThreads + tid;
+    const bool valid = b < nb;
+    uint32_t k = 0;
+    if (valid) {
+      B.num[b] = c0[u];
+      B.cnt[b] = 0;
+      B.num[nb + b] = c1[u];
+      B.cnt[nb + b] = 0;
+      tot += c0[u] + c1[u];
+      k = order_class(c0[u] + 16u * c1[u]);
+    }
+    const uint64_t em = __ballot(valid && k == 0);
+    if (em && lane == __ffsll((unsigned long long)em) - 1) atomicAdd(&hist[0], (uint32_t)__popcll(em));
+    if (valid && k != 0) atomicAdd(&hist[k], 1u);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+  __syncthreads();
+  if (tid < kOrderClasses && hist[tid]) atomicAdd(&sc->hist[tid], hist[tid]);
+  if (lane == 0 && tot) atomicAdd(&sc->total, tot);
+}
+
+__global__ __launch_bounds__(kWideThreads) void k_order_scatter(AxrBins B, OrderScratch* sc,
+                                                                uint32_t* __restrict__ ctr,
+                                                                int ctr_total) {
+  __shared__ uint32_t offs[kOrderClasses];  // heavy-first offset of each class
+  __shared__ uint32_t cnt[kOrderClasses];   // this block's bins per class
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int nb = B.nb;
+  if (tid < kOrderClasses) {
+    const uint32_t h = __hip_atomic_load(&sc->hist[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t sfx = h;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_down(sfx, o);
+      if (lane + o < 64) sfx += t;
+    }
+    offs[tid] = sfx - h;  // classes above this one come first
+    cnt[tid] = 0;
+  }
+  if (tid == 0 && blockIdx.x == 0)
+    ctr[ctr_total] = __hip_atomic_load(&sc->total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  // the histogram is read: the last block to get here clears it
+  if (tid == 0) {
+    __threadfence();
+    if (atomicAdd(&sc->ticket, 1u) == gridDim.x - 1) {
+      for (int k = 0; k < kOrderClasses; k++) sc->hist[k] = 0;
+      sc->total = 0;
+      sc->ticket = 0;
+    }
+  }
+  const int b0 = blockIdx.x * kWideBins;
+  uint32_t kk[kWidePer], rank[kWidePer];
+#pragma unroll
+  for (int u = 0; u < kWidePer; u++) {
+    const int b = b0 + u * kWideThreads + tid;
+    const bool valid = b < nb;
+    const uint32_t k = valid ? order_class(B.num[b] + 16u * B.num[nb + b]) : 0u;
+    kk[u] = k;
+    // empty bins take their ranks a wave at a time
+    const uint64_t em = __ballot(valid && k == 0);
+    uint32_t base = 0;
+    const int leader = em ? __ffsll((unsigned long long)em) - 1 : 0;
+    if (em && lane == leader) base = atomicAdd(&cnt[0], (uint32_t)__popcll(em));
+    base = __shfl(base, leader);
+    rank[u] = !valid ? 0u : k == 0 ? base + (uint32_t)__popcll(em & lanemask_lt())
+                                   : atomicAdd(&cnt[k], 1u);
+  }
+  __syncthreads();
+  if (tid < kOrderClasses && cnt[tid]) offs[tid] += atomicAdd(&sc->cur[tid], cnt[tid]);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kWidePer; u++) {
+    const int b = b0 + u * kWideThreads + tid;
+    if (b < nb) B.order[offs[kk[u]] + rank[u]] = (uint32_t)b;
+  }
+}
+
 // K3: one block per 64x64 texel bin of every pending map, heavy bins first;
 // minimum depth keys in LDS, written once.  A bin starts from its map
 // (from_map) or from the clear depth.
@@ -484,6 +610,17 @@ hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins
   if (n == 1) j[1] = j[0];
   hipLaunchKernelGGL(k_bin_order, dim3(n), dim3(1024), 0, s, j[0], j[1], ctr,
                      main_bins ? clear : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_order_wide(const AxrBins& B, OrderScratch* sc, uint32_t* ctr,
+                                 uint32_t* clear, hipStream_t s) {
+  if (B.nb <= 0) return launch_bin_order(&B, nullptr, ctr, clear, nullptr, s);
+  const unsigned grid = (unsigned)((B.nb + kWideBins - 1) / kWideBins);
+  hipLaunchKernelGGL(k_order_count, dim3(grid), dim3(kWideThreads), 0, s, B, sc, ctr,
+                     (int)AXR_CTR_BIN_SPILL, clear);
+  hipLaunchKernelGGL(k_order_scatter, dim3(grid), dim3(kWideThreads), 0, s, B, sc, ctr,
+                     (int)AXR_CTR_BIN_TOTAL);
   return hipGetLastError();
 }
 
