@@ -403,9 +403,8 @@ const char* axr_last_error(void) { return g_err.c_str(); }
 int axr_create(int w, int h, axr_ctx** out) {
   if (!out) return fail(AXR_EINVAL, "null out pointer");
   *out = nullptr;
-  if (w <= 0 || h <= 0 || (w % 16) != 0)
-    return fail(AXR_EINVAL, "width must be a positive multiple of 16 and height positive (%dx%d)",
-                w, h);
+  if (w <= 0 || h <= 0)
+    return fail(AXR_EINVAL, "width and height must be positive (%dx%d)", w, h);
   if ((int64_t)w * h > (int64_t)1 << 28) return fail(AXR_EINVAL, "frame too large");
   axr_ctx* c = new axr_ctx();
   HIPCHK(hipGetDevice(&c->device));
@@ -796,7 +795,9 @@ int axr_draw_ssr(axr_ctx* c, const int* groups, const int* refl, int n, int step
     if ((rc = materialize(c, SHADE_HDR, 1.0f))) return rc;
   }
   const size_t np = (size_t)c->W * c->H;
-  const size_t snap = np * (sizeof(float) + 3 * sizeof(uint16_t));
+  // planes of the snapshot start 16-byte aligned: a stride of np rounded up to 8
+  const size_t np8 = (np + 7) & ~(size_t)7;
+  const size_t snap = np8 * (sizeof(float) + 3 * sizeof(uint16_t));
   if (c->snap_bytes < snap) {
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->d_snap) (void)hipFree(c->d_snap);
@@ -807,17 +808,16 @@ int axr_draw_ssr(axr_ctx* c, const int* groups, const int* refl, int n, int step
   }
   AxrSsrConst S{};
   S.F0 = (const float*)c->d_snap;
-  S.R0 = (const uint16_t*)((char*)c->d_snap + np * sizeof(float));
-  S.G0 = S.R0 + np;
-  S.B0 = S.G0 + np;
+  S.R0 = (const uint16_t*)((char*)c->d_snap + np8 * sizeof(float));
+  S.G0 = S.R0 + np8;
+  S.B0 = S.G0 + np8;
   S.steps = steps;
   S.step = step;
   S.absorb = absorb;
   S.fresnel = fresnel ? 1 : 0;
   {
     Timed tm(c, "ssr_copy");
-    // W is a multiple of 16, so np is a multiple of 8 and every plane of the
-    // snapshot starts 16-byte aligned: one launch of 16-byte vectors
+    // one launch: 16-byte vectors, the last np % 8 pixels one by one
     HIPCHK(launch_snap_frame(c->d_depth, c->d_r, c->d_g, c->d_b, np, (float*)S.F0,
                              (uint16_t*)S.R0, (uint16_t*)S.G0, (uint16_t*)S.B0, c->stream));
   }

@@ -77,9 +77,8 @@ struct BinCursor {
 // 1 << SHIFT pixels that hold it: a 32-px bin gets (gi << 2) | the tile's
 // quadrant; a 16-px bin is the tile; each 8-px bin of the tile gets gi << 2.
 template <int SHIFT>
-__device__ __forceinline__ void append_tile(const AxrBins& B, uint32_t slot_base, int t, int WC,
+__device__ __forceinline__ void append_tile(const AxrBins& B, uint32_t slot_base, int tx, int ty,
                                             int NBX, uint32_t gi) {
-  const int tx = t % WC, ty = t / WC;
   if (SHIFT == 5) {
     const uint32_t slot = slot_base + (uint32_t)((ty >> 1) * NBX + (tx >> 1));
     const uint32_t sub = (uint32_t)(((ty & 1) << 1) | (tx & 1));
@@ -96,15 +95,16 @@ __device__ __forceinline__ void append_tile(const AxrBins& B, uint32_t slot_base
   }
 }
 
-// Reference 16-px tile of a large-triangle entry v of bin (bx, by).
+// The walked 16-px tile (walked_tile) of a large-triangle entry v of bin
+// (bx, by), as append_tile stored it.
 template <int SHIFT>
-__device__ __forceinline__ int entry_tile(uint32_t v, int bx, int by, int WC) {
+__device__ __forceinline__ TileBox entry_box(uint32_t v, int bx, int by) {
   if (SHIFT == 5) {
     const uint32_t sub = v & 3u;
-    return ((by << 1) + (int)(sub >> 1)) * WC + (bx << 1) + (int)(sub & 1u);
+    return tile_box_xy((bx << 1) + (int)(sub & 1u), (by << 1) + (int)(sub >> 1));
   }
-  if (SHIFT == 4) return by * WC + bx;
-  return (by >> 1) * WC + (bx >> 1);
+  if (SHIFT == 4) return tile_box_xy(bx, by);
+  return tile_box_xy(bx >> 1, by >> 1);
 }
 
 // ---------------------------------------------- wave-cooperative span walk
@@ -420,7 +420,7 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
     }
     if (valid) {
       const uint32_t gi = v >> 2;
-      box = tile_box(entry_tile<SHIFT>(v, bx, by, dc.WC), dc.W);
+      box = entry_box<SHIFT>(v, bx, by);
       g = tile_geom(r.px, r.py, r.z);
       order = dc.T + gi;
       // the triangle's box inside the tile

@@ -89,8 +89,8 @@ hipError_t launch_ssr(const AxrTriRec* rec, const AxrBins& B, const float4* vpos
                       const float4* vattr, const AxrGroupDev* groups, const AxrDrawConst& dc,
                       const AxrSsrConst& S, int NBX, float* depth, uint16_t* r, uint16_t* g,
                       uint16_t* b, hipStream_t s);
-// Copies depth and the three HDR planes (np pixels, np a multiple of 8,
-// every pointer 16-byte aligned) into the snapshot the SSR pass reads.
+// Copies depth and the three HDR planes (np pixels, every pointer 16-byte
+// aligned) into the snapshot the SSR pass reads.
 hipError_t launch_snap_frame(const float* depth, const uint16_t* r, const uint16_t* g,
                              const uint16_t* b, size_t np, float* F0, uint16_t* R0,
                              uint16_t* G0, uint16_t* B0, hipStream_t s);

@@ -90,7 +90,7 @@ __device__ __forceinline__ void replay_frag(uint32_t seq, int x, int y,
     Attr a3 = load_attr(vpos, vattr, 3 * gi + (uint32_t)g.k3, g.z3);
     const Attr a4 = a_lerp4(a1, a3, g.yd);
     if (grp.flags & AXR_SH_MIP) mi = mip_setup(a1, a2, a3, g, grp.tex_size, grp.mip_bias);
-    const TileBox box = tile_box((y >> 4) * dc.WC + (x >> 4), dc.W);
+    const TileBox box = tile_box_xy(x >> 4, y >> 4);  // the walked tile holding the pixel
     const Edges e = tile_edges(g, half);
     if (half == 0) {
       Attr s1 = a_slope(a2, a1, e.inv), s2 = a_slope(a4, a1, e.inv);

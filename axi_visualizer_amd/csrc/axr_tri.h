@@ -497,15 +497,31 @@ __device__ __forceinline__ void small_bounds(const float px[3], const float py[3
 struct TileBox {
   float xMin, xMax, yMin, yMax;
 };
-// Templates.py:23-29.
-__device__ __forceinline__ TileBox tile_box(int bx, int W) {
-  int tileX = bx % (int)((float)W / 16.0f);
-  int tileY = f2i((float)bx / ((float)W / 16.0f));
+// Templates.py:23-29: the box the tiled draw's work-group bx walks.  Its
+// column is bx % (int)(W/16) and its row bx / (W/16) in float arithmetic,
+// truncated; for the index bx = WC*cy + ax of coarse tile (cy, ax) that is
+// the tile itself when W is a multiple of 16.  For other widths W/16 has a
+// fraction and a tile whose column is small against its row (ax < cy/2 at
+// W/16 = k + 1/2) maps to a box of the row above: the reference walks the
+// triangle there, and the oracle and the kernels do the same.
+AXR_HD void walked_tile(int cy, int ax, int WC, int W, int& tx, int& ty) {
+  if ((W & 15) == 0) {
+    tx = ax;
+    ty = cy;
+    return;
+  }
+  const int bx = WC * cy + ax;
+  const float wt = (float)W / 16.0f;
+  tx = bx % (int)wt;
+  ty = f2i((float)bx / wt);
+}
+// The box of 16-px tile (tx, ty).
+AXR_HD TileBox tile_box_xy(int tx, int ty) {
   TileBox b;
-  b.xMin = (float)tileX * 16.0f;
-  b.xMax = ((float)tileX + 1.0f) * 16.0f - 1.0f;
-  b.yMin = (float)tileY * 16.0f;
-  b.yMax = ((float)tileY + 1.0f) * 16.0f - 1.0f;
+  b.xMin = (float)tx * 16.0f;
+  b.xMax = ((float)tx + 1.0f) * 16.0f - 1.0f;
+  b.yMin = (float)ty * 16.0f;
+  b.yMax = ((float)ty + 1.0f) * 16.0f - 1.0f;
   return b;
 }
 

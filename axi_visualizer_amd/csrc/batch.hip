@@ -170,10 +170,12 @@ __global__ __launch_bounds__(kCoarseThreads) void k_batch_coarse(
                          // columns in [0, WC - 1], cy in [0, HC - 1]
 #pragma unroll
                          for (int j = 0; j < 8; j++) {
-                           const int tx = ax + j * G;
-                           if ((float)tx <= xe)
-                             append_tile<SHIFT>(B, (uint32_t)B.nb + J.slot0, cy * WC + tx, WC,
-                                                J.nbx, gi);
+                           const int c = ax + j * G;
+                           if ((float)c <= xe) {
+                             int tx, ty;
+                             walked_tile(cy, c, WC, J.dc.W, tx, ty);
+                             append_tile<SHIFT>(B, (uint32_t)B.nb + J.slot0, tx, ty, J.nbx, gi);
+                           }
                          }
                        });
   }

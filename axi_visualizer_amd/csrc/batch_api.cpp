@@ -383,8 +383,10 @@ int axr_batch_render(axr_batch* b, const axr_job* jobs, int njobs, int output) {
   for (int i = 0; i < njobs; i++) {
     const axr_job& j = jobs[i];
     if (!j.scene) return fail(AXR_EINVAL, "job %d: null scene", i);
-    if (j.width <= 0 || j.height <= 0 || (j.width % 16) != 0)
-      return fail(AXR_EINVAL, "job %d: width must be a positive multiple of 16", i);
+    if (j.width <= 0 || j.height <= 0)
+      return fail(AXR_EINVAL, "job %d: width and height must be positive", i);
+    if ((int64_t)j.width * j.height > (int64_t)1 << 28)
+      return fail(AXR_EINVAL, "job %d: frame too large", i);
     if (j.pcf_taps != 5 && j.pcf_taps != 9) return fail(AXR_EINVAL, "job %d: pcf_taps 5 or 9", i);
     if (j.scene->device != b->device) return fail(AXR_EINVAL, "job %d: scene on another device", i);
   }

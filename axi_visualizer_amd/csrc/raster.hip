@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse(const AxrTriRec* __re
                                                            const uint32_t* __restrict__ large,
                                                            const uint32_t* __restrict__ ctl,
                                                            AxrBins MB,
-                                                           int WC, int HC, int NBX) {
+                                                           int W, int WC, int HC, int NBX) {
   const uint32_t n = ctl[0];
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
     const uint32_t gi = large[i];
@@ -210,22 +210,22 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse(const AxrTriRec* __re
                        [&](int cy, int ax, int G, float xe) {
                          // columns in [0, WC - 1] and cy in [0, HC - 1]
                          // (coarse_block_tiles, coarse_rows)
-                         uint32_t slot[8], pos[8];
+                         // each to the bin and quadrant of the tile the
+                         // tiled draw walks for it (walked_tile)
+                         uint32_t slot[8], pos[8], sb[8];
                          bool on[8];
 #pragma unroll
                          for (int j = 0; j < 8; j++) {
-                           const int tx = ax + j * G;
-                           on[j] = (float)tx <= xe;
-                           slot[j] = (uint32_t)MB.nb + (uint32_t)((cy >> 1) * NBX + (tx >> 1));
+                           int tx, ty;
+                           walked_tile(cy, ax + j * G, WC, W, tx, ty);
+                           on[j] = (float)(ax + j * G) <= xe;
+                           slot[j] = (uint32_t)MB.nb + (uint32_t)((ty >> 1) * NBX + (tx >> 1));
+                           sb[j] = (uint32_t)(((ty & 1) << 1) | (tx & 1));
                            pos[j] = on[j] ? atomicAdd(&MB.cnt[slot[j]], 1u) : 0u;
                          }
 #pragma unroll
-                         for (int j = 0; j < 8; j++) {
-                           if (!on[j]) continue;
-                           const int tx = ax + j * G;
-                           const uint32_t sb = (uint32_t)(((cy & 1) << 1) | (tx & 1));
-                           bin_store(MB, slot[j], pos[j], (gi << 2) | sb);
-                         }
+                         for (int j = 0; j < 8; j++)
+                           if (on[j]) bin_store(MB, slot[j], pos[j], (gi << 2) | sb[j]);
                        });
   }
 }
@@ -589,7 +589,7 @@ hipError_t launch_setup(const float4* vpos, const float4* vattr, const AxrGroupD
                      large, large_ctl);
   if (do_main)
     hipLaunchKernelGGL(k_coarse, dim3(kCoarseBlocks), dim3(kCoarseThreads), 0, s, rec, large,
-                       large_ctl, MB, dc.WC, dc.HC, NBX);
+                       large_ctl, MB, dc.W, dc.WC, dc.HC, NBX);
   return hipGetLastError();
 }
 

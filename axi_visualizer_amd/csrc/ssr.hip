@@ -225,23 +225,32 @@ __global__ __launch_bounds__(AXR_SSR_THREADS) void k_ssr_raster(
   }
 }
 
-// The opaque frame's snapshot for the pass in one launch: each thread copies
-// eight pixels, depth as two 16-byte vectors and each colour plane as one
-// (frame widths are multiples of 16, so the pixel count is a multiple of
-// eight and every plane of the snapshot starts 16-byte aligned).
-__global__ void k_snap_frame(const float4* __restrict__ depth, const uint4* __restrict__ r,
-                             const uint4* __restrict__ g, const uint4* __restrict__ b, size_t n8,
-                             float4* __restrict__ F0, uint4* __restrict__ R0,
-                             uint4* __restrict__ G0, uint4* __restrict__ B0) {
+// The opaque frame's snapshot for the pass in one launch: thread i < n8
+// copies pixels 8i..8i+7, depth as two 16-byte vectors and each colour plane
+// as one; threads n8..n8+tail-1 copy the last np % 8 pixels one each.  The
+// snapshot's planes start 16-byte aligned (a stride rounded up to 8 pixels).
+__global__ void k_snap_frame(const float* __restrict__ depth, const uint16_t* __restrict__ r,
+                             const uint16_t* __restrict__ g, const uint16_t* __restrict__ b,
+                             size_t n8, size_t tail, float* __restrict__ F0,
+                             uint16_t* __restrict__ R0, uint16_t* __restrict__ G0,
+                             uint16_t* __restrict__ B0) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n8) return;
-  const float4 d0 = depth[2 * i], d1 = depth[2 * i + 1];
-  const uint4 rr = r[i], gg = g[i], bb = b[i];
-  F0[2 * i] = d0;
-  F0[2 * i + 1] = d1;
-  R0[i] = rr;
-  G0[i] = gg;
-  B0[i] = bb;
+  if (i < n8) {
+    const float4* d4 = (const float4*)depth;
+    const float4 d0 = d4[2 * i], d1 = d4[2 * i + 1];
+    const uint4 rr = ((const uint4*)r)[i], gg = ((const uint4*)g)[i], bb = ((const uint4*)b)[i];
+    ((float4*)F0)[2 * i] = d0;
+    ((float4*)F0)[2 * i + 1] = d1;
+    ((uint4*)R0)[i] = rr;
+    ((uint4*)G0)[i] = gg;
+    ((uint4*)B0)[i] = bb;
+  } else if (i < n8 + tail) {
+    const size_t p = 8 * n8 + (i - n8);
+    F0[p] = depth[p];
+    R0[p] = r[p];
+    G0[p] = g[p];
+    B0[p] = b[p];
+  }
 }
 
 // Six faces' HDR planes [6][S][S] into the strip the shader samples:
@@ -274,12 +283,10 @@ hipError_t launch_ssr(const AxrTriRec* rec, const AxrBins& B, const float4* vpos
 hipError_t launch_snap_frame(const float* depth, const uint16_t* r, const uint16_t* g,
                              const uint16_t* b, size_t np, float* F0, uint16_t* R0,
                              uint16_t* G0, uint16_t* B0, hipStream_t s) {
-  if (np % 8 != 0) return hipErrorInvalidValue;
-  const size_t n8 = np / 8;
-  if (n8 == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_snap_frame, dim3((unsigned)((n8 + 255) / 256)), dim3(256), 0, s,
-                     (const float4*)depth, (const uint4*)r, (const uint4*)g, (const uint4*)b, n8,
-                     (float4*)F0, (uint4*)R0, (uint4*)G0, (uint4*)B0);
+  const size_t n8 = np / 8, tail = np - 8 * n8;
+  if (np == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_snap_frame, dim3((unsigned)((n8 + tail + 255) / 256)), dim3(256), 0, s,
+                     depth, r, g, b, n8, tail, F0, R0, G0, B0);
   return hipGetLastError();
 }
 

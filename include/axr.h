@@ -44,7 +44,9 @@ typedef struct axr_ctx axr_ctx;
 const char* axr_last_error(void);
 
 /* CLDraw.__init__(size_sky, max_uv, w, h, max_particles), Ops.py:128-229.
- * w must be a multiple of 16 (the 16-px tile grid, Ops.py:131). */
+ * Any positive w and h.  The tiled pass's grid is w/16 x h/16 tiles
+ * (Ops.py:130-131); at widths that are not multiples of 16 its work-groups
+ * walk the reference's folded tile boxes (Templates.py:23-24). */
 int axr_create(int w, int h, axr_ctx** out);
 int axr_destroy(axr_ctx* ctx);
 
@@ -221,7 +223,7 @@ typedef struct axr_job {
   float pos[3];           /* camera position (setPos) */
   float vm[9];            /* view basis rows (setVM) */
   float scale, cax, cay;  /* setScaleCull for this resolution */
-  int width, height;      /* width a positive multiple of 16 */
+  int width, height;      /* positive; as axr_create */
   int shadows;            /* sample the scene's first two placed shadow maps */
   int pcf_taps;           /* 5 (reference cross) or 9 (3x3) */
   float exposure;         /* gamma exposure of the RGBA8 output */

@@ -1102,6 +1102,18 @@ void draw_pass(Ctx& cx, bool water, int nthreads, int64_t st[5], OnFrag&& on_fra
   }
   st[3] = maxbin;
   st[4] = entries;
+  // The box each coarse tile's list is walked in: tile_box(bx, W), the tile
+  // itself when W is a multiple of 16; for other widths some tiles fold onto
+  // a box of a row above.  A box walked for several tiles (the reference
+  // runs their work-groups concurrently on it) takes the union of their
+  // lists in ascending global index: the large pass's order is the triangle
+  // order, as for a box walked once.
+  std::vector<std::vector<int>> src(NT);
+  for (int bx = 0; bx < NT; bx++) {
+    const TileBox tb = tile_box(bx, W);
+    const int tx = f2i(tb.xMin / 16.0f), ty = f2i(tb.yMin / 16.0f);
+    src[(size_t)ty * WC + tx].push_back(bx);
+  }
 
   std::atomic<int64_t> frags{0};
   parallel_for(nb, nthreads, [&](int b) {
@@ -1142,9 +1154,17 @@ void draw_pass(Ctx& cx, bool water, int nthreads, int64_t st[5], OnFrag&& on_fra
         if (bins[g].empty()) continue;
         const OrcGroup& G = groups[g];
         for (int tx = 0; tx < WC; tx++) {
-          const int bx = b * WC + tx;
-          TileBox tb = tile_box(bx, W);
-          for (uint32_t gi : bins[g][bx]) {
+          const std::vector<int>& from = src[(size_t)b * WC + tx];
+          if (from.empty()) continue;
+          const TileBox tb = tile_box(from[0], W);
+          std::vector<uint32_t> merged;
+          const std::vector<uint32_t>* list = &bins[g][from[0]];
+          if (from.size() > 1) {
+            for (int bx : from) merged.insert(merged.end(), bins[g][bx].begin(), bins[g][bx].end());
+            std::stable_sort(merged.begin(), merged.end());
+            list = &merged;
+          }
+          for (uint32_t gi : *list) {
             const int64_t li = gi - cx.base[g];
             TileTri s;
             s.x1 = cx.P[3 * gi].x; s.y1 = cx.P[3 * gi].y;
@@ -1453,7 +1473,7 @@ int orc_vertlight(const float* xyz, const float* vn, int nvert, const float* dir
 int orc_render(const OrcGroup* groups, int ngroups, const OrcFrame* fr, OrcOut* out) {
   if (!groups || ngroups <= 0 || !fr || !out) return -1;
   const int W = fr->W, H = fr->H;
-  if (W <= 0 || H <= 0 || (W % 16) != 0) return -2;
+  if (W <= 0 || H <= 0) return -2;
   const int nthreads = resolve_threads(fr->nthreads);
   Ctx cx;
   cx.groups = groups;

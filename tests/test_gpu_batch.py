@@ -133,10 +133,12 @@ def test_bin_size_does_not_change_pixels(spheres_ctx, bin_size):
 
 def test_large_triangles_in_small_bins_match_single_frames():
     """Close-up terrain: large triangles take the 16-px tile path, which an
-    8-px bin clips and a 32-px bin holds four of."""
+    8-px bin clips and a 32-px bin holds four of; at widths that are not
+    multiples of 16 the tiles fold as the reference's tiled draw folds them."""
     ctx = SceneContext(base_terrain(cam_dist=18.0, grid=24))
     try:
-        jobs = [Job(ctx, scene_view(ctx.scene, w, h), w, h) for w, h in [(320, 240), (160, 112)]]
+        jobs = [Job(ctx, scene_view(ctx.scene, w, h), w, h)
+                for w, h in [(320, 240), (160, 112), (200, 117), (37, 29)]]
         for size in (8, 16, 32):
             b = Batch(bin_size=size)
             try:

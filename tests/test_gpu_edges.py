@@ -18,10 +18,23 @@ pytestmark = pytest.mark.gpu
 NONE = 0xFFFFFFFF
 
 
-@pytest.mark.parametrize("W,H", [(208, 143), (48, 17), (16, 1)])
+@pytest.mark.parametrize("W,H", [(208, 143), (48, 17), (16, 1), (200, 143), (1000, 61),
+                                 (24, 160), (37, 29)])
 def test_ragged_frame_sizes(W, H):
+    """Heights and widths that are not multiples of the bin or tile size; at
+    widths that are not multiples of 16 the tiled pass walks the reference's
+    folded tile boxes (Templates.py:23-24)."""
     sc = scenes.random_tris_scene(W=W, H=H, ntri=900, seed=13, name=f"ragged_{W}x{H}")
     check_scene(sc)
+
+
+@pytest.mark.parametrize("W,H", [(200, 80), (24, 160), (192, 80)])
+def test_folded_tiles_of_a_ragged_width(W, H):
+    """A rectangle of large triangles over the whole view: at W = 200 the
+    tiles the reference's tiled draw never walks stay undrawn, bit-exact with
+    the oracle (whose tile pattern test_oracle pins by hand)."""
+    from tests.test_oracle import rect_scene
+    check_scene(rect_scene(W, H), hdr=True)  # the scene has no gamma (exposure 0)
 
 
 def test_hostile_geometry():

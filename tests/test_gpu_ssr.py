@@ -59,10 +59,11 @@ def test_water_with_shadows_and_64_steps():
     check(scenes.water_scene(W=256, H=192, shadow_dim=256, ssr_steps=64), "water_shadow_64")
 
 
-@pytest.mark.parametrize("W,H", [(208, 117), (48, 37)])
+@pytest.mark.parametrize("W,H", [(208, 117), (48, 37), (201, 117), (37, 29)])
 def test_water_ragged_frame(W, H):
     """Frames whose 32-px bins are cut at the right and bottom edges (the
-    march and the snapshot of the opaque frame run to the last pixel)."""
+    march and the snapshot of the opaque frame run to the last pixel; a
+    pixel count that is not a multiple of 8 takes the snapshot's tail)."""
     check(scenes.water_scene(W=W, H=H, ssr_steps=64), f"water_{W}x{H}")
 
 

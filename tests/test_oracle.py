@@ -228,6 +228,41 @@ def test_coarse_misses_are_rare():
 
 
 # ------------------------------------------------------ shadow walk (a11)
+def rect_scene(W, H, z=4.0):
+    """Two large triangles forming a rectangle that covers the whole view."""
+    a = np.array([[-1.2, -0.6, 1], [1.2, -0.6, 1], [1.2, 0.6, 1],
+                  [-1.2, -0.6, 1], [1.2, 0.6, 1], [-1.2, 0.6, 1]], np.float32) * z
+    tex, side = scenes.texture_for(np.full((4, 4, 3), 1000, np.uint16), False)
+    g = scenes.Group(xyz=a, vn=np.tile([0, 0, -1], (6, 1)).astype(np.float32),
+                     uv=np.zeros((6, 2), np.float32), tex=tex, tex_side=side, mip=False, flags=0)
+    return scenes.Scene(name=f"rect_{W}x{H}", W=W, H=H, groups=[g],
+                        view=scenes.make_camera(W, H, [0, 0, 0], 0.0, 0.0, fovx=90.0),
+                        light_int=np.ones(3, np.float32),
+                        light_dir=np.array([0, 0, 1], np.float32), exposure=0.0)
+
+
+def _tile_coverage(ids, W, H):
+    cov = ids != 0xFFFFFFFF
+    return ["".join("#" if cov[ty * 16:ty * 16 + 16, tx * 16:tx * 16 + 16].all() else
+                    "." if not cov[ty * 16:ty * 16 + 16, tx * 16:tx * 16 + 16].any() else "+"
+                    for tx in range((W + 15) // 16)) for ty in range((H + 15) // 16)]
+
+
+def test_tiled_pass_folds_tiles_at_a_ragged_width():
+    """Templates.py:23-24 at W = 200: work-group bx walks column bx % (int)(W/16)
+    = bx % 12 of row (int)(bx / 12.5).  Of the 60 coarse tiles (WC = 12,
+    HC = 5), bx 0-12 land in row 0, 13-24 in row 1, 25-37 in row 2, 38-49 in
+    row 3 (each row gets all 12 columns) and bx 50-59 in row 4 at columns
+    2-11 only; no tile covers x >= 192.  A rectangle of large triangles over
+    the whole view therefore leaves tiles (0, 4) and (1, 4) and the last 8
+    columns undrawn.  At W = 192 (a multiple of 16) every tile is drawn."""
+    W, H = 200, 80
+    out = po.render(rect_scene(W, H))
+    assert _tile_coverage(out["id"], W, H) == ["############."] * 4 + ["..##########."]
+    out = po.render(rect_scene(192, H))
+    assert _tile_coverage(out["id"], 192, H) == ["############"] * 5
+
+
 def test_shadow_walk_strict_top_loop():
     """drawmin.c for (0,0),(8,0),(0,8): the flat bottom half (y1 = y2 = 0)
     has slopes 8/0 = inf and 0/0 = NaN and walks ax from (int)cx2 = 0 to
