@@ -226,16 +226,55 @@ int bin_raster(axr_ctx* c, int mode, float exposure) {
   // Bins share blocks only when there are more of them than the device holds
   // blocks at once (two per CU): a small frame gives every bin a whole block.
   const bool share = c->NB > 2 * c->cu_count;
+  // winner ids leave the depth buffer as it is: it may hold a later pass's
+  // depths (SSR water) that the ids do not describe
   HIPCHK(launch_bin_raster(mode, c->d_rec, B, empty, c->d_vpos, c->d_vattr, c->d_groups, dc,
-                           c->NBX, exposure, c->d_depth, c->d_r, c->d_g, c->d_b, c->d_rgba,
-                           c->d_ids, share ? c->d_large_ctl + 1 : nullptr, c->stream));
+                           c->NBX, exposure, mode == SHADE_IDS ? nullptr : c->d_depth, c->d_r,
+                           c->d_g, c->d_b, c->d_rgba, c->d_ids,
+                           share ? c->d_large_ctl + 1 : nullptr, c->stream));
   return 0;
 }
 
 int materialize(axr_ctx* c, int mode, float exposure) {
-  int rc = bin_raster(c, mode, exposure);
-  if (rc) return rc;
+  int rc;
+  if (c->accum && c->state == ST_PENDING) {
+    // a draw composed onto the frame: the pixels it wins, in HDR (with their
+    // winner ids); gamma, if asked for, from the planes
+    if ((rc = bin_raster(c, SHADE_ACCUM, 1.0f))) return rc;
+    c->accum = false;
+    c->state = ST_HDR;
+    if (mode == SHADE_LDR8) {
+      Timed tm(c, "gamma");
+      HIPCHK(launch_gamma_pack(c->d_r, c->d_g, c->d_b, c->d_rgba, (size_t)c->W * c->H, exposure,
+                               c->stream));
+      c->state = ST_LDR8;
+    }
+    return 0;
+  }
+  if ((rc = bin_raster(c, mode, exposure))) return rc;
   c->state = mode == SHADE_LDR8 ? ST_LDR8 : ST_HDR;
+  return 0;
+}
+
+// Before a drawAll onto a frame that already holds a draw (Ops.py:716-937
+// draws into the current buffers; only clearZBuffer resets them): the frame
+// so far gets its winner ids (the last draw's, from its bins) and its colour
+// in the planes, so the new draw can be rastered onto depth, planes and ids.
+int compose_prepare(axr_ctx* c) {
+  int rc;
+  if (!c->composed) {
+    if (!c->records_valid)
+      return fail(AXR_ESTATE, "drawAll onto a frame drawn before groups were added: clear first");
+    if ((rc = bin_raster(c, SHADE_IDS, 1.0f))) return rc;
+    c->composed = true;
+  }
+  if (c->state == ST_PENDING) {
+    if ((rc = materialize(c, SHADE_HDR, 1.0f))) return rc;
+  } else if (c->state == ST_LDR8) {
+    HIPCHK(launch_expand_rgba8(c->d_rgba, c->d_r, c->d_g, c->d_b, (size_t)c->W * c->H,
+                               c->stream));
+    c->state = ST_LDR16;
+  }
   return 0;
 }
 
@@ -667,6 +706,7 @@ int axr_clear_zbuffer(axr_ctx* c) {
   // and the per-bin walk starts from the clear key: nothing to launch.
   c->state = ST_CLEARED;
   c->frame_incomplete = false;
+  c->composed = c->accum = false;
   return AXR_OK;
 }
 
@@ -678,11 +718,13 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
     return fail(AXR_EINVAL, "drawAll: %d shader entries for %d groups", ngroups,
                 (int)c->groups.size());
   if (!shader_flags) return fail(AXR_EINVAL, "drawAll: null shaders");
-  if (c->state != ST_CLEARED)
-    return fail(AXR_ESTATE, "drawAll needs a clearZBuffer first (one drawAll per clear)");
+  if (c->state == ST_UNCLEARED) return fail(AXR_ESTATE, "drawAll needs a clearZBuffer first");
   if (nshadow < 0 || nshadow > AXR_SHADOW_BIND)
     return fail(AXR_EINVAL, "drawAll: at most %d shadow maps", AXR_SHADOW_BIND);
   if (pcf_taps != 5 && pcf_taps != 9) return fail(AXR_EINVAL, "pcf_taps must be 5 or 9");
+  // a draw onto the frame so far (no clearZBuffer since the last drawAll)
+  const bool onto = c->state != ST_CLEARED;
+  if (onto && (rc = compose_prepare(c))) return rc;
   AxrDrawConst dc = base_dc(c);
   dc.pcf_taps = pcf_taps;
   bool any_shadow = false, changed = false;
@@ -719,7 +761,12 @@ int axr_draw_all(axr_ctx* c, const uint32_t* shader_flags, const uint8_t* mask, 
   if (c->T > 0 && (rc = run_setup(c, &dc, nullptr, 0, c->T))) return rc;
   c->last = dc;
   c->records_valid = true;
-  c->state = c->T > 0 ? ST_PENDING : ST_CLEARED;
+  if (c->T > 0) {
+    c->state = ST_PENDING;
+    c->accum = onto;
+  } else if (!onto) {
+    c->state = ST_CLEARED;
+  }
   return AXR_OK;
 }
 
@@ -1081,12 +1128,16 @@ int axr_get_ids(axr_ctx* c, uint32_t* ids) {
   if ((rc = check_ctx(c))) return rc;
   if (!ids) return fail(AXR_EINVAL, "null output");
   if (c->state == ST_UNCLEARED) return fail(AXR_ESTATE, "getIds before any clearZBuffer");
-  if (c->state != ST_CLEARED && !c->records_valid)
+  if (!c->composed && c->state != ST_CLEARED && !c->records_valid)
     return fail(AXR_ESTATE, "getIds: groups were added since the last drawAll");
   if ((rc = flush_shadows(c))) return rc;
   const size_t n = (size_t)c->W * c->H;
-  // The winners of the current draw (also after it has been shaded).
-  if ((rc = bin_raster(c, SHADE_IDS, 1.0f))) return rc;
+  if (c->composed) {
+    // several draws since the clear: their winners are kept as each is composed
+    if (c->state == ST_PENDING && (rc = materialize(c, SHADE_HDR, 1.0f))) return rc;
+  } else if ((rc = bin_raster(c, SHADE_IDS, 1.0f))) {  // the current draw's winners
+    return rc;
+  }
   HIPCHK(hipMemcpyAsync(ids, c->d_ids, n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return check_overflow(c, true);

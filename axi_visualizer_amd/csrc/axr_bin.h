@@ -465,7 +465,15 @@ __device__ __forceinline__ void raster_bin(
   }
   const int x0 = bx << SHIFT, y0 = by << SHIFT;
   const int xe = imin(x0 + BIN, dc.W), ye = imin(y0 + BIN, dc.H);
-  for (int i = tid; i < NPX; i += nthr) lk[i] = key_clear();
+  constexpr bool ACC = MODE == SHADE_ACCUM;
+  for (int i = tid; i < NPX; i += nthr) {
+    uint64_t k0 = key_clear();
+    if (ACC) {  // the frame's depth with the lowest order: any later fragment wins a tie
+      const int x = x0 + (i & (BIN - 1)), y = y0 + (i >> SHIFT);
+      if (x < xe && y < ye) k0 = ((uint64_t)ordf(depth[(size_t)y * dc.W + x]) << 32) | 0xFFFFFFFFull;
+    }
+    lk[i] = k0;
+  }
   if (tid == 0) *ncov = 0;
   __syncthreads();
   if (!empty) walk_bin<SHIFT, false, DEAL>(lk, rec, B, s_small, s_large, bx, by, dc, tid, nthr);
@@ -489,10 +497,12 @@ __device__ __forceinline__ void raster_bin(
         id = order >= dc.T ? order - dc.T : order;
       }
       ids[p] = id;
-      depth[p] = unordf((uint32_t)(key >> 32));
+      if (depth) depth[p] = unordf((uint32_t)(key >> 32));  // null: ids alone
     } else if (low == 0xFFFFFFFFu) {
-      uint16_t rgb[3] = {dc.clear_rgb[0], dc.clear_rgb[1], dc.clear_rgb[2]};
-      put_px<MODE>(p, AXR_DEPTH_CLEAR, rgb, exposure, depth, ro, go, bo, rgba);
+      if (!ACC) {  // composing: the pixel keeps what the frame holds
+        uint16_t rgb[3] = {dc.clear_rgb[0], dc.clear_rgb[1], dc.clear_rgb[2]};
+        put_px<MODE>(p, AXR_DEPTH_CLEAR, rgb, exposure, depth, ro, go, bo, rgba);
+      }
     } else {
       covered = true;
     }
@@ -516,8 +526,12 @@ __device__ __forceinline__ void raster_bin(
     const uint64_t key = lk[i];
     uint16_t rgb[3];
     resolve_pixel(key, x, y, rec, vpos, vattr, groups, dc, rgb);
-    put_px<MODE>((size_t)y * dc.W + x, unordf((uint32_t)(key >> 32)), rgb, exposure, depth, ro,
-                 go, bo, rgba);
+    const size_t p = (size_t)y * dc.W + x;
+    put_px<MODE>(p, unordf((uint32_t)(key >> 32)), rgb, exposure, depth, ro, go, bo, rgba);
+    if (ACC) {
+      const uint32_t order = ((~(uint32_t)key) - 1u) >> 1;
+      ids[p] = order >= dc.T ? order - dc.T : order;
+    }
   }
 }
 

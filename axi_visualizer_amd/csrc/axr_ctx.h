@@ -169,6 +169,10 @@ struct axr_ctx {
   // the frame since the last clearZBuffer lost bin entries, or samples an
   // incomplete shadow map: every frame readback fails until the next clear
   bool frame_incomplete = false;
+  // More than one drawAll since the clear: the frame's winner ids are kept
+  // in d_ids as each draw is composed (composed), and the pending draw is to
+  // be rastered onto the frame rather than from the clear (accum).
+  bool composed = false, accum = false;
 
   bool timing = false;
   std::vector<TimingRec> trecs;

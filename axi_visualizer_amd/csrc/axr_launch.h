@@ -61,8 +61,11 @@ hipError_t launch_shadow_bins(const AxrBins& B, const AxrPassSet& ps, hipStream_
 hipError_t launch_stats(const uint8_t* cls, const uint8_t* ok, uint32_t T, uint32_t* out,
                         hipStream_t s);
 
-// Bin raster modes: HDR u16 planes, gamma fused into RGBA8, or winner ids.
-enum ShadeMode { SHADE_HDR = 0, SHADE_LDR8 = 1, SHADE_IDS = 2 };
+// Bin raster modes: HDR u16 planes, gamma fused into RGBA8, or winner ids;
+// SHADE_ACCUM draws onto the frame already there (a drawAll with no clear
+// before it): each pixel's key starts from its depth, and only pixels a new
+// fragment wins are written (HDR planes, depth and winner id).
+enum ShadeMode { SHADE_HDR = 0, SHADE_LDR8 = 1, SHADE_IDS = 2, SHADE_ACCUM = 3 };
 hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const AxrBins& B, int empty,
                              const float4* vpos, const float4* vattr, const AxrGroupDev* groups,
                              const AxrDrawConst& dc, int NBX, float exposure, float* depth,

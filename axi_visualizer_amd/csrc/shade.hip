@@ -213,6 +213,8 @@ hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const AxrBins& B, i
     AXR_LAUNCH_BIN(SHADE_LDR8);
   else if (mode == SHADE_HDR)
     AXR_LAUNCH_BIN(SHADE_HDR);
+  else if (mode == SHADE_ACCUM)
+    AXR_LAUNCH_BIN(SHADE_ACCUM);
   else
     AXR_LAUNCH_BIN(SHADE_IDS);
 #undef AXR_LAUNCH_BIN

@@ -93,7 +93,9 @@ int axr_set_clear_color(axr_ctx* ctx, const uint16_t rgb[3]);
  * shader_flags[tn]: AXR_SH_* bits; mask[tn] != 0 skips the group (the
  * reference's mask sense).  shadow_ids: shadow maps sampled by "shadow"
  * groups (at most 2).  pcf_taps: 5 (reference cross) or 9 (3x3 grid).
- * One drawAll per clearZBuffer. */
+ * The first drawAll after axr_create needs a clearZBuffer; a later drawAll
+ * with no clear before it draws onto the frame (its depth test against the
+ * depth the earlier draws left), as the reference's does. */
 int axr_draw_all(axr_ctx* ctx, const uint32_t* shader_flags, const uint8_t* mask, int ngroups,
                  const int* shadow_ids, int nshadow, int pcf_taps);
 

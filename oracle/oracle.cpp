@@ -1498,13 +1498,22 @@ int orc_render(const OrcGroup* groups, int ngroups, const OrcFrame* fr, OrcOut* 
       std::memcpy(out->shadow[c], cx.sm[c].F.data(), cx.sm[c].F.size() * sizeof(float));
   }
 
-  // clearZBuffer: depth 65535 (clearzb.c); colour to the clear colour.
+  // clearZBuffer: depth 65535 (clearzb.c); colour to the clear colour.  A
+  // drawAll with no clear before it draws onto the frame it is given.
   const int64_t NP = (int64_t)W * H;
-  cx.F.assign(NP, 65535.0f);
-  cx.id.assign(NP, 0xFFFFFFFFu);
-  cx.R.assign(NP, fr->clear_rgb[0]);
-  cx.G.assign(NP, fr->clear_rgb[1]);
-  cx.B.assign(NP, fr->clear_rgb[2]);
+  if (fr->init_depth && fr->init_id && fr->init_r && fr->init_g && fr->init_b) {
+    cx.F.assign(fr->init_depth, fr->init_depth + NP);
+    cx.id.assign(fr->init_id, fr->init_id + NP);
+    cx.R.assign(fr->init_r, fr->init_r + NP);
+    cx.G.assign(fr->init_g, fr->init_g + NP);
+    cx.B.assign(fr->init_b, fr->init_b + NP);
+  } else {
+    cx.F.assign(NP, 65535.0f);
+    cx.id.assign(NP, 0xFFFFFFFFu);
+    cx.R.assign(NP, fr->clear_rgb[0]);
+    cx.G.assign(NP, fr->clear_rgb[1]);
+    cx.B.assign(NP, fr->clear_rgb[2]);
+  }
 
   int64_t st[5];
   draw_pass(cx, false, nthreads, st, [&](const OrcGroup& G, const MipInfo& mi, uint32_t gi,

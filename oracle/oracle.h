@@ -101,6 +101,14 @@ typedef struct {
   int ssr_steps, ssr_step; /* REFL_LENGTH, REFL_STEP */
   float ssr_absorb;        /* ABSORB */
   int ssr_fresnel;         /* 0: fr = 1 (metal variant) */
+  /* optional: the frame this drawAll draws onto (no clearZBuffer before it,
+   * Ops.py:716-946): depth, winner ids and colour planes as a previous
+   * render left them (exposure 0); null: start from the clear */
+  const float* init_depth;
+  const uint32_t* init_id;
+  const uint16_t* init_r;
+  const uint16_t* init_g;
+  const uint16_t* init_b;
 } OrcFrame;
 
 typedef struct {

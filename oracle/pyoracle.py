@@ -46,7 +46,10 @@ class OrcFrame(ctypes.Structure):
                 ("refl_r", ctypes.c_void_p * 4), ("refl_g", ctypes.c_void_p * 4),
                 ("refl_b", ctypes.c_void_p * 4), ("refl_size", ctypes.c_int * 4),
                 ("ssr_steps", ctypes.c_int), ("ssr_step", ctypes.c_int),
-                ("ssr_absorb", ctypes.c_float), ("ssr_fresnel", ctypes.c_int)]
+                ("ssr_absorb", ctypes.c_float), ("ssr_fresnel", ctypes.c_int),
+                ("init_depth", ctypes.c_void_p), ("init_id", ctypes.c_void_p),
+                ("init_r", ctypes.c_void_p), ("init_g", ctypes.c_void_p),
+                ("init_b", ctypes.c_void_p)]
 
 
 class OrcOut(ctypes.Structure):
@@ -141,9 +144,11 @@ def scene_vertlight(scene, xyz, vn):
                      pointI=pts[0], pointP=pts[1], spotI=sps[0], spotD=sps[1], spotP=sps[2])
 
 
-def render(scene, nthreads=0, with_shadows_out=False):
+def render(scene, nthreads=0, with_shadows_out=False, onto=None):
     """Renders a scenes.Scene: shadow cascades, clear, drawAll, gamma.
-    Returns dict(depth, id, r, g, b, stats[, shadow])."""
+    Returns dict(depth, id, r, g, b, stats[, shadow]).  onto: a previous
+    render's output (rendered with exposure 0) that this drawAll draws onto
+    instead of a cleared frame."""
     keep = []
     groups = (OrcGroup * len(scene.groups))()
     for k, g in enumerate(scene.groups):
@@ -190,6 +195,11 @@ def render(scene, nthreads=0, with_shadows_out=False):
     fr.exposure = float(scene.exposure)
     fr.clear_rgb[:] = list(scene.clear_rgb)
     fr.nthreads = nthreads
+    if onto is not None:
+        init = [_c(onto[k], t) for k, t in (("depth", np.float32), ("id", np.uint32),
+                                            ("r", np.uint16), ("g", np.uint16), ("b", np.uint16))]
+        keep += init
+        fr.init_depth, fr.init_id, fr.init_r, fr.init_g, fr.init_b = (x.ctypes.data for x in init)
     n = scene.W * scene.H
     out = {"depth": np.empty((scene.H, scene.W), np.float32),
            "id": np.empty((scene.H, scene.W), np.uint32),

@@ -56,10 +56,12 @@ def test_shadow_clear_is_lazy_and_map_rerender_is_idempotent():
         r.close()
 
 
-def test_second_draw_without_clear_is_refused():
+def test_draw_before_any_clear_is_refused():
+    """A new context's buffers hold nothing to draw onto: the first drawAll
+    needs a clearZBuffer (later ones compose onto the frame,
+    test_gpu_compose.py)."""
     r = Renderer(small_spheres())
     try:
-        r.frame()
         with pytest.raises(RuntimeError, match="clearZBuffer"):
             r.draw.drawAll(r.shaders, mask=r.mask)
     finally:

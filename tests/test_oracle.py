@@ -444,6 +444,32 @@ def test_equal_depth_last_in_order_wins():
     np.testing.assert_allclose(out["depth"][cov], 4.0, rtol=2e-7)
 
 
+def _only(sc, keep):
+    """The scene with every group but `keep` skipped (drawAll's mask)."""
+    import dataclasses
+    return dataclasses.replace(sc, groups=[dataclasses.replace(g, skip=k != keep)
+                                           for k, g in enumerate(sc.groups)])
+
+
+@pytest.mark.parametrize("z_b,winner", [(3.0, 1), (7.0, 0), (5.0, 1)])
+def test_draw_onto_a_frame_without_clear(z_b, winner):
+    """A second drawAll with no clearZBuffer before it tests its fragments
+    against the depth the first left (F >= tz, Templates.py:185): a nearer or
+    equally deep second quad takes the pixels, a farther one leaves the first
+    draw's ids, depth and colour as they were; pixels neither covers keep the
+    clear."""
+    sc = _quad_scene(5.0, z_b)
+    first = po.render(_only(sc, 0))
+    both = po.render(_only(sc, 1), onto=first)
+    cov = first["id"] != 0xFFFFFFFF
+    assert cov.mean() > 0.8
+    assert (both["id"][cov] == winner).all() and (both["id"][~cov] == 0xFFFFFFFF).all()
+    alone = po.render(_only(sc, winner))
+    for k in ("depth", "r", "g", "b"):
+        assert np.array_equal(both[k][cov], alone[k][cov]), k
+    assert np.array_equal(both["r"][~cov], first["r"][~cov])
+
+
 def test_uniform_texture_shading_closed_form():
     """Base shader (av_draw_base.c:44-55) with a uniform texture c, normal n,
     I = amb and no shadow: colour = c * (max(0, n.L) * LInt + amb) up to
