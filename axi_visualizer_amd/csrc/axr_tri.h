@@ -801,49 +801,53 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
     zs[2] = b3 / D32;
     zs[3] = b4 / D32;
   }
-  {
-    float slope1 = div_rn((float)(x2 - x1), D21, r21);
-    float slope2 = div_rn((float)(x4 - x1), D21, r21);
-    float slopez1 = zs[0];
-    float slopez2 = zs[1];
-    float cx1 = (float)x1, cx2 = cx1, cz1 = z1, cz2 = z1;
-    if (slope1 < slope2) {
-      swp(slope1, slope2);
-      swp(slopez1, slopez2);
-    }
-    for (int cy = y1; cy <= y2; cy++) {
-      if (cy >= yhi) break;
-      if (cy >= ylo) {
-        span_run(imax(f2i(cx2), xlo), imin(f2i(cx1), xhi - 1), cy, cx1, cx2, cz1, cz2, frag,
-                 frag_key, proven);
-      }
-      cx1 += slope1;
-      cx2 += slope2;
-      cz1 += slopez1;
-      cz2 += slopez2;
-    }
+  // Both halves in one row loop: rows y1.. of the top half (edges from
+  // vertex 1 down, drawmin.c's first loop), then rows y3.. of the bottom
+  // half (edges from vertex 3 up, its second loop), each accumulated exactly
+  // as its own loop does (x - s is x + (-s)).  One loop lets a wave's lanes
+  // run their rows back to back: a caster with a long top and a short
+  // bottom half no longer waits for another's long bottom half.
+  float t1 = div_rn((float)(x2 - x1), D21, r21), t2 = div_rn((float)(x4 - x1), D21, r21);
+  float tz1 = zs[0], tz2 = zs[1];
+  if (t1 < t2) {
+    swp(t1, t2);
+    swp(tz1, tz2);
   }
-  {
-    float slope1 = div_rn((float)(x3 - x2), D32, r32);
-    float slope2 = div_rn((float)(x3 - x4), D32, r32);
-    float slopez1 = zs[2];
-    float slopez2 = zs[3];
-    float cx1 = (float)x3, cx2 = cx1, cz1 = z3, cz2 = z3;
-    if (slope1 < slope2) {
-      swp(slope1, slope2);
-      swp(slopez1, slopez2);
+  float b1 = div_rn((float)(x3 - x2), D32, r32), b2 = div_rn((float)(x3 - x4), D32, r32);
+  float bz1 = zs[2], bz2 = zs[3];
+  if (b1 < b2) {
+    swp(b1, b2);
+    swp(bz1, bz2);
+  }
+  // rows of the top half before its first row at or past yhi; of the bottom
+  // half before its first row above ylo
+  const int n0 = imax(0, imin(y2, yhi - 1) - y1 + 1);
+  const int n1 = imax(0, y3 - imax(y2 + 1, ylo) + 1);
+  float cx1 = (float)x1, cx2 = cx1, cz1 = z1, cz2 = z1;
+  float s1 = t1, s2 = t2, sz1 = tz1, sz2 = tz2;
+  int cy = y1, dy = 1;
+  for (int i = 0; i < n0 + n1; i++) {
+    if (i == n0) {  // the bottom half starts at vertex 3
+      cx1 = cx2 = (float)x3;
+      cz1 = cz2 = z3;
+      s1 = -b1;
+      s2 = -b2;
+      sz1 = -bz1;
+      sz2 = -bz2;
+      cy = y3;
+      dy = -1;
     }
-    for (int cy = y3; cy > y2; cy--) {
-      if (cy < ylo) break;
-      if (cy < yhi) {
-        span_run(imax(f2i(cx1), xlo), imin(f2i(cx2), xhi - 1), cy, cx1, cx2, cz1, cz2, frag,
-                 frag_key, proven);
-      }
-      cx1 -= slope1;
-      cx2 -= slope2;
-      cz1 -= slopez1;
-      cz2 -= slopez2;
+    const bool top = i < n0;
+    if (top ? cy >= ylo : cy < yhi) {
+      const float lo = top ? cx2 : cx1, hi = top ? cx1 : cx2;
+      span_run(imax(f2i(lo), xlo), imin(f2i(hi), xhi - 1), cy, cx1, cx2, cz1, cz2, frag, frag_key,
+               proven);
     }
+    cx1 += s1;
+    cx2 += s2;
+    cz1 += sz1;
+    cz2 += sz2;
+    cy += dy;
   }
 }
 
