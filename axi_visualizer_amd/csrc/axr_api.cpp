@@ -195,7 +195,17 @@ AxrGroupDev group_dev(const HostGroup& g) {
   return d;
 }
 
+// Orders the scene's stream after a cube pass still reading the scene.
+int wait_cube(axr_ctx* c) {
+  if (!c->cube_pending) return 0;
+  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_cube, 0));
+  c->cube_pending = false;
+  return 0;
+}
+
 int upload_groups(axr_ctx* c) {
+  int rc;
+  if ((rc = wait_cube(c))) return rc;
   std::vector<AxrGroupDev> h(c->groups.size());
   for (size_t i = 0; i < h.size(); i++) h[i] = group_dev(c->groups[i]);
   // Kernels in flight read the table: let them finish before rewriting it.
@@ -296,6 +306,8 @@ bool bound_to_pending(axr_ctx* c, int i) {
 int apply_shadow_clear(axr_ctx* c, int i) {
   HostShadow& s = c->sm[i];
   if (!s.clear_pending) return 0;
+  int rc;
+  if ((rc = wait_cube(c))) return rc;
   Timed tm(c, "shadow_clear");
   HIPCHK(launch_fill_u32(s.d_map, (size_t)s.dim * s.dim, ordf(AXR_DEPTH_CLEAR), c->stream));
   s.clear_pending = false;
@@ -395,6 +407,7 @@ int run_setup(axr_ctx* c, const AxrDrawConst* dc_main, const AxrGroupDev* table,
   }
   c->pend.clear();
   int rc;
+  if (ps.n && (rc = wait_cube(c))) return rc;  // the passes write shadow maps
   if (ps.n && (rc = bins_prepare(c->sbins, nb_total, c->stream))) return rc;
   if (dc_main && (rc = bins_prepare(c->mbins, c->NB, c->stream))) return rc;
   AxrDrawConst dc{};
@@ -491,7 +504,11 @@ int axr_create(int w, int h, axr_ctx** out) {
 int axr_destroy(axr_ctx* c) {
   if (!c) return AXR_OK;
   (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream) {
+    (void)wait_cube(c);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  if (c->ev_cube) (void)hipEventDestroy(c->ev_cube);
   for (auto& r : c->trecs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -549,6 +566,7 @@ int axr_add_texture_group(axr_ctx* c, const float* xyz, const float* uv, const f
     return fail(AXR_ECAP, "at most %d groups per context", AXR_MAX_GROUPS);
   if ((uint64_t)c->T + (uint64_t)ntri >= (1ull << 30))
     return fail(AXR_ECAP, "at most 2^30 triangles per context");
+  if ((rc = wait_cube(c))) return rc;  // geometry and textures may be reallocated
   // Texture shape: N*N texels, the full smallest-first chain of N*N, or
   // its N*N base level (chain built here).
   if (is_mip < 0 || is_mip > 2) return fail(AXR_EINVAL, "is_mip must be 0, 1 or 2");
@@ -672,6 +690,7 @@ int axr_vert_light(axr_ctx* c, const uint8_t* mask, int ngroups, const float* di
   L.spotD = c->d_light + oSD;
   L.spotP = c->d_light + oSP;
   c->amb = amb;
+  if ((rc = wait_cube(c))) return rc;  // the lights are written into the vertex attributes
   for (int g = 0; g < ngroups; g++) {
     if (!mask[g]) continue;
     const HostGroup& hg = c->groups[g];
@@ -787,6 +806,7 @@ int axr_set_refl_tex(axr_ctx* c, int slot, const uint16_t* r, const uint16_t* g,
   const size_t n = (size_t)6 * size * size;
   std::vector<ushort4> h(n);
   for (size_t i = 0; i < n; i++) h[i] = make_ushort4(r[i], g[i], b[i], 0);
+  if ((rc = wait_cube(c))) return rc;  // a cube pass may still write the strip
   HIPCHK(hipStreamSynchronize(c->stream));
   if (t.size != size) {
     dfree(t.d);
@@ -805,6 +825,7 @@ int axr_get_refl_tex(axr_ctx* c, int slot, uint16_t* r, uint16_t* g, uint16_t* b
   const ReflTex& t = c->refl[(size_t)slot];
   const size_t n = (size_t)6 * t.size * t.size;
   std::vector<ushort4> h(n);
+  if ((rc = wait_cube(c))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(h.data(), t.d, n * sizeof(ushort4), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < n; i++) {
@@ -841,6 +862,9 @@ int axr_draw_ssr(axr_ctx* c, const int* groups, const int* refl, int n, int step
   if (c->state == ST_CLEARED || c->state == ST_PENDING) {
     if ((rc = materialize(c, SHADE_HDR, 1.0f))) return rc;
   }
+  // the water samples cube strips a cube pass may still be rendering (the
+  // opaque frame above did not need to wait for it)
+  if ((rc = wait_cube(c))) return rc;
   const size_t np = (size_t)c->W * c->H;
   // planes of the snapshot start 16-byte aligned: a stride of np rounded up to 8
   const size_t np8 = (np + 7) & ~(size_t)7;

@@ -152,6 +152,13 @@ struct axr_ctx {
   AxrGroupDev* d_groups_ssr = nullptr;   // group table of SSR draws
   std::vector<AxrGroupDev> h_groups_ssr;  // as last uploaded
   std::vector<ReflTex> refl;
+  // A cube pass (axr_render_cube) renders this scene on a batch's stream:
+  // its reads of the geometry, group table, textures and shadow maps and its
+  // write of the reflection strip end at ev_cube.  The scene's stream waits
+  // on it (wait_cube) only before it writes one of those or reads a strip,
+  // so the scene's own drawAll overlaps the cube pass.
+  hipEvent_t ev_cube = nullptr;
+  bool cube_pending = false;
   float* d_light = nullptr;  // vertLight light arrays
   size_t light_cap = 0;
 
@@ -231,5 +238,6 @@ void bins_grow(BinStore& S, uint32_t spilled, uint32_t lost);
 int check_ctx(axr_ctx* c);
 int apply_shadow_clear(axr_ctx* c, int i);
 int flush_shadows(axr_ctx* c);
+int wait_cube(axr_ctx* c);
 
 }  // namespace axri

@@ -659,12 +659,13 @@ int axr_render_cube(axr_batch* b, axr_ctx* scene, int slot, const float pos[3], 
   }
   HIPCHK(launch_pack_cube(b->d_hdr, b->d_hdr + b->hdr_cap, b->d_hdr + 2 * b->hdr_cap, size, t.d,
                           b->stream));
-  // the scene's later draws (an SSR pass) read the strip after this
-  hipEvent_t ev;
-  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  HIPCHK(hipEventRecord(ev, b->stream));
-  HIPCHK(hipStreamWaitEvent(scene->stream, ev, 0));
-  HIPCHK(hipEventDestroy(ev));
+  // The scene's stream waits for this only where it reads the strip (an
+  // SSR pass) or writes what the pass reads (wait_cube): its own drawAll
+  // runs beside the cube pass.
+  if (!scene->ev_cube)
+    HIPCHK(hipEventCreateWithFlags(&scene->ev_cube, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(scene->ev_cube, b->stream));
+  scene->cube_pending = true;
   HIPCHK(hipEventRecord(b->ev_done, b->stream));
   return slot;
 }

@@ -98,7 +98,10 @@ class FullFrame(Renderer):
         self.draw.setShaders([0 if w else s for s, w in zip(self.shaders, water)],
                              mask=[m or w for m, w in zip(self.mask, water)])
 
-    def frame(self, gamma=True, after_cascade=None):
+    def frame(self, gamma=True, after_cascade=None, serial=False):
+        """One frame.  The cube pass runs on its own stream beside the opaque
+        drawAll (the SSR pass waits for it); serial=True finishes it before
+        the drawAll is issued (per-pass timing without the overlap)."""
         sc, d = self.scene, self.draw
         for i, c in enumerate(sc.cascades):
             d.clearShadowMap(i)
@@ -107,6 +110,8 @@ class FullFrame(Renderer):
                 after_cascade(i)
         self.cube.render_cube(d, 0, self.probe, self.cube_size, shadows=bool(sc.cascades),
                               pcf_taps=sc.pcf_taps)
+        if serial:
+            self.cube.wait()
         d.setPos(sc.view.pos)
         d.setVM(sc.view.vm)
         d.clearZBuffer()

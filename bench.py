@@ -600,11 +600,13 @@ def run_full_frame(args):
               shadowIds=list(range(len(sc.cascades))), pcf=sc.pcf_taps)
     st = d.stats()
     d.sync()
+    # per-pass times with the cube pass finished before the drawAll is
+    # issued (the timed frames above overlap the two)
     d.setTiming(True)
     nt = max(10, args.steps // 4)
     cube_ms = []
     for _ in range(nt):
-        ff.frame()
+        ff.frame(serial=True)
         cube_ms.append(ff.cube.wait())
     tm = d.timing()
     d.setTiming(False)
