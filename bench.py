@@ -47,20 +47,20 @@ CPU_SHARE_MAX = 16
 # `bench.py --config c2 --steps 5 --warmup 2 --latency-frames 2
 # --no-cpu-baseline --calibrate-copy 256`, each scaled by what the same
 # counter reads for the plain 256 MiB device copy (fetch x 1.9997, write
-# x 1.0000); setup_bin is k_setup + k_coarse.  Valid for the kernels as
-# they are; re-measure after changing one.
+# x 1.0000), headline pose; setup_bin is k_setup + k_coarse.  Valid for
+# the kernels as they are; re-measure after changing one.
 MEASURED_TRAFFIC = {
-    "c2": {"shadow_raster": 95281021},
+    "c2": {"shadow_raster": 95276286, "raster_shade": 134329023},
     # the batch raster of the bench's own batch (the 64 views; the first 64
     # requests of the saturated burst), same calibration, from
     # `bench.py --config c4|c5 --steps 5 --warmup 2 --latency-frames 2
     # --serve-seconds 0.3 --no-cpu-baseline --calibrate-copy 256`
-    "c4": {"batch_raster": 2257594956},
-    "c5": {"batch_raster": 5253218519},
+    "c4": {"batch_raster": 2256144798},
+    "c5": {"batch_raster": 5258640272},
     # the opaque HDR raster of the 1440p full frame, from
     # `bench.py --config c3 --steps 5 --warmup 2 --latency-frames 2
     # --no-cpu-baseline --calibrate-copy 256`
-    "c3": {"raster_hdr": 157939038},
+    "c3": {"raster_hdr": 157900476},
 }
 
 
