@@ -405,6 +405,31 @@ __device__ __forceinline__ void span_run(int xa, int xb, int cy, float cx1, floa
   }
 }
 
+// span_run for a row of a caster shadow_fast proves: rn_row's ranges hold,
+// and a zero span width needs no division either.  IEEE gives (ax - cx2)/0
+// = +inf or NaN for ax >= cx2 and -inf below, which the clamp to [0, 1]
+// turns into t = 1 or 0.  Such a row (cx1 == cx2) holds at most one texel,
+// so the row clamps to [s, s] with s that texel's t, which the clamp
+// returns whatever q the division of a zero width left: the texel loop is
+// the same straight-line code for every row.  The clamp is the hardware's
+// median of three: q itself is never NaN on a row of nonzero width (the
+// proven ranges), and with lo == hi the median is lo whatever q is.  It
+// matches omax(lo, omin(hi, q)) up to the sign of a zero t, which the
+// interpolation (1 - t)*cz2 + t*cz1 of nonzero depths does not see.
+template <class K>
+__device__ __forceinline__ void span_run_proven(int xa, int xb, int cy, float cx1, float cx2,
+                                                float cz1, float cz2, K&& frag_key) {
+  const float d = cx1 - cx2, y = rcp_rn(d);
+  const float s = (float)xa - cx2 < 0.0f ? 0.0f : 1.0f;
+  const bool flat = d == 0.0f;
+  const float lo = flat ? s : 0.0f, hi = flat ? s : 1.0f;
+  const uint32_t flip = cz1 > 0.0f ? 0x80000000u : 0xffffffffu;
+  for (int ax = xa; ax <= xb; ax++) {
+    const float t = __builtin_amdgcn_fmed3f(div_rn((float)ax - cx2, d, y), lo, hi);
+    frag_key(ax, cy, __float_as_uint(rcp_rn((1 - t) * cz2 + t * cz1)) ^ flip);
+  }
+}
+
 // ------------------------------------------------- a7: small-path walk
 // Small triangles whose rows can all take div_rn and rcp_rn: vertex x in
 // [1, 2^20], so every span end (rounding accumulated over fewer than 128
@@ -823,32 +848,44 @@ __device__ __forceinline__ void shadow_walk(const AxrShRec& r, int dim, int xlo,
   // half before its first row above ylo
   const int n0 = imax(0, imin(y2, yhi - 1) - y1 + 1);
   const int n1 = imax(0, y3 - imax(y2 + 1, ylo) + 1);
-  float cx1 = (float)x1, cx2 = cx1, cz1 = z1, cz2 = z1;
-  float s1 = t1, s2 = t2, sz1 = tz1, sz2 = tz2;
-  int cy = y1, dy = 1;
-  for (int i = 0; i < n0 + n1; i++) {
-    if (i == n0) {  // the bottom half starts at vertex 3
-      cx1 = cx2 = (float)x3;
-      cz1 = cz2 = z3;
-      s1 = -b1;
-      s2 = -b2;
-      sz1 = -bz1;
-      sz2 = -bz2;
-      cy = y3;
-      dy = -1;
+  auto rows = [&](auto all_proven) {
+    float cx1 = (float)x1, cx2 = cx1, cz1 = z1, cz2 = z1;
+    float s1 = t1, s2 = t2, sz1 = tz1, sz2 = tz2;
+    int cy = y1, dy = 1;
+    for (int i = 0; i < n0 + n1; i++) {
+      if (i == n0) {  // the bottom half starts at vertex 3
+        cx1 = cx2 = (float)x3;
+        cz1 = cz2 = z3;
+        s1 = -b1;
+        s2 = -b2;
+        sz1 = -bz1;
+        sz2 = -bz2;
+        cy = y3;
+        dy = -1;
+      }
+      // top-half rows lie above yhi and bottom-half rows at or below ylo
+      // by n0 and n1, so one test of both bounds serves either half
+      const bool top = i < n0;
+      if ((unsigned)(cy - ylo) < (unsigned)(yhi - ylo)) {
+        const float lo = top ? cx2 : cx1, hi = top ? cx1 : cx2;
+        const int xa = imax(f2i(lo), xlo), xb = imin(f2i(hi), xhi - 1);
+        if (decltype(all_proven)::value)
+          span_run_proven(xa, xb, cy, cx1, cx2, cz1, cz2, frag_key);
+        else
+          span_run(xa, xb, cy, cx1, cx2, cz1, cz2, frag, frag_key, proven);
+      }
+      cx1 += s1;
+      cx2 += s2;
+      cz1 += sz1;
+      cz2 += sz2;
+      cy += dy;
     }
-    const bool top = i < n0;
-    if (top ? cy >= ylo : cy < yhi) {
-      const float lo = top ? cx2 : cx1, hi = top ? cx1 : cx2;
-      span_run(imax(f2i(lo), xlo), imin(f2i(hi), xhi - 1), cy, cx1, cx2, cz1, cz2, frag, frag_key,
-               proven);
-    }
-    cx1 += s1;
-    cx2 += s2;
-    cz1 += sz1;
-    cz2 += sz2;
-    cy += dy;
-  }
+  };
+  // a wave whose casters are all proven takes the one-path row loop
+  if (__ballot(!proven) == 0)
+    rows(std::true_type{});
+  else
+    rows(std::false_type{});
 }
 
 // Texel rectangle containing every fragment of a caster: rows y1..y3;
@@ -860,6 +897,26 @@ __device__ __forceinline__ void shadow_bounds(const AxrShRec& r, int& x0, int& y
   x1 = imax(r.x[0], imax(r.x[1], r.x[2])) + 2;
   y0 = imin(r.y[0], imin(r.y[1], r.y[2]));
   y1 = imax(r.y[0], imax(r.y[1], r.y[2]));
+}
+
+// Work class of a caster, by its rows y_max - y_min + 1: 1-2, 3, 4-6, 7 and
+// more give 0..3.  Setup stores it in the top two bits of the caster's
+// shadow-bin entries (triangle indices stay below 2^30) so that the walk
+// can order a bin's casters by size without loading their records.
+#ifndef AXR_SHC_LO
+#define AXR_SHC_LO 3
+#endif
+#ifndef AXR_SHC_MID
+#define AXR_SHC_MID 4
+#endif
+#ifndef AXR_SHC_HI
+#define AXR_SHC_HI 7
+#endif
+#define AXR_SH_ENTRY_MASK 0x3fffffffu
+__device__ __forceinline__ uint32_t shadow_class(const AxrShRec& r) {
+  const int rows = imax(r.y[0], imax(r.y[1], r.y[2])) - imin(r.y[0], imin(r.y[1], r.y[2])) + 1;
+  return (uint32_t)(rows >= AXR_SHC_LO) + (uint32_t)(rows >= AXR_SHC_MID) +
+         (uint32_t)(rows >= AXR_SHC_HI);
 }
 
 // Pipe/trisetupOrtho.c:20-59.

@@ -169,10 +169,12 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
       if (ok) P.rec[gi] = r;
     }
     BinRect br = {0, 0, 1, 0};
+    uint32_t entry = gi;
     if (ok) {
       int x0, y0, x1, y1;
       shadow_bounds(r, x0, y0, x1, y1);
       br = bin_rect(x0, y0, x1, y1, P.dim, P.dim, AXR_SHADOW_SHIFT);
+      entry |= shadow_class(r) << 30;
     }
     BinCursor bc;
     for (int k = 0;; k++) {
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
       if (__ballot(act) == 0) break;
       const uint32_t slot = act ? P.slot0 + bc.bin(br, P.nbx) : 0u;
       const uint32_t pos = wave_bin_add(SB.cnt, act, slot);
-      if (act) bin_store(SB, slot, pos, gi);
+      if (act) bin_store(SB, slot, pos, entry);
       bc.next(br);
     }
   }
@@ -491,11 +493,34 @@ __global__ __launch_bounds__(kWideThreads) void k_order_scatter(AxrBins B, Order
   }
 }
 
+// The lanes of a wave in each of four classes: of(k) is the mask of the
+// valid lanes whose class is k (k < 4).
+struct ClassMasks {
+  uint64_t b0, b1, vm;
+  __device__ __forceinline__ ClassMasks(bool valid, uint32_t c)
+      : b0(__ballot(valid && (c & 1u))), b1(__ballot(valid && (c & 2u))), vm(__ballot(valid)) {}
+  __device__ __forceinline__ uint64_t of(uint32_t k) const {
+    return ((k & 1u) ? b0 : vm & ~b0) & ((k & 2u) ? b1 : vm & ~b1);
+  }
+};
+
+// Entries a shadow bin sorts in LDS; a bin with more (or with spilled
+// entries) walks them in bucket order.
+#ifndef AXR_SHADOW_SORT
+#define AXR_SHADOW_SORT 8192
+#endif
+#ifndef AXR_SHADOW_SORT_MIN
+#define AXR_SHADOW_SORT_MIN 64
+#endif
+
 // K3: one block per 64x64 texel bin of every pending map, heavy bins first;
 // minimum depth keys in LDS, written once.  A bin starts from its map
 // (from_map) or from the clear depth.
 __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, AxrPassSet ps) {
   __shared__ uint32_t lk[AXR_SHADOW_BIN * AXR_SHADOW_BIN];
+  __shared__ uint32_t srt[AXR_SHADOW_SORT];  // the bin's entries by class
+  __shared__ uint32_t ccur[4];               // class counts, then cursors
+  if (threadIdx.x < 4) ccur[threadIdx.x] = 0;
   const uint32_t slot = B.order[blockIdx.x];
   // the pass owning the slot (slot0 ascends); selected field by field so the
   // by-value pass table is never indexed dynamically
@@ -519,14 +544,31 @@ __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, A
   if (from_map && n == 0) return;
   const int x0 = (b % nbx) << AXR_SHADOW_SHIFT, y0 = (b / nbx) << AXR_SHADOW_SHIFT;
   const int xe = imin(x0 + AXR_SHADOW_BIN, dim), ye = imin(y0 + AXR_SHADOW_BIN, dim);
+  const int tid = (int)threadIdx.x, nthr = (int)blockDim.x, lane = tid & 63;
+  // A bin of at most AXR_SHADOW_SORT entries, none spilled, is walked with
+  // its casters ordered by work class, largest first (a counting sort into
+  // LDS), so that a wave walks casters of like row counts instead of
+  // waiting on its longest one.  Its entries are loaded once, up front.
+  const bool sorted = n > AXR_SHADOW_SORT_MIN && n <= B.cap0 && n <= AXR_SHADOW_SORT;
+  constexpr int kPer = AXR_SHADOW_SORT / AXR_SHADOW_THREADS;
+  static_assert(kPer * AXR_SHADOW_THREADS == AXR_SHADOW_SORT, "whole rounds of the block");
+  uint32_t v[kPer];
+  if (sorted) {
+    const uint32_t* __restrict__ ent = B.ent + bin_base(B, slot);
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const uint32_t e = (uint32_t)(u * AXR_SHADOW_THREADS + tid);
+      v[u] = e < n ? ent[e] : 0u;
+    }
+  }
   const uint32_t clear = ordf(AXR_DEPTH_CLEAR);
-  for (int i = threadIdx.x; i < AXR_SHADOW_BIN * AXR_SHADOW_BIN; i += blockDim.x) {
+  for (int i = tid; i < AXR_SHADOW_BIN * AXR_SHADOW_BIN; i += nthr) {
     const int x = x0 + (i & (AXR_SHADOW_BIN - 1)), y = y0 + (i >> AXR_SHADOW_SHIFT);
     lk[i] = (from_map && x < xe && y < ye) ? map[(size_t)y * dim + x] : clear;
   }
-  __syncthreads();
-  for_bin_entries(B, slot, n, (int)threadIdx.x, (int)blockDim.x, [&](uint32_t gi) {
-    const AxrShRec r = rec[gi];
+  __syncthreads();  // lk and the class counts start cleared
+  auto walk = [&](uint32_t entry) {
+    const AxrShRec r = rec[entry & AXR_SH_ENTRY_MASK];
     // texels lie in [x0, xe) x [y0, ye): shadow_walk clips every row to
     // the bin, so the tile index stays inside the LDS tile
     shadow_walk(
@@ -538,7 +580,37 @@ __global__ __launch_bounds__(AXR_SHADOW_THREADS) void k_shadow_bins(AxrBins B, A
         [&](int ax, int cy, uint32_t key) {
           atomicMin(&lk[((cy - y0) << AXR_SHADOW_SHIFT) + (ax - x0)], key);
         });
-  });
+  };
+  if (sorted) {
+    // lanes 0..3 of a wave keep, per round, the wave's first position among
+    // the bin's casters of class `lane`
+    uint32_t wb[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      wb[u] = 0;
+      if ((uint32_t)(u * AXR_SHADOW_THREADS) >= n) break;
+      const uint32_t e = (uint32_t)(u * AXR_SHADOW_THREADS + tid);
+      const ClassMasks m(e < n, v[u] >> 30);
+      const uint64_t ml = m.of((uint32_t)lane);
+      if (lane < 4 && ml) wb[u] = atomicAdd(&ccur[lane], (uint32_t)__popcll(ml));
+    }
+    __syncthreads();
+    const uint32_t n3 = ccur[3], n2 = ccur[2], n1 = ccur[1];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      if ((uint32_t)(u * AXR_SHADOW_THREADS) >= n) break;
+      const uint32_t e = (uint32_t)(u * AXR_SHADOW_THREADS + tid), c = v[u] >> 30;
+      const ClassMasks m(e < n, c);
+      // classes 3, 2, 1, 0 in turn
+      const uint32_t off = c == 3 ? 0u : c == 2 ? n3 : c == 1 ? n3 + n2 : n3 + n2 + n1;
+      const uint32_t base = (uint32_t)__shfl((int)wb[u], (int)c);
+      if (e < n) srt[off + base + (uint32_t)__popcll(m.of(c) & lanemask_lt())] = v[u];
+    }
+    __syncthreads();
+    for (uint32_t e = (uint32_t)tid; e < n; e += (uint32_t)nthr) walk(srt[e]);
+  } else {
+    for_bin_entries(B, slot, n, tid, nthr, walk);
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < AXR_SHADOW_BIN * AXR_SHADOW_BIN; i += blockDim.x) {
     const int x = x0 + (i & (AXR_SHADOW_BIN - 1)), y = y0 + (i >> AXR_SHADOW_SHIFT);
