@@ -438,7 +438,8 @@ __device__ __forceinline__ void span_run_proven(int xa, int xb, int cy, float cx
 // magnitudes in [2^-57, 2^57] within a factor 256 of each other, so the
 // fewer than 128 rows' accumulated rounding keeps cz, and so the span
 // interpolant w, inside [2^-58, 2^58].  Rows with cx1 == cx2 (a half's
-// apex: at most one pixel) still divide in IEEE arithmetic.
+// apex: at most one pixel) take the exact 0 or 1 a division by zero
+// clamps to.
 __device__ __forceinline__ bool small_fast(const Geom<int>& g) {
   const float a = fabsf(g.z1), b = fabsf(g.z2), c = fabsf(g.z3);
   bool ok = imin(g.x1, imin(g.x2, g.x3)) >= 1;
@@ -461,14 +462,20 @@ __device__ __forceinline__ void small_walk(const Geom<int>& g, int xlo, int ylo,
                                            F&& frag, K&& frag_key) {
   const bool fast = small_fast(g);
   const uint32_t flip = g.z1 > 0.0f ? 0x80000000u : 0xffffffffu;
-  // the row's pixels xa.. with (float)ax <= xr and ax < xhi
+  // the row's pixels xa.. with (float)ax <= xr and ax < xhi.  On a fast
+  // triangle's row of zero width (each half's first row among them) the
+  // interpolant clamps to [s, s], s the exact 0 or 1 IEEE's (ax - cx2)/0
+  // clamps to for its one pixel, as span_run_proven does.
   auto row = [&](int xa, float xr, int cy, float cx1, float cx2, float cz1, float cz2, int half) {
     const float d = cx1 - cx2;
-    if (fast && d != 0.0f) {
+    if (fast) {
       const int xb = imin(xhi - 1, f2i(floorf(xr)));
       const float y = rcp_rn(d);
+      const float s = (float)xa - cx2 < 0.0f ? 0.0f : 1.0f;
+      const bool flat = d == 0.0f;
+      const float lo = flat ? s : 0.0f, hi = flat ? s : 1.0f;
       for (int ax = xa; ax <= xb; ax++) {
-        const float t = omax(0.0f, omin(1.0f, div_rn((float)ax - cx2, d, y)));
+        const float t = __builtin_amdgcn_fmed3f(div_rn((float)ax - cx2, d, y), lo, hi);
         frag_key(ax, cy, __float_as_uint(rcp_rn((1 - t) * cz2 + t * cz1)) ^ flip, half);
       }
     } else {
