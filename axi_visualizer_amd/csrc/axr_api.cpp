@@ -364,6 +364,7 @@ bool pass_pending(const axr_ctx* c, int i) {
 AxrDrawConst base_dc(const axr_ctx* c) {
   AxrDrawConst dc{};
   dc.view = c->view;
+  dc.cull = make_cull(dc.view);
   for (int k = 0; k < 3; k++) {
     dc.lint[k] = c->lint[k];
     dc.ldir[k] = c->ldir[k];
@@ -391,6 +392,7 @@ int run_setup(axr_ctx* c, const AxrDrawConst* dc_main, const AxrGroupDev* table,
     const HostShadow& s = c->sm[q.map];
     AxrShadowPass& P = ps.p[k];
     P.cam = q.cam;
+    P.cull = make_cull(P.cam);
     P.dim = s.dim;
     P.nbx = s.nbx;
     P.bias = q.bias;

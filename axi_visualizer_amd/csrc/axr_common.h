@@ -34,6 +34,45 @@ struct AxrCam {
   float scale, cax, cay;
 };
 
+// Constants of the chunk culling tests (chunk_outside_view/ortho), derived
+// from a camera once per draw on the host: the view cone's side planes
+// widened by 1e-4 -- R - kx*F, -R - kx*F, U - ky*F, -U - ky*F with kx =
+// 1.0001*caX, ky = 1.0001*caY (rows F, R, U of vm) -- and their lengths;
+// |F|; |R| and |U| times |scale| (the ortho test's pixel radii); the L1 norm
+// of the eye.  ok = 0, as a zero-initialised block has, culls nothing.
+struct AxrCull {
+  float n[4][3];
+  float nn[4];
+  float nf, nrs, nus, l1eye;
+  int ok;
+};
+inline AxrCull make_cull(const AxrCam& c) {
+  auto row = [&](int r, float (&v)[3]) {
+    for (int k = 0; k < 3; k++) v[k] = c.vm[3 * r + k];
+  };
+  auto len = [](const float (&v)[3]) { return sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); };
+  float F[3], R[3], U[3];
+  row(0, F);
+  row(1, R);
+  row(2, U);
+  const float kx = c.cax * 1.0001f, ky = c.cay * 1.0001f;
+  AxrCull k{};
+  for (int a = 0; a < 3; a++) {
+    k.n[0][a] = R[a] - F[a] * kx;
+    k.n[1][a] = R[a] * -1.0f - F[a] * kx;
+    k.n[2][a] = U[a] - F[a] * ky;
+    k.n[3][a] = U[a] * -1.0f - F[a] * ky;
+  }
+  for (int i = 0; i < 4; i++) k.nn[i] = len(k.n[i]);
+  const float s = fabsf(c.scale);
+  k.nf = len(F);
+  k.nrs = len(R) * s;
+  k.nus = len(U) * s;
+  k.l1eye = fabsf(c.pos[0]) + fabsf(c.pos[1]) + fabsf(c.pos[2]);
+  k.ok = 1;
+  return k;
+}
+
 // Per-group device table; re-uploaded only when a group or its flags change.
 struct AxrGroupDev {
   uint32_t base;   // first global triangle index
@@ -57,6 +96,7 @@ struct AxrShadowDev {
 // Per-draw constants, passed by value to the raster and shade kernels.
 struct AxrDrawConst {
   AxrCam view;
+  AxrCull cull;  // make_cull(view)
   float lint[3];
   float ldir[3];
   float amb;
@@ -92,6 +132,7 @@ struct AxrCastMask {
 // launch and one depth launch over the bins of every pending map.
 struct AxrShadowPass {
   AxrCam cam;
+  AxrCull cull;  // make_cull(cam)
   int dim, nbx;
   float bias;
   uint32_t slot0;  // first bin slot of this map in the shadow bin store

@@ -46,41 +46,44 @@ __device__ __forceinline__ V3 cam_row(const AxrCam& c, int r) {
 // arithmetic -- so a skipped triangle is one the setup would have rejected.
 
 __device__ __forceinline__ float v3len(V3 a) { return sqrtf(dot(a, a)); }
+// The L1 norm, an upper bound of the length: margins use it, so the tests
+// take no square root per chunk.
+__device__ __forceinline__ float l1norm(V3 a) { return fabsf(a.x) + fabsf(a.y) + fabsf(a.z); }
 
 // True when no point of the sphere can pass the perspective accept test.
-__device__ __forceinline__ bool chunk_outside_view(const float4& sph, const AxrCam& cam) {
-  const V3 c = v3(sph.x, sph.y, sph.z), eye = v3(cam.pos[0], cam.pos[1], cam.pos[2]);
+// k: the camera's cull constants (make_cull); a block with ok == 0 skips
+// nothing.
+__device__ __forceinline__ bool chunk_outside_view(const float4& sph, const AxrCam& cam,
+                                                   const AxrCull& k) {
   const float r = sph.w;
-  if (!(r >= 0.0f)) return false;  // no bounds (NaN input): keep
+  if (!k.ok || !(r >= 0.0f)) return false;  // no constants, or no bounds (NaN input): keep
+  const V3 c = v3(sph.x, sph.y, sph.z), eye = v3(cam.pos[0], cam.pos[1], cam.pos[2]);
   const V3 x = sub(c, eye);
-  const float m = 1e-4f * (v3len(c) + v3len(eye) + v3len(x) + 2.0f * r) + 1e-30f;
-  const V3 F = cam_row(cam, 0), R = cam_row(cam, 1), U = cam_row(cam, 2);
-  if (dot(x, F) + r * v3len(F) <= -m) return true;  // behind the eye
-  const float kx = cam.cax * 1.0001f, ky = cam.cay * 1.0001f;
-  const V3 n[4] = {sub(R, scl(F, kx)), sub(scl(R, -1.0f), scl(F, kx)), sub(U, scl(F, ky)),
-                   sub(scl(U, -1.0f), scl(F, ky))};
+  const float m = 1e-4f * (l1norm(c) + k.l1eye + l1norm(x) + 2.0f * r) + 1e-30f;
+  if (dot(x, cam_row(cam, 0)) + r * k.nf <= -m) return true;  // behind the eye
 #pragma unroll
   for (int i = 0; i < 4; i++)
-    if (dot(x, n[i]) - r * v3len(n[i]) >= m) return true;  // beyond a side plane
+    if (dot(x, v3(k.n[i][0], k.n[i][1], k.n[i][2])) - r * k.nn[i] >= m)
+      return true;  // beyond a side plane
   return false;
 }
 
 // True when no point of the sphere can pass the ortho accept test.
-__device__ __forceinline__ bool chunk_outside_ortho(const float4& sph, const AxrCam& cam, int dim,
-                                                   float bias) {
-  const V3 c = v3(sph.x, sph.y, sph.z), eye = v3(cam.pos[0], cam.pos[1], cam.pos[2]);
+__device__ __forceinline__ bool chunk_outside_ortho(const float4& sph, const AxrCam& cam,
+                                                   const AxrCull& k, int dim, float bias) {
   const float r = sph.w;
-  if (!(r >= 0.0f)) return false;
+  if (!k.ok || !(r >= 0.0f)) return false;
+  const V3 c = v3(sph.x, sph.y, sph.z), eye = v3(cam.pos[0], cam.pos[1], cam.pos[2]);
   const V3 x = sub(c, eye);
-  const float mag = v3len(c) + v3len(eye) + v3len(x) + 2.0f * r;
+  const float mag = l1norm(c) + k.l1eye + l1norm(x) + 2.0f * r;
   const V3 F = cam_row(cam, 0), R = cam_row(cam, 1), U = cam_row(cam, 2);
-  if (dot(x, F) + r * v3len(F) + bias <= -(1e-4f * mag + fabsf(bias) * 1e-4f + 1e-30f))
+  if (dot(x, F) + r * k.nf + bias <= -(1e-4f * mag + fabsf(bias) * 1e-4f + 1e-30f))
     return true;  // every point has biased depth <= 0
   const float s = fabsf(cam.scale), half = (float)(dim / 2);
   const float mp = 1.0f + 1e-4f * (mag * s + (float)dim);  // pixels
   // projected coordinate ranges of the sphere: x * scale + half, y * -scale + half
-  const float cxp = dot(x, R) * cam.scale + half, rx = r * v3len(R) * s;
-  const float cyp = dot(x, U) * -cam.scale + half, ry = r * v3len(U) * s;
+  const float cxp = dot(x, R) * cam.scale + half, rx = r * k.nrs;
+  const float cyp = dot(x, U) * -cam.scale + half, ry = r * k.nus;
   const float lo = -121.0f - mp, hi = (float)(dim + 120) + mp;
   if (cxp + rx <= lo || cxp - rx >= hi) return true;
   if (cyp + ry <= lo || cyp - ry >= hi) return true;

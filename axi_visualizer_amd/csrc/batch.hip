@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
       for (uint32_t k = threadIdx.x; k < kWaves * G.z; k += 64) {
         const uint32_t q = k / kWaves;
         const float4 cs = J0.chunks[imin(c0 + (int)(k % kWaves), last_chunk)];
-        jvis[k] = chunk_outside_view(cs, jobs[jidx[G.y + q]].dc.view) ? 0 : 1;
+        jvis[k] = chunk_outside_view(cs, jobs[jidx[G.y + q]].dc.view, jobs[jidx[G.y + q]].dc.cull) ? 0 : 1;
       }
     }
     __syncthreads();
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kSetupBlock) void k_batch_setup(const AxrJobDev* __
     int cls = 0;
     AxrTriRec r;
     const bool in_view = pre ? jvis[q * kWaves + (threadIdx.x >> 6)] != 0
-                             : !chunk_outside_view(sph, dc.view);
+                             : !chunk_outside_view(sph, dc.view, dc.cull);
     if (live && (f & AXR_GRP_DRAW) && in_view) {
       cls = tri_setup(vp, J.vattr, gi, dc.view, dc.W, dc.H, (f & AXR_SH_CULL) != 0, r.px, r.py,
                       r.z);

@@ -162,6 +162,7 @@ AxrJobDev make_job(const axr_batch* b, const axr_job& j, const JobInfo& in, uint
   dc.view.scale = j.scale;
   dc.view.cax = j.cax;
   dc.view.cay = j.cay;
+  dc.cull = make_cull(dc.view);
   for (int k = 0; k < 3; k++) {
     dc.lint[k] = sc->lint[k];
     dc.ldir[k] = sc->ldir[k];

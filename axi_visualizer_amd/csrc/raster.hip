@@ -106,21 +106,30 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
   const bool live = gi < t1;
   // The wave's 64 triangles are one chunk: skip the passes it cannot reach.
   const float4 sph = chunks[imin((int)(gi >> 6), (int)((dc.T - 1) >> 6))];
-  const bool main_on = do_main && !chunk_outside_view(sph, dc.view);
+  const bool main_on = do_main && !chunk_outside_view(sph, dc.view, dc.cull);
   bool any_on = main_on;
   uint32_t pass_on = 0;
 #pragma unroll
   for (int p = 0; p < AXR_MAX_SHADOWS; p++) {
     if (p >= ps.n) break;
-    if (!chunk_outside_ortho(sph, ps.p[p].cam, ps.p[p].dim, ps.p[p].bias)) pass_on |= 1u << p;
+    if (!chunk_outside_ortho(sph, ps.p[p].cam, ps.p[p].cull, ps.p[p].dim, ps.p[p].bias)) pass_on |= 1u << p;
   }
   any_on |= pass_on != 0;
   float4 vp[3];
   int grp = 0;
+  if (any_on) {
+    // The wave's triangles are consecutive: one wave-wide search (scalar
+    // loads) finds the group of its first one, and only lanes past that
+    // group's end search on their own.
+    const int ng = (int)dc.ngroups;
+    const int gw = find_group(groups, ng, __builtin_amdgcn_readfirstlane(gi));
+    const uint32_t gend = gw + 1 < ng ? groups[gw + 1].base : 0xFFFFFFFFu;
+    grp = gw;
+    if (live && gi >= gend) grp = find_group(groups, ng, gi);
+  }
   if (live && any_on) {
 #pragma unroll
     for (int k = 0; k < 3; k++) vp[k] = vpos[3 * (size_t)gi + k];
-    grp = find_group(groups, (int)dc.ngroups, gi);
   } else {
 #pragma unroll
     for (int k = 0; k < 3; k++) vp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
