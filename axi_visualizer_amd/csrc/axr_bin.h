@@ -10,6 +10,38 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
+// wave_bin_add in two steps, so that several appends' atomics can be in
+// flight at once: wave_bin_add_issue groups the lanes and sends one atomic
+// per group; finish() waits for it and gives each active lane its position.
+struct WaveAdd {
+  uint32_t base;   // the atomic's result, on the first lane of each group
+  int leader_of;   // first lane of this lane's group
+  uint32_t rank;   // rank in the group
+  bool active;
+  __device__ __forceinline__ uint32_t finish() const {
+    const uint32_t b = __shfl(base, leader_of);
+    return active ? b + rank : 0u;
+  }
+};
+__device__ __forceinline__ WaveAdd wave_bin_add_issue(uint32_t* __restrict__ ctr, bool active,
+                                                      uint32_t bin) {
+  uint64_t pending = __ballot(active);
+  WaveAdd w{0u, 0, 0u, active};
+  uint32_t cnt = 0;  // the group's size on its first lane
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t b = __shfl(bin, leader);
+    const uint64_t grp = __ballot(active && bin == b) & pending;
+    if ((grp >> __lane_id()) & 1ull) {
+      w.leader_of = leader;
+      w.rank = (uint32_t)__popcll(grp & lanemask_lt());
+    }
+    if ((int)__lane_id() == leader) cnt = (uint32_t)__popcll(grp);
+    pending &= ~grp;
+  }
+  if (cnt) w.base = atomicAdd(&ctr[bin], cnt);
+  return w;
+}
 // Adds one per active lane to ctr[bin] with one atomic per distinct bin in
 // the wave; returns the counter's previous value plus the lane's rank among
 // the lanes that share its bin.  All lanes of the wave must call it.  The
@@ -18,24 +50,7 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 // one round trip however many bins it touches.
 __device__ __forceinline__ uint32_t wave_bin_add(uint32_t* __restrict__ ctr, bool active,
                                                  uint32_t bin) {
-  uint64_t pending = __ballot(active);
-  int leader_of = 0;           // first lane of this lane's group
-  uint32_t rank = 0, cnt = 0;  // rank in the group; the group's size on its first lane
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t b = __shfl(bin, leader);
-    const uint64_t grp = __ballot(active && bin == b) & pending;
-    if ((grp >> __lane_id()) & 1ull) {
-      leader_of = leader;
-      rank = (uint32_t)__popcll(grp & lanemask_lt());
-    }
-    if ((int)__lane_id() == leader) cnt = (uint32_t)__popcll(grp);
-    pending &= ~grp;
-  }
-  uint32_t base = 0;
-  if (cnt) base = atomicAdd(&ctr[bin], cnt);
-  base = __shfl(base, leader_of);
-  return active ? base + rank : 0u;
+  return wave_bin_add_issue(ctr, active, bin).finish();
 }
 
 // Bin rectangle (bins of 1 << shift pixels) of a pixel rectangle clipped to

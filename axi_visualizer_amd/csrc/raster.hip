@@ -134,6 +134,10 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
 #pragma unroll
     for (int k = 0; k < 3; k++) vp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
+  // The screen setup and the first two shadow passes' setups run first;
+  // their bin appends then share one loop, so that a wave's three streams
+  // of atomics are in flight together instead of one round trip each.
+  BinRect brm = {0, 0, 1, 0};
   if (do_main) {
     int cls = 0;
     AxrTriRec r;
@@ -146,29 +150,18 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
       cls_out[gi] = (uint8_t)cls;
       if (cls) rec[gi] = r;
     }
-    BinRect br = {0, 0, 1, 0};
     if (cls == 1) {
       int x0, y0, x1, y1;
       small_bounds(r.px, r.py, x0, y0, x1, y1);
-      br = bin_rect(x0, y0, x1, y1, dc.W, dc.H, AXR_BIN_SHIFT);
-    }
-    BinCursor bc;
-    for (int k = 0;; k++) {
-      const bool act = k < br.n;
-      if (__ballot(act) == 0) break;
-      const uint32_t slot = act ? bc.bin(br, NBX) : 0u;
-      const uint32_t pos = wave_bin_add(MB.cnt, act, slot);
-      if (act) bin_store(MB, slot, pos, gi);
-      bc.next(br);
+      brm = bin_rect(x0, y0, x1, y1, dc.W, dc.H, AXR_BIN_SHIFT);
     }
     // large triangles are binned by k_coarse, a block per triangle
     const uint32_t at = wave_bin_add(large_ctl, cls == 2, 0u);
     if (cls == 2) large[at] = gi;
   }
-#pragma unroll
-  for (int p = 0; p < AXR_MAX_SHADOWS; p++) {
-    if (p >= ps.n) break;
-    const AxrShadowPass& P = ps.p[p];
+  // shadow pass p: ortho setup, its record, and the bin rectangle and entry
+  // of its caster
+  auto shadow_pass = [&](const AxrShadowPass& P, int p, BinRect& br, uint32_t& entry) {
     bool ok = false;
     AxrShRec r;
     if (live) {
@@ -177,14 +170,47 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float4* __restrict__ vpo
       P.ok[gi] = ok ? 1 : 0;
       if (ok) P.rec[gi] = r;
     }
-    BinRect br = {0, 0, 1, 0};
-    uint32_t entry = gi;
+    br = BinRect{0, 0, 1, 0};
+    entry = gi;
     if (ok) {
       int x0, y0, x1, y1;
       shadow_bounds(r, x0, y0, x1, y1);
       br = bin_rect(x0, y0, x1, y1, P.dim, P.dim, AXR_SHADOW_SHIFT);
       entry |= shadow_class(r) << 30;
     }
+  };
+  BinRect br0 = {0, 0, 1, 0}, br1 = {0, 0, 1, 0};
+  uint32_t e0 = gi, e1 = gi;
+  if (ps.n > 0) shadow_pass(ps.p[0], 0, br0, e0);
+  if (ps.n > 1) shadow_pass(ps.p[1], 1, br1, e1);
+  {
+    BinCursor cm, c0, c1;
+    for (int k = 0;; k++) {
+      const bool am = k < brm.n, a0 = k < br0.n, a1 = k < br1.n;
+      if (__ballot(am || a0 || a1) == 0) break;
+      const uint32_t sm = am ? cm.bin(brm, NBX) : 0u;
+      const uint32_t s0 = a0 ? ps.p[0].slot0 + c0.bin(br0, ps.p[0].nbx) : 0u;
+      const uint32_t s1 = a1 ? ps.p[1].slot0 + c1.bin(br1, ps.p[1].nbx) : 0u;
+      const WaveAdd wm = wave_bin_add_issue(MB.cnt, am, sm);
+      const WaveAdd w0 = wave_bin_add_issue(SB.cnt, a0, s0);
+      const WaveAdd w1 = wave_bin_add_issue(SB.cnt, a1, s1);
+      const uint32_t qm = wm.finish(), q0 = w0.finish(), q1 = w1.finish();
+      if (am) bin_store(MB, sm, qm, gi);
+      if (a0) bin_store(SB, s0, q0, e0);
+      if (a1) bin_store(SB, s1, q1, e1);
+      cm.next(brm);
+      c0.next(br0);
+      c1.next(br1);
+    }
+  }
+  // further pending passes (more than two maps rendered before a draw)
+#pragma unroll
+  for (int p = 2; p < AXR_MAX_SHADOWS; p++) {
+    if (p >= ps.n) break;
+    const AxrShadowPass& P = ps.p[p];
+    BinRect br;
+    uint32_t entry;
+    shadow_pass(P, p, br, entry);
     BinCursor bc;
     for (int k = 0;; k++) {
       const bool act = k < br.n;
