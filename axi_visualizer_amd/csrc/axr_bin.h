@@ -130,6 +130,8 @@ struct SpanRec {
   float cx1, cx2, cz1, cz2;
   uint32_t seq;
   int cy, xa;
+  float y;  // RN(1/(cx1 - cx2)) for a small_fast triangle's span, else 0
+  int fast;  // the span's pixels take the fast exact path (small_fast)
 };
 // Triangles whose box inside the bin has more pixels than this walk
 // spread over the wave; smaller ones walk on their own lane (48, 64 and 96
@@ -145,10 +147,13 @@ struct WaveSpans {
 // One step of a cooperative walk: every lane offers one span (len may be
 // 0); the wave walks all their pixels 64 at a time -- prefix sum of the
 // lengths, binary search for each pixel's span -- calling frag(ax, cy, tz,
-// seq) with the span's own arithmetic (span_t, span_tz).  All lanes of the
-// wave must call it.
-template <class F>
-__device__ __forceinline__ void coop_spans(WaveSpans& ws, int len, const SpanRec& sr, F&& frag) {
+// seq) with the span's own arithmetic (span_t, span_tz), or, for a span of
+// a small_fast triangle, frag_key(ax, cy, ordf(tz), seq) with the walk's
+// fast exact arithmetic (small_walk's row: div_rn/rcp_rn, a zero-width row
+// clamped to the exact 0 or 1).  All lanes of the wave must call it.
+template <class F, class K>
+__device__ __forceinline__ void coop_spans(WaveSpans& ws, int len, const SpanRec& sr, F&& frag,
+                                           K&& frag_key) {
   const int lane = (int)__lane_id();
   uint32_t incl = (uint32_t)len;
 #pragma unroll
@@ -177,8 +182,18 @@ __device__ __forceinline__ void coop_spans(WaveSpans& ws, int len, const SpanRec
       const uint32_t start = lo ? ws.end[lo - 1] : 0u;
       const SpanRec r = ws.r[lo];
       const int ax = r.xa + (int)(i - start);
-      const float t = span_t(ax, r.cx1, r.cx2);
-      frag(ax, r.cy, span_tz(t, r.cz1, r.cz2), r.seq);
+      if (r.fast) {
+        const float d = r.cx1 - r.cx2, u = (float)ax - r.cx2;
+        const float sv = u < 0.0f ? 0.0f : 1.0f;
+        const bool flat = d == 0.0f;
+        const float t = __builtin_amdgcn_fmed3f(div_rn(u, d, r.y), flat ? sv : 0.0f,
+                                                flat ? sv : 1.0f);
+        const uint32_t flip = r.cz1 > 0.0f ? 0x80000000u : 0xffffffffu;
+        frag_key(ax, r.cy, __float_as_uint(rcp_rn((1 - t) * r.cz2 + t * r.cz1)) ^ flip, r.seq);
+      } else {
+        const float t = span_t(ax, r.cx1, r.cx2);
+        frag(ax, r.cy, span_tz(t, r.cz1, r.cz2), r.seq);
+      }
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -194,11 +209,12 @@ __device__ __forceinline__ void coop_spans(WaveSpans& ws, int len, const SpanRec
 // lengths, binary search for each pixel's span).  frag(ax, cy, tz, seq)
 // sees the same fragments as small_walk; only their order differs.  All
 // lanes of the wave must call it; `valid` marks lanes with a triangle.
-template <class F>
+template <class F, class K>
 __device__ __forceinline__ void coop_small_walk(WaveSpans& ws, bool valid, const Geom<int>& g,
                                                 uint32_t order, int xlo, int ylo, int xhi,
-                                                int yhi, F&& frag) {
+                                                int yhi, F&& frag, K&& frag_key) {
   int half = valid ? 0 : 2;
+  const int fast = valid && small_fast(g) ? 1 : 0;
   Edges e;
   float cx1 = 0.f, cx2 = 0.f, cz1 = 0.f, cz2 = 0.f;
   int cy = 0;
@@ -223,7 +239,8 @@ __device__ __forceinline__ void coop_small_walk(WaveSpans& ws, bool valid, const
         const int xa = imax(f2i(cx2), xlo);
         if (cx1 >= (float)xa) len = imin(xhi - 1, f2i(floorf(cx1))) - xa + 1;
         len = imax(len, 0);
-        sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 0), cy, xa};
+        sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 0), cy, xa,
+                     fast ? rcp_rn(cx1 - cx2) : 0.0f, fast};
         cx1 += e.s1;
         cx2 += e.s2;
         cz1 += e.sz1;
@@ -248,7 +265,8 @@ __device__ __forceinline__ void coop_small_walk(WaveSpans& ws, bool valid, const
         const int xa = imax(f2i(cx1), xlo);
         if (cx2 >= (float)xa) len = imin(xhi - 1, f2i(floorf(cx2))) - xa + 1;
         len = imax(len, 0);
-        sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 1), cy, xa};
+        sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 1), cy, xa,
+                     fast ? rcp_rn(cx1 - cx2) : 0.0f, fast};
         cx1 -= e.s1;
         cx2 -= e.s2;
         cz1 -= e.sz1;
@@ -258,7 +276,7 @@ __device__ __forceinline__ void coop_small_walk(WaveSpans& ws, bool valid, const
         half = 2;
       }
     }
-    coop_spans(ws, len, sr, frag);
+    coop_spans(ws, len, sr, frag, frag_key);
   }
 }
 
@@ -303,7 +321,7 @@ __device__ __forceinline__ void coop_tile_walk(WaveSpans& ws, bool valid, const 
           const float xe = oclamp(cx1, b.xMin, b.xMax);
           const int xa = f2i(oclamp(cx2, b.xMin, b.xMax));
           if ((float)xa <= xe) len = f2i(floorf(xe)) - xa + 1;
-          sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 0), cy, xa};
+          sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 0), cy, xa, 0.0f, 0};
         }
         cx1 += e.s1;
         cx2 += e.s2;
@@ -322,7 +340,7 @@ __device__ __forceinline__ void coop_tile_walk(WaveSpans& ws, bool valid, const 
           const float xe = oclamp(cx2, b.xMin, b.xMax);
           const int xa = f2i(oclamp(cx1, b.xMin, b.xMax));
           if ((float)xa <= xe) len = f2i(floorf(xe)) - xa + 1;
-          sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 1), cy, xa};
+          sr = SpanRec{cx1, cx2, cz1, cz2, frag_seq(order, 1), cy, xa, 0.0f, 0};
         }
         cx1 -= e.s1;
         cx2 -= e.s2;
@@ -333,7 +351,7 @@ __device__ __forceinline__ void coop_tile_walk(WaveSpans& ws, bool valid, const 
         half = 2;
       }
     }
-    coop_spans(ws, len, sr, frag);
+    coop_spans(ws, len, sr, frag, frag);
   }
 }
 
@@ -409,7 +427,7 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
               put_key(ax, cy, ord, frag_seq(gi, half));
             });
     }
-    coop_small_walk(ws, valid && big, g, gi, x0, y0, xe, ye, put);
+    coop_small_walk(ws, valid && big, g, gi, x0, y0, xe, ye, put, put_key);
   });
   auto put_tile = [&](int ax, int cy, float tz, uint32_t seq) {
     if (!(tz <= AXR_DEPTH_CLEAR)) return;
