@@ -234,8 +234,9 @@ int bin_raster(axr_ctx* c, int mode, float exposure) {
   AxrBins B = c->mbins.B;
   B.nb = c->NB;
   // Bins share blocks only when there are more of them than the device holds
-  // blocks at once (two per CU): a small frame gives every bin a whole block.
-  const bool share = c->NB > 2 * c->cu_count;
+  // blocks at once (two per CU); a frame of fewer bins is rastered a 16-px
+  // quadrant per block (no tiers).
+  const bool share = c->NB > AXR_QUAD_BINS_PER_CU * c->cu_count;
   // winner ids leave the depth buffer as it is: it may hold a later pass's
   // depths (SSR water) that the ids do not describe
   HIPCHK(launch_bin_raster(mode, c->d_rec, B, empty, c->d_vpos, c->d_vattr, c->d_groups, dc,

@@ -370,11 +370,14 @@ __device__ __forceinline__ bool behind_bin(const float z[3], float zfar) {
 // on their reference tile (a9).  FIRST: the key's low word is seq, so the
 // first fragment in draw order wins a depth tie (a strict test); otherwise
 // ~seq, the last one wins (the reference's F >= tz).
+// quad >= 0 (16-px bins only): the slots are those of the 32-px bin
+// holding this one as its quadrant quad, and only the large-triangle tile
+// entries of that quadrant are walked; small triangles clip to the bin.
 template <int SHIFT, bool FIRST, bool DEAL = false>
 __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec* __restrict__ rec,
                                          const AxrBins& B, uint32_t s_small, uint32_t s_large,
                                          int bx, int by, const AxrDrawConst& dc, int tid,
-                                         int nthr, float zfar = 0.0f) {
+                                         int nthr, float zfar = 0.0f, int quad = -1) {
   constexpr int BIN = 1 << SHIFT, NPX = BIN * BIN;
   const int x0 = bx << SHIFT, y0 = by << SHIFT;
   const int xe = imin(x0 + BIN, dc.W), ye = imin(y0 + BIN, dc.H);
@@ -447,6 +450,7 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
     uint32_t order = 0;
     bool big = false;
     AxrTriRec r;
+    if (quad >= 0 && (int)(v & 3u) != quad) valid = false;  // another quadrant's tile
     if (valid) {
       r = rec[v >> 2];
       if (FIRST && behind_bin(r.z, zfar)) valid = false;  // cannot pass the strict test
@@ -477,6 +481,7 @@ __device__ __forceinline__ void walk_bin(unsigned long long* lk, const AxrTriRec
 // (orderable tz << 32) | ~seq.  Covered pixels are then compacted and
 // shaded once from their winning key; empty ones get the clear colour.
 // `empty` draws only the background.  lk, list, ncov are the bin's LDS;
+// quad >= 0: a 16-px quadrant of a 32-px bin's slots (walk_bin);
 // threads tid of nthr work on it (a block may hold two bins, each with half
 // of its threads: every group passes the same barriers, an inactive group
 // with no work).
@@ -488,7 +493,7 @@ __device__ __forceinline__ void raster_bin(
     const AxrGroupDev* __restrict__ groups, const AxrDrawConst& dc, float exposure,
     float* __restrict__ depth, uint16_t* __restrict__ ro, uint16_t* __restrict__ go,
     uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba, uint32_t* __restrict__ ids,
-    int tid, int nthr, bool active = true) {
+    int tid, int nthr, bool active = true, int quad = -1) {
   constexpr int BIN = 1 << SHIFT, NPX = BIN * BIN;
   if (!active) {  // a thread group without a bin: the same barriers, no work
     __syncthreads();
@@ -509,7 +514,8 @@ __device__ __forceinline__ void raster_bin(
   }
   if (tid == 0) *ncov = 0;
   __syncthreads();
-  if (!empty) walk_bin<SHIFT, false, DEAL>(lk, rec, B, s_small, s_large, bx, by, dc, tid, nthr);
+  if (!empty)
+    walk_bin<SHIFT, false, DEAL>(lk, rec, B, s_small, s_large, bx, by, dc, tid, nthr, 0.0f, quad);
   __syncthreads();
   // nthr is a multiple of the wave size, so every lane of a wave runs the
   // same rounds and the covered pixels are appended with one LDS atomic a

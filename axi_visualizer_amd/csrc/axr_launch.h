@@ -40,6 +40,11 @@ hipError_t launch_chunk_bounds(const float4* vpos, uint32_t T, float4* chunks, h
 #ifndef AXR_MEDIUM_WEIGHT
 #define AXR_MEDIUM_WEIGHT 256
 #endif
+// Frames of at most this many 32-px bins per CU are rastered a 16-px
+// quadrant per block; larger ones a bin (or two, or four) per block.
+#ifndef AXR_QUAD_BINS_PER_CU
+#define AXR_QUAD_BINS_PER_CU 2
+#endif
 constexpr uint32_t kHeavyWeight = AXR_HEAVY_WEIGHT;
 constexpr uint32_t kMediumWeight = AXR_MEDIUM_WEIGHT;
 hipError_t launch_bin_order(const AxrBins* main_bins, const AxrBins* shadow_bins,

@@ -76,6 +76,33 @@ __global__ __launch_bounds__(AXR_RASTER_THREADS) void k_bin_raster(
                                   active);
 }
 
+// A frame of few bins (no more than two per CU) is rastered a 16-px
+// quadrant per block: four blocks share each 32-px bin, each walking the
+// bin's small triangles clipped to its quadrant and the large-triangle tiles
+// of its quadrant, then shading its 256 pixels.  The pixels and their
+// winners are those of the whole-bin raster; four times the blocks run.
+constexpr int kQuadThreads = 256;
+template <int MODE>
+__global__ __launch_bounds__(kQuadThreads) void k_bin_raster_quads(
+    const AxrTriRec* __restrict__ rec, AxrBins B, int empty, const float4* __restrict__ vpos,
+    const float4* __restrict__ vattr, const AxrGroupDev* __restrict__ groups, AxrDrawConst dc,
+    int NBX, float exposure, float* __restrict__ depth, uint16_t* __restrict__ ro,
+    uint16_t* __restrict__ go, uint16_t* __restrict__ bo, uint32_t* __restrict__ rgba,
+    uint32_t* __restrict__ ids) {
+  constexpr int NPX = (AXR_BIN / 2) * (AXR_BIN / 2);
+  __shared__ unsigned long long lk[NPX];
+  __shared__ uint16_t list[NPX];
+  __shared__ uint32_t ncov;
+  const int q = (int)(blockIdx.x & 3u);
+  const uint32_t j = blockIdx.x >> 2;
+  const int b = empty ? (int)j : (int)B.order[j];  // heavy bins first
+  const int bx = 2 * (b % NBX) + (q & 1), by = 2 * (b / NBX) + (q >> 1);
+  raster_bin<AXR_BIN_SHIFT - 1, MODE, true>(lk, list, &ncov, rec, B, (uint32_t)b,
+                                            (uint32_t)(B.nb + b), empty, bx, by, vpos, vattr,
+                                            groups, dc, exposure, depth, ro, go, bo, rgba, ids,
+                                            (int)threadIdx.x, kQuadThreads, true, q);
+}
+
 __global__ void k_gamma_planes(uint16_t* __restrict__ r, uint16_t* __restrict__ g,
                                uint16_t* __restrict__ b, size_t n, float ex) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -206,9 +233,18 @@ hipError_t launch_bin_raster(int mode, const AxrTriRec* rec, const AxrBins& B, i
                              uint16_t* r, uint16_t* g, uint16_t* b, uint32_t* rgba,
                              uint32_t* ids, const uint32_t* tiers, hipStream_t s) {
   if (B.nb == 0) return hipSuccess;
+  // no tiers: a frame of few bins, rastered a quadrant per block
 #define AXR_LAUNCH_BIN(M)                                                                     \
-  hipLaunchKernelGGL(k_bin_raster<M>, dim3(B.nb), dim3(AXR_RASTER_THREADS), 0, s, rec, B, empty, \
-                     vpos, vattr, groups, dc, NBX, exposure, depth, r, g, b, rgba, ids, tiers)
+  do {                                                                                        \
+    if (tiers)                                                                                \
+      hipLaunchKernelGGL(k_bin_raster<M>, dim3(B.nb), dim3(AXR_RASTER_THREADS), 0, s, rec, B,  \
+                         empty, vpos, vattr, groups, dc, NBX, exposure, depth, r, g, b, rgba,   \
+                         ids, tiers);                                                         \
+    else                                                                                      \
+      hipLaunchKernelGGL(k_bin_raster_quads<M>, dim3(4 * B.nb), dim3(kQuadThreads), 0, s, rec,  \
+                         B, empty, vpos, vattr, groups, dc, NBX, exposure, depth, r, g, b,      \
+                         rgba, ids);                                                          \
+  } while (0)
   if (mode == SHADE_LDR8)
     AXR_LAUNCH_BIN(SHADE_LDR8);
   else if (mode == SHADE_HDR)
