@@ -13,11 +13,11 @@ namespace axr {
 struct V3 {
   float x, y, z;
 };
-__device__ __forceinline__ V3 v3(float x, float y, float z) { return {x, y, z}; }
-__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+AXR_HD V3 v3(float x, float y, float z) { return {x, y, z}; }
+AXR_HD V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ V3 scl(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+AXR_HD float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 __device__ __forceinline__ V3 normalize(V3 v) { return scl(v, 1.0f / sqrtf(dot(v, v))); }
 __device__ __forceinline__ float pown(float x, int n) {
   float r = 1.0f, b = x;
@@ -29,7 +29,7 @@ __device__ __forceinline__ float pown(float x, int n) {
   }
   return r;
 }
-__device__ __forceinline__ V3 cam_row(const AxrCam& c, int r) {
+AXR_HD V3 cam_row(const AxrCam& c, int r) {
   return {c.vm[3 * r], c.vm[3 * r + 1], c.vm[3 * r + 2]};
 }
 
@@ -48,13 +48,12 @@ __device__ __forceinline__ V3 cam_row(const AxrCam& c, int r) {
 __device__ __forceinline__ float v3len(V3 a) { return sqrtf(dot(a, a)); }
 // The L1 norm, an upper bound of the length: margins use it, so the tests
 // take no square root per chunk.
-__device__ __forceinline__ float l1norm(V3 a) { return fabsf(a.x) + fabsf(a.y) + fabsf(a.z); }
+AXR_HD float l1norm(V3 a) { return fabsf(a.x) + fabsf(a.y) + fabsf(a.z); }
 
 // True when no point of the sphere can pass the perspective accept test.
 // k: the camera's cull constants (make_cull); a block with ok == 0 skips
 // nothing.
-__device__ __forceinline__ bool chunk_outside_view(const float4& sph, const AxrCam& cam,
-                                                   const AxrCull& k) {
+AXR_HD bool chunk_outside_view(const float4& sph, const AxrCam& cam, const AxrCull& k) {
   const float r = sph.w;
   if (!k.ok || !(r >= 0.0f)) return false;  // no constants, or no bounds (NaN input): keep
   const V3 c = v3(sph.x, sph.y, sph.z), eye = v3(cam.pos[0], cam.pos[1], cam.pos[2]);
@@ -69,8 +68,8 @@ __device__ __forceinline__ bool chunk_outside_view(const float4& sph, const AxrC
 }
 
 // True when no point of the sphere can pass the ortho accept test.
-__device__ __forceinline__ bool chunk_outside_ortho(const float4& sph, const AxrCam& cam,
-                                                   const AxrCull& k, int dim, float bias) {
+AXR_HD bool chunk_outside_ortho(const float4& sph, const AxrCam& cam, const AxrCull& k,
+                                int dim, float bias) {
   const float r = sph.w;
   if (!k.ok || !(r >= 0.0f)) return false;
   const V3 c = v3(sph.x, sph.y, sph.z), eye = v3(cam.pos[0], cam.pos[1], cam.pos[2]);
